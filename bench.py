@@ -1,0 +1,154 @@
+"""Benchmark: particle-substeps/s of the gfx950 particle solver substep.
+
+A step is one substep (grid build + sort + 4 solver iterations + post passes)
+of the plain solver over a synthetic scene resident in HBM.  Prints one JSON
+line.  The default workload is the 282,624-particle mixed-phase scene (fluid +
+64 shape-matched boxes + 128x128 cloth under wind); the same line also carries
+the 4,194,304-particle large block, the HBM-roofline case.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "brass-pan_amd"))
+
+import numpy as np  # noqa: E402
+
+import brasspan as bp  # noqa: E402
+import scenes  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
+
+# Algorithmic bytes per particle of each kernel family (each array read or
+# written once per pass; FP32 vec3 = 12 B, scalars 4 B; cell-range table reads
+# 8 B per occupied cell, `cpn` = occupied cells / particles).
+ALG_BYTES = {
+    "predict_hash": lambda cpn: 12 + 12 + 4 + 8 * cpn,           # R x, v; W key; clear old ranges
+    "radix_sort": lambda cpn: 16,                                # R+W key, value (single-pass bound)
+    "cell_ranges": lambda cpn: 4 + 8 * cpn,                      # R key; W start/end
+    "reorder": lambda cpn: 104,                                  # R perm + 6 fields; W 6 fields + inverse map
+    "fluid_lambda": lambda cpn: 28 + 8 * cpn,                    # R x*, m, phase; W lambda, rho
+    "fluid_position": lambda cpn: 36 + 8 * cpn,                  # R x*, lambda, m, phase; W x*
+    "particle_particle": lambda cpn: 44 + 8 * cpn,               # R x*, x, w, phase; W x*
+    "omega_normal": lambda cpn: 56 + 8 * cpn,                    # R x, v, rho, phase; W omega, n
+    "vorticity_tension": lambda cpn: 68 + 8 * cpn,               # R x, omega, v, n, rho, phase; W v
+    "xsph": lambda cpn: 40 + 8 * cpn,                            # R x, v, phase; W v
+    "finish": lambda cpn: 48,                                    # R x*, x; W v, x
+    "plane_collide": lambda cpn: 36,                             # R x*, x; W x*
+}
+
+
+def build_workload(name):
+    builder, pbuilder, dt, substeps = scenes.WORKLOADS[name]
+    sc = builder()
+    return sc, pbuilder(), dt / substeps
+
+
+def time_plain(sc, prm, subdt, steps, warmup):
+    s = bp.Solver(sc, prm)
+    for _ in range(warmup):
+        s.step(1, subdt)
+    s.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step(1, subdt)
+    s.sync()
+    wall = time.perf_counter() - t0
+    return s, wall
+
+
+def kernel_roofline(s, subdt, substeps=5):
+    """Per-family HIP-event timing on the solver's stream over `substeps`."""
+    s.enable_kernel_timing(True)
+    s.timed_substeps(substeps, subdt)
+    times = s.kernel_times()
+    s.enable_kernel_timing(False)
+    keys = s.debug(2)
+    cpn = float(len(np.unique(keys))) / float(len(keys))
+    fam = max(times, key=lambda k: times[k][0])
+    total_ms, launches = times[fam]
+    avg_ms = total_ms / launches
+    per = ALG_BYTES.get(fam, lambda c: float("nan"))(cpn)
+    alg_bytes = per * s.n
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    return {
+        "kernel": fam, "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+        "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "avg_launch_ms": round(avg_ms, 4),
+        "alg_bytes_per_particle": round(per, 2), "occupied_cells_per_particle": round(cpn, 4),
+        "launches": launches,
+    }, {k: [round(v[0], 3), v[1]] for k, v in sorted(times.items(), key=lambda kv: -kv[1][0])}
+
+
+def cpu_baseline(sc, prm, subdt, budget_s=20.0):
+    """Single-threaded FP32 oracle (reference order) on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as orc
+
+    o = orc.Oracle(bp.SceneArrays(sc), prm, fp64=False, mode=orc.SCATTER)
+    o.step(1, subdt)  # warm-up substep (untimed)
+    done = 0
+    t0 = time.perf_counter()
+    while True:
+        o.step(1, subdt)
+        done += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or done >= 50:
+            break
+    return {"value": round(sc.num_particles() * done / el, 1), "unit": "particle-substeps/s", "cores": 1,
+            "kind": "port", "sample": f"{done} substeps after 1 warm-up substep, {el:.1f} s, single-threaded "
+                                      f"FP32 oracle in reference order"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="mixed_phase_283k", choices=sorted(scenes.WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-large", action="store_true", help="skip the 4M large-block measurement")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the solver shards by spatial slab only in design, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    sc, prm, subdt = build_workload(args.workload)
+    s, wall = time_plain(sc, prm, subdt, args.steps, args.warmup)
+    n = s.n
+    value = n * args.steps / wall
+    roof, fam_times = kernel_roofline(s, subdt)
+    s.close()
+    out = {
+        "metric": "particle-substeps/s (4 iters)", "value": round(value, 1), "unit": "particle-substeps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded splitmix64 scene, no checkpoint)",
+        "config": {"workload": args.workload, "particles": n, "iterations": prm.iterations,
+                   "substep_dt": subdt, "distance_constraints": len(sc.distance_pairs),
+                   "wind_faces": len(sc.faces), "rigid_bodies": len(sc.rigidbody_particle_id_ranges),
+                   "parallelism": "single GPU"},
+        "roofline": roof,
+        "kernel_ms": fam_times,
+    }
+    if not args.no_large and args.workload != "large_block_4m":
+        lsc, lprm, lsub = build_workload("large_block_4m")
+        ls, lwall = time_plain(lsc, lprm, lsub, 20, 5)
+        lroof, ltimes = kernel_roofline(ls, lsub)
+        out["large_block_4m"] = {"value": round(ls.n * 20 / lwall, 1), "ms_per_step": round(lwall / 20 * 1e3, 4),
+                                 "steps": 20, "warmup": 5, "roofline": lroof, "kernel_ms": ltimes}
+        ls.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(sc, prm, subdt)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,394 @@
+// bp_constraints.hip -- boundary, solid and cloth passes on gfx950: plane
+// collisions, particle-particle contact, distance and dihedral bending
+// constraints, aerodynamic wind, rigid shape matching and pins.
+// Constraint groups keep the reference's Jacobi semantics (Δx from the same
+// x*, then x* += Δx, particlesolver.cuh:887-969) in gather form: a compute
+// kernel per constraint writes its corrections, then one thread per affected
+// particle sums them in constraint-index order (CSR built on the host) and
+// applies them.  No floating-point atomics.
+#include "bp_kernels.h"
+
+namespace bp {
+
+// particlePlaneCollisionConstraint over all planes (particleplane.cuh:21-58,
+// particlesolver.cuh:872-880), in place.
+__global__ __launch_bounds__(kBlock) void k_plane_collide(float4 *__restrict__ xs, const float4 *__restrict__ x,
+                                                          int n, Planes pl, float r, float fs, float fd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = xs[i];
+    const f3 out = plane_collide_one(xyz(a), xyz(x[i]), pl, r, fs, fd);
+    xs[i] = f4(out, a.w);
+}
+
+// particleParticleCollisionConstraint (particleparticle.cuh:5-101) + Δx apply.
+__global__ __launch_bounds__(kBlock) void k_particle_particle(const float4 *__restrict__ xs,
+                                                              const float4 *__restrict__ x,
+                                                              const int2 *__restrict__ ranges,
+                                                              float4 *__restrict__ xs_out, int n, GridDesc G,
+                                                              float r, float fs, float fd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = xs[i];
+    const float4 xa = x[i];
+    const f3 xi = xyz(xa), xip = xyz(a);
+    const float wi = xa.w;
+    const int pi = phase_of(a);
+    f3 sumD = mk3(0.f, 0.f, 0.f), sumF = mk3(0.f, 0.f, 0.f);
+    int cnt = 0;
+    int g[3];
+    G.pos(xip, g);
+    const float r2x4 = r * r * 4.0f;
+    const float thr = r * r * 0.001f * 0.001f;
+    for_each_run(G, ranges, g, [&](int s, int e) {
+        for (int j = s; j < e; ++j) {
+            const float4 b = xs[j];
+            const int pj = phase_of(b);
+            if (i != j && pi != pj && (pi > 0 || pj > 0)) {
+                const f3 xjp = xyz(b);
+                const f3 diff = xip - xjp;
+                const float d2 = length2(diff);
+                if (d2 < r2x4 && d2 > 1e-5f) {
+                    const float dist = sqrtf(d2);
+                    const float4 xb = x[j];
+                    const float wj = xb.w;
+                    const float w1 = wi / (wi + wj);
+                    const float w2 = wj / (wi + wj);
+                    const f3 proj = diff * (2.0f * r / dist - 1.0f);
+                    const f3 dxi = w1 * proj;
+                    const f3 xistar = dxi + xip;
+                    sumD += dxi;
+                    const f3 dxj = -w2 * proj;
+                    const float l2 = length2(dxi);
+                    if (l2 > thr) {
+                        cnt += 1;
+                        const f3 xjstar = dxj + xjp;
+                        const f3 term1 = (xistar - xi) - (xjstar - xyz(xb));
+                        const f3 nn = diff / dist;
+                        const f3 tan = term1 - dot(term1, nn) * nn;
+                        const float tl2 = length2(tan);
+                        if (tl2 <= (fs * fs) * l2)
+                            sumF -= w1 * tan;
+                        else
+                            sumF -= w1 * tan * fminf(fd * sqrtf(l2 / tl2), 1.0f);
+                    }
+                }
+            }
+        }
+    });
+    const f3 d = (cnt == 0) ? sumD : sumD + sumF / (float)cnt;
+    xs_out[i] = f4(xip + d, a.w);
+}
+
+// distanceConstraints (distance.cuh:3-41): k of each pair
+__global__ __launch_bounds__(kBlock) void k_distance_compute(const float4 *__restrict__ xs,
+                                                             const float4 *__restrict__ x,
+                                                             const int2 *__restrict__ pairs,
+                                                             const float2 *__restrict__ params,
+                                                             const int *__restrict__ map_to_new,
+                                                             float4 *__restrict__ kbuf, int nc) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int2 pr = pairs[c];
+    const float2 pa = params[c];
+    const int id1 = map_to_new[pr.x], id2 = map_to_new[pr.y];
+    const f3 x12 = xyz(xs[id1]) - xyz(xs[id2]);
+    const float d = length(x12);
+    f3 k = mk3(0.f, 0.f, 0.f);
+    float valid = 0.0f;
+    if (!(d <= 1e-5f)) {
+        const f3 nn = x12 / d;
+        const float w1 = x[id1].w, w2 = x[id2].w;
+        const float sw = w1 + w2;
+        k = nn * ((d - pa.x) / sw) * pa.y;
+        valid = 1.0f;
+    }
+    kbuf[c] = f4(k, valid);
+}
+// Δx of each constrained particle: -w1 k on the first end, +w2 k on the second
+__global__ __launch_bounds__(kBlock) void k_distance_apply(float4 *__restrict__ xs, const float4 *__restrict__ x,
+                                                           const int *__restrict__ verts,
+                                                           const int *__restrict__ offs,
+                                                           const int *__restrict__ ents,
+                                                           const float4 *__restrict__ kbuf,
+                                                           const int *__restrict__ map_to_new, int nv) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nv) return;
+    const int slot = map_to_new[verts[t]];
+    const float w = x[slot].w;
+    f3 dx = mk3(0.f, 0.f, 0.f);
+    for (int e = offs[t]; e < offs[t + 1]; ++e) {
+        const int en = ents[e];
+        const float4 k4 = kbuf[en >> 1];
+        if (k4.w == 0.0f) continue;
+        const f3 k = xyz(k4);
+        if (en & 1)
+            dx += w * k;
+        else
+            dx += -w * k;
+    }
+    const float4 a = xs[slot];
+    xs[slot] = f4(xyz(a) + dx, a.w);
+}
+
+// bendingConstraints (bending.cuh:3-76): the four corrections of each dihedral
+__global__ __launch_bounds__(kBlock) void k_bending_compute(const float4 *__restrict__ xs,
+                                                            const float4 *__restrict__ x,
+                                                            const int4 *__restrict__ bends,
+                                                            const int *__restrict__ map_to_new,
+                                                            float4 *__restrict__ bbuf, int nb, float kb) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nb) return;
+    const int4 ids = bends[c];
+    const int id1 = map_to_new[ids.x], id2 = map_to_new[ids.y], id3 = map_to_new[ids.z], id4 = map_to_new[ids.w];
+    for (int a = 0; a < 4; ++a) bbuf[4 * c + a] = make_float4(0.f, 0.f, 0.f, 0.f);
+    const f3 P1 = xyz(xs[id1]);
+    const f3 P2 = xyz(xs[id2]) - P1, P3 = xyz(xs[id3]) - P1, P4 = xyz(xs[id4]) - P1;
+    const f3 c23 = cross(P2, P3), c24 = cross(P2, P4);
+    const float l23 = length(c23);
+    if (l23 <= 1e-5f) return;
+    const float l24 = length(c24);
+    if (l24 <= 1e-5f) return;
+    const f3 N1 = c23 * (1.0f / l23);
+    const f3 N2 = c24 * (1.0f / l24);
+    float d = dot(N1, N2);
+    d = fminf(fmaxf(d, -1.0f), 1.0f);
+    const f3 c2n1 = cross(P2, N1), c2n2 = cross(P2, N2);
+    const f3 c3n1 = cross(P3, N1), c3n2 = cross(P3, N2);
+    const f3 c4n1 = cross(P4, N1), c4n2 = cross(P4, N2);
+    const f3 Q3 = (c2n2 - c2n1 * d) / l23;
+    const f3 Q4 = (c2n1 - c2n2 * d) / l24;
+    const f3 Q2 = (-(c3n2 - c3n1 * d)) / l23 - (c4n1 - c4n2 * d) / l24;
+    const f3 Q1 = -Q2 - Q3 - Q4;
+    const float w1 = x[id1].w, w2 = x[id2].w, w3 = x[id3].w, w4 = x[id4].w;
+    const float den = w1 * length2(Q1) + w2 * length2(Q2) + w3 * length2(Q3) + w4 * length2(Q4);
+    const float num = sqrtf(1.0f - d * d) * (acosf(d) - 3.141592f);
+    if (den <= 1e-5f) return;
+    const float sc = num / den;
+    bbuf[4 * c + 0] = f4(-w1 * Q1 * sc * kb, 1.0f);
+    bbuf[4 * c + 1] = f4(-w2 * Q2 * sc * kb, 1.0f);
+    bbuf[4 * c + 2] = f4(-w3 * Q3 * sc * kb, 1.0f);
+    bbuf[4 * c + 3] = f4(-w4 * Q4 * sc * kb, 1.0f);
+}
+// sum of per-constraint corrections listed in a CSR, applied in place
+__global__ __launch_bounds__(kBlock) void k_gather_apply(float4 *__restrict__ xs, const int *__restrict__ verts,
+                                                         const int *__restrict__ offs, const int *__restrict__ ents,
+                                                         const float4 *__restrict__ buf,
+                                                         const int *__restrict__ map_to_new,
+                                                         const float *__restrict__ m, int divide_by_mass, int nv) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nv) return;
+    const int slot = map_to_new[verts[t]];
+    const float mass = divide_by_mass ? m[slot] : 1.0f;
+    f3 dx = mk3(0.f, 0.f, 0.f);
+    for (int e = offs[t]; e < offs[t + 1]; ++e) {
+        const float4 b = buf[ents[e]];
+        if (b.w == 0.0f) continue;
+        if (divide_by_mass)
+            dx += xyz(b) / mass;
+        else
+            dx += xyz(b);
+    }
+    const float4 a = xs[slot];
+    xs[slot] = f4(xyz(a) + dx, a.w);
+}
+
+// applyWindForce (wind.cuh:3-50) with a wind vector: relative air velocity is
+// the face-average particle velocity minus `wind` (0 reproduces the reference).
+__global__ __launch_bounds__(kBlock) void k_wind_compute(const float4 *__restrict__ xs, const float4 *__restrict__ v,
+                                                         const int4 *__restrict__ faces,
+                                                         const int *__restrict__ map_to_new,
+                                                         float4 *__restrict__ fbuf, int nf, f3 wind, float cd,
+                                                         float cl, float dt) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    const int4 ids = faces[f];
+    const int id1 = map_to_new[ids.x], id2 = map_to_new[ids.y], id3 = map_to_new[ids.z];
+    fbuf[f] = make_float4(0.f, 0.f, 0.f, 0.f);
+    const f3 p1 = xyz(xs[id1]), p2 = xyz(xs[id2]), p3 = xyz(xs[id3]);
+    const f3 a = p2 - p1, b = p3 - p1;
+    f3 vel = (xyz(v[id1]) + xyz(v[id2]) + xyz(v[id3])) / 3.0f;
+    vel = vel - wind;
+    const float lv = length(vel);
+    if (lv <= 1e-5f) return;
+    const f3 vhat = vel / lv;
+    f3 fn = cross(a, b);
+    const float lfn = length(fn);
+    if (lfn <= 1e-5f) return;
+    fn = fn / lfn;
+    fn = (dot(fn, vel) > 0.0f) ? fn : -fn;
+    const float area = 0.5f * lfn;
+    const f3 drag = cd * (lv * lv) * area * dot(fn, vhat) * (-vhat);
+    const f3 uhat = cross(cross(fn, vhat), vhat);
+    const f3 lift = cl * (lv * lv) * area * dot(fn, vhat) * uhat;
+    const f3 force = drag + lift;
+    const f3 move = force * dt * dt;
+    fbuf[f] = f4(move, 1.0f);
+}
+
+// shapeMatchingAlphaOne (shapematching.cuh:8-99): one 64-lane wave per body.
+// Sums are taken in particle order (every lane walks the body in LDS), so all
+// lanes hold the same centre of mass and Apq; the Müller 2016 rotation
+// extraction (shapematching.cuh:70-88) runs uniformly on every lane.
+__global__ __launch_bounds__(64) void k_shape_matching(float4 *__restrict__ xs, const int *__restrict__ map_to_new,
+                                                       const float4 *__restrict__ rest,
+                                                       const int2 *__restrict__ ranges, float4 *__restrict__ rot,
+                                                       float4 *__restrict__ cms, int iters) {
+    __shared__ float sp[kMaxRigidParticles][3];
+    __shared__ float sq[kMaxRigidParticles][3];
+    const int b = blockIdx.x, t = threadIdx.x;
+    const int2 rg = ranges[b];
+    const int np = rg.y - rg.x;
+    int slot = 0;
+    f3 q = mk3(0.f, 0.f, 0.f);
+    float pw = 0.f;
+    if (t < np) {
+        slot = map_to_new[rg.x + t];
+        const float4 a = xs[slot];
+        pw = a.w;
+        sp[t][0] = a.x; sp[t][1] = a.y; sp[t][2] = a.z;
+        const float4 r4 = rest[rg.x + t];
+        q = xyz(r4);
+        sq[t][0] = r4.x; sq[t][1] = r4.y; sq[t][2] = r4.z;
+    }
+    __syncthreads();
+    f3 sum = mk3(0.f, 0.f, 0.f);
+    for (int u = 0; u < np; ++u) sum += mk3(sp[u][0], sp[u][1], sp[u][2]);
+    const f3 cm = sum / (float)np;
+    f3 A0 = mk3(0.f, 0.f, 0.f), A1 = A0, A2 = A0;
+    for (int u = 0; u < np; ++u) {
+        const f3 p = mk3(sp[u][0], sp[u][1], sp[u][2]) - cm;
+        A0 += p * sq[u][0];
+        A1 += p * sq[u][1];
+        A2 += p * sq[u][2];
+    }
+    float4 qq = rot[b];
+    for (int it = 0; it < iters; ++it) {
+        const float x2 = qq.x * qq.x, y2 = qq.y * qq.y, z2 = qq.z * qq.z;
+        const f3 R0 = mk3(1.0f - 2.0f * y2 - 2.0f * z2, 2.0f * qq.x * qq.y + 2.0f * qq.w * qq.z,
+                          2.0f * qq.x * qq.z - 2.0f * qq.w * qq.y);
+        const f3 R1 = mk3(2.0f * qq.x * qq.y - 2.0f * qq.w * qq.z, 1.0f - 2.0f * x2 - 2.0f * z2,
+                          2.0f * qq.y * qq.z + 2.0f * qq.w * qq.x);
+        const f3 R2 = mk3(2.0f * qq.x * qq.z + 2.0f * qq.w * qq.y, 2.0f * qq.y * qq.z - 2.0f * qq.w * qq.x,
+                          1.0f - 2.0f * x2 - 2.0f * y2);
+        const f3 num = cross(R0, A0) + cross(R1, A1) + cross(R2, A2);
+        const float den = 1.0f / fabsf(dot(R0, A0) + dot(R1, A1) + dot(R2, A2)) + 1e-9f;
+        const f3 om = num * den;
+        const float w2 = length2(om);
+        if (w2 <= 1e-9f) break;
+        const float w = sqrtf(w2);
+        const f3 dir = om / w;
+        const float sh = sinf(w * 0.5f), ch = cosf(w * 0.5f);
+        const float4 aa = make_float4(dir.x * sh, dir.y * sh, dir.z * sh, ch);
+        float4 r;  // mul(aa, qq) (cudamatrix.cuh:112-121)
+        r.x = aa.w * qq.x + aa.x * qq.w + aa.y * qq.z - aa.z * qq.y;
+        r.y = aa.w * qq.y - aa.x * qq.z + aa.y * qq.w + aa.z * qq.x;
+        r.z = aa.w * qq.z + aa.x * qq.y - aa.y * qq.x + aa.z * qq.w;
+        r.w = aa.w * qq.w - aa.x * qq.x - aa.y * qq.y - aa.z * qq.z;
+        const float inv = 1.0f / sqrtf(r.x * r.x + r.y * r.y + r.z * r.z + r.w * r.w);
+        qq = make_float4(r.x * inv, r.y * inv, r.z * inv, r.w * inv);
+    }
+    const float x2 = qq.x * qq.x, y2 = qq.y * qq.y, z2 = qq.z * qq.z;
+    const f3 R0 = mk3(1.0f - 2.0f * y2 - 2.0f * z2, 2.0f * qq.x * qq.y + 2.0f * qq.w * qq.z,
+                      2.0f * qq.x * qq.z - 2.0f * qq.w * qq.y);
+    const f3 R1 = mk3(2.0f * qq.x * qq.y - 2.0f * qq.w * qq.z, 1.0f - 2.0f * x2 - 2.0f * z2,
+                      2.0f * qq.y * qq.z + 2.0f * qq.w * qq.x);
+    const f3 R2 = mk3(2.0f * qq.x * qq.z + 2.0f * qq.w * qq.y, 2.0f * qq.y * qq.z - 2.0f * qq.w * qq.x,
+                      1.0f - 2.0f * x2 - 2.0f * y2);
+    if (t == 0) {
+        rot[b] = qq;
+        cms[b] = f4(cm, 0.0f);
+    }
+    if (t < np) {
+        f3 o = R0 * q.x;  // matrix3 * float3 (cudamatrix.cuh:87-95)
+        o += R1 * q.y;
+        o += R2 * q.z;
+        xs[slot] = f4(o + cm, pw);
+    }
+}
+
+// immovableConstraints (immovable.cuh:3-14)
+__global__ __launch_bounds__(kBlock) void k_immovable(float4 *__restrict__ xs, const float4 *__restrict__ x,
+                                                      const int *__restrict__ ids,
+                                                      const int *__restrict__ map_to_new, int ni) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ni) return;
+    const int slot = map_to_new[ids[t]];
+    const float4 a = xs[slot];
+    const float4 b = x[slot];
+    xs[slot] = make_float4(b.x, b.y, b.z, a.w);
+}
+
+// setParticle (particlesolver.cuh:388-393) on the picked particle, found by
+// original id through the inverse map (queryNewParticleId, :360-366).
+__global__ void k_pick(float4 *__restrict__ pos, float4 *__restrict__ v, const int *__restrict__ map_to_new,
+                       int orig, f3 p, int keep_pos, f3 vel) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int slot = map_to_new[orig];
+    const float4 a = pos[slot];
+    if (!keep_pos) pos[slot] = make_float4(p.x, p.y, p.z, a.w);
+    v[slot] = make_float4(vel.x, vel.y, vel.z, 0.0f);
+}
+
+// updateVelocity + updatePositions (particlesolver.cuh:71-81,121-137)
+__global__ __launch_bounds__(kBlock) void k_finish(const float4 *__restrict__ xs, float4 *__restrict__ x,
+                                                   float4 *__restrict__ v, int n, float inv_dt, float eps2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = xs[i];
+    const float4 x4 = x[i];
+    const f3 out = xyz(a), xp = xyz(x4);
+    v[i] = f4((out - xp) * inv_dt, 0.0f);
+    const float d2 = length2(out - xp);
+    if (d2 >= eps2 || phase_of(a) < 0) x[i] = f4(out, x4.w);
+}
+
+// ------------------------------------------------------------ launchers
+void launch_plane_collide(float4 *xs, const float4 *x, int n, const Planes &pl, float r, float fs, float fd,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(k_plane_collide, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, x, n, pl, r, fs, fd);
+}
+void launch_particle_particle(const float4 *xs, const float4 *x, const int2 *ranges, float4 *xs_out, int n,
+                              const GridDesc &G, float r, float fs, float fd, hipStream_t st) {
+    hipLaunchKernelGGL(k_particle_particle, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, x, ranges, xs_out, n, G,
+                       r, fs, fd);
+}
+void launch_distance(float4 *xs, const float4 *x, const int2 *pairs, const float2 *params, const int *map_to_new,
+                     float4 *kbuf, int nc, const int *verts, const int *offs, const int *ents, int nv,
+                     hipStream_t st) {
+    hipLaunchKernelGGL(k_distance_compute, dim3(grid_blocks(nc)), dim3(kBlock), 0, st, xs, x, pairs, params,
+                       map_to_new, kbuf, nc);
+    hipLaunchKernelGGL(k_distance_apply, dim3(grid_blocks(nv)), dim3(kBlock), 0, st, xs, x, verts, offs, ents, kbuf,
+                       map_to_new, nv);
+}
+void launch_bending(float4 *xs, const float4 *x, const int4 *bends, const int *map_to_new, float4 *bbuf, int nb,
+                    float kb, const int *verts, const int *offs, const int *ents, int nv, hipStream_t st) {
+    hipLaunchKernelGGL(k_bending_compute, dim3(grid_blocks(nb)), dim3(kBlock), 0, st, xs, x, bends, map_to_new,
+                       bbuf, nb, kb);
+    hipLaunchKernelGGL(k_gather_apply, dim3(grid_blocks(nv)), dim3(kBlock), 0, st, xs, verts, offs, ents, bbuf,
+                       map_to_new, (const float *)nullptr, 0, nv);
+}
+void launch_wind(float4 *xs, const float4 *v, const float *m, const int4 *faces, const int *map_to_new,
+                 float4 *fbuf, int nf, f3 wind, float cd, float cl, float dt, const int *verts, const int *offs,
+                 const int *ents, int nv, hipStream_t st) {
+    hipLaunchKernelGGL(k_wind_compute, dim3(grid_blocks(nf)), dim3(kBlock), 0, st, xs, v, faces, map_to_new, fbuf,
+                       nf, wind, cd, cl, dt);
+    hipLaunchKernelGGL(k_gather_apply, dim3(grid_blocks(nv)), dim3(kBlock), 0, st, xs, verts, offs, ents, fbuf,
+                       map_to_new, m, 1, nv);
+}
+void launch_shape_matching(float4 *xs, const int *map_to_new, const float4 *rest, const int2 *ranges, float4 *rot,
+                           float4 *cms, int nr, int iters, hipStream_t st) {
+    hipLaunchKernelGGL(k_shape_matching, dim3(nr), dim3(64), 0, st, xs, map_to_new, rest, ranges, rot, cms, iters);
+}
+void launch_immovable(float4 *xs, const float4 *x, const int *ids, const int *map_to_new, int ni, hipStream_t st) {
+    hipLaunchKernelGGL(k_immovable, dim3(grid_blocks(ni)), dim3(kBlock), 0, st, xs, x, ids, map_to_new, ni);
+}
+void launch_pick(float4 *pos, float4 *v, const int *map_to_new, int orig, f3 p, int keep_pos, f3 vel,
+                 hipStream_t st) {
+    hipLaunchKernelGGL(k_pick, dim3(1), dim3(64), 0, st, pos, v, map_to_new, orig, p, keep_pos, vel);
+}
+void launch_finish(const float4 *xs, float4 *x, float4 *v, int n, float inv_dt, float eps2, hipStream_t st) {
+    hipLaunchKernelGGL(k_finish, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, x, v, n, inv_dt, eps2);
+}
+
+}  // namespace bp

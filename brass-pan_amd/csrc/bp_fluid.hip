@@ -1,0 +1,279 @@
+// bp_fluid.hip -- position-based fluid passes on gfx950: density/lambda,
+// position correction, vorticity, Akinci normal + tension and XSPH.
+// Gather form: each thread owns one sorted slot and writes only its own
+// outputs, so there are no floating-point atomics and results are
+// deterministic.  Neighbour candidates are the reference's 27 cells around the
+// particle's current cell (fluid.cuh:104-115), walked as 9 column runs of
+// consecutive slots.
+#include "bp_kernels.h"
+
+namespace bp {
+
+__device__ __forceinline__ void unpack_cell(uint32_t c, int g[3]) {
+    g[0] = (int)(c >> 20);
+    g[1] = (int)((c >> 10) & 1023u);
+    g[2] = (int)(c & 1023u);
+}
+
+// fluidLambda (fluid.cuh:78-159).  Also records the particle's current cell
+// (packed) for the position pass, which searches from the same x*.
+__global__ __launch_bounds__(kBlock) void k_lambda(const float4 *__restrict__ xs, const float *__restrict__ m,
+                                                   const int2 *__restrict__ ranges, float *__restrict__ lambda,
+                                                   float *__restrict__ density, uint32_t *__restrict__ ccell, int n,
+                                                   GridDesc G, SphConst K, FluidParams F) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = xs[i];
+    const f3 pi = xyz(a);
+    int g[3];
+    G.pos(pi, g);
+    ccell[i] = G.pack(g);
+    if (phase_of(a) > 0) {  // solids carry no lambda (see DESIGN)
+        lambda[i] = 0.0f;
+        density[i] = 0.0f;
+        return;
+    }
+    float dens = 0.0f, sumg2 = 0.0f;
+    f3 gi = mk3(0.f, 0.f, 0.f);
+    const float sds = F.solid_density_scale;
+    for_each_run(G, ranges, g, [&](int s, int e) {
+        for (int j = s > i ? s : i; j < e; ++j) {  // Q1: only j >= i (fluid.cuh:117)
+            const float4 b = xs[j];
+            const f3 diff = pi - xyz(b);
+            const float d2 = length2(diff);
+            if (d2 <= K.h2) {
+                const float mj = m[j];
+                const float ld = mj * K.poly6(d2);
+                dens += (phase_of(b) < 0) ? ld : sds * ld;
+                if (F.lambda_mode == 1 && d2 > 0.0f) {
+                    const f3 gr = (-mj * K.grad(diff, d2)) / F.rest_density;
+                    sumg2 += dot(gr, gr);
+                    gi -= gr;
+                }
+            }
+        }
+    });
+    if (F.lambda_mode == 1) sumg2 += dot(gi, gi);
+    float C = dens / F.rest_density - 1.0f;
+    if (F.akinci) C = fmaxf(C, 0.0f);
+    lambda[i] = (F.lambda_mode == 1) ? -C / (sumg2 + F.lambda_relaxation) : -C / F.lambda_denominator;
+    density[i] = dens;
+}
+
+// fluidPosition pair term (fluid.cuh:199-221): delta of pair (k,j) seen from k.
+__device__ __forceinline__ f3 pair_delta(f3 diff, float d2, float lam_sum, const SphConst &K, const FluidParams &F) {
+    float sl = lam_sum;
+    if (!F.akinci) sl += -F.scorr_k * powf(K.poly6(d2) / K.wdq, F.scorr_n);
+    f3 delta = sl * K.grad(diff, d2);
+    const float len = length(delta);
+    return (len < F.delta_clamp) ? delta : delta * fminf(len, F.delta_clamp) / len;
+}
+
+// fluidPosition (fluid.cuh:161-228) + Δx apply (particlesolver.cuh:919-932)
+// in gather form.  Pair (i<j) exists in the reference iff particle i is not a
+// solid (fluid.cuh:176) and j lies in i's search cells; slot k therefore sums
+//   partners j > k found by its own search (k not solid), and
+//   partners j < k (j not solid) whose search around their current cell
+//   contains k's cell,
+// each as m_j δ(k,j), δ(k,j) = -δ(j,k) exactly.
+__global__ __launch_bounds__(kBlock) void k_position(const float4 *__restrict__ xs, const float *__restrict__ m,
+                                                     const float *__restrict__ lambda,
+                                                     const uint32_t *__restrict__ ccell,
+                                                     const int *__restrict__ build_keys,
+                                                     const int2 *__restrict__ ranges, float4 *__restrict__ xs_out,
+                                                     float4 *__restrict__ x_io, float4 *__restrict__ v_out, int n,
+                                                     GridDesc G, SphConst K, FluidParams F, Planes pl, Epilogue E) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float4 a = xs[k];
+    const f3 pk = xyz(a);
+    const bool kf = phase_of(a) <= 0;
+    const float lk = lambda[k];
+    const uint32_t ck = ccell[k];
+    const uint32_t bk = G.pack_addr(build_keys[k]);
+    int g[3];
+    unpack_cell(ck, g);
+    f3 sum = mk3(0.f, 0.f, 0.f);
+    for_each_run(G, ranges, g, [&](int s, int e) {
+        const int e2 = kf ? e : (e < k ? e : k);
+        for (int j = s; j < e2; ++j) {
+            if (j == k) continue;
+            const float4 b = xs[j];
+            if (j < k && phase_of(b) > 0) continue;
+            const f3 diff = pk - xyz(b);
+            const float d2 = length2(diff);
+            if (!(d2 <= K.h2 && d2 > 0.0f)) continue;
+            if (j < k && !G.in_block(ccell[j], bk)) continue;
+            const f3 delta = pair_delta(diff, d2, lk + lambda[j], K, F);
+            sum += m[j] * delta;
+        }
+    });
+    f3 out = pk + sum * F.inv_rest_density;
+    if (E.planes_next || E.finish) {
+        float4 x4 = x_io[k];
+        const f3 xp = xyz(x4);
+        if (E.planes_next) out = plane_collide_one(out, xp, pl, E.radius, E.friction_static, E.friction_dynamic);
+        if (E.finish) {  // updateVelocity + updatePositions (particlesolver.cuh:71-81,121-137)
+            v_out[k] = f4((out - xp) * E.inv_dt, 0.0f);
+            const float d2 = length2(out - xp);
+            if (d2 >= E.sleep_eps2 || phase_of(a) < 0) x_io[k] = f4(out, x4.w);
+        }
+    }
+    xs_out[k] = f4(out, a.w);
+}
+
+// fluidOmega (fluid.cuh:231-279) and fluidNormal (fluid.cuh:387-434) in one
+// sweep (independent sums over the same neighbours).  omega.w = |omega|.
+__global__ __launch_bounds__(kBlock) void k_post_a(const float4 *__restrict__ xs, const float4 *__restrict__ v,
+                                                   const float *__restrict__ density,
+                                                   const int2 *__restrict__ ranges, float4 *__restrict__ omega,
+                                                   float4 *__restrict__ normal, uint32_t *__restrict__ ccell, int n,
+                                                   GridDesc G, SphConst K, int do_omega, int do_normal) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = xs[i];
+    const int ph = phase_of(a);
+    if (ph >= 0) return;
+    const f3 pi = xyz(a);
+    int g[3];
+    G.pos(pi, g);
+    ccell[i] = G.pack(g);
+    const f3 vi = xyz(v[i]);
+    f3 om = mk3(0.f, 0.f, 0.f), nn = mk3(0.f, 0.f, 0.f);
+    for_each_run(G, ranges, g, [&](int s, int e) {
+        for (int j = s; j < e; ++j) {
+            if (j == i) continue;
+            const float4 b = xs[j];
+            if (phase_of(b) != ph) continue;
+            const f3 diff = pi - xyz(b);
+            const float d2 = length2(diff);
+            if (d2 <= K.h2 && d2 > 0.0f) {
+                const f3 gr = K.grad(diff, d2);
+                if (do_omega) om += cross(xyz(v[j]) - vi, gr);
+                if (do_normal) nn += (1.0f / density[j]) * gr;
+            }
+        }
+    });
+    if (do_omega) omega[i] = f4(om, length(om));
+    if (do_normal) normal[i] = f4(K.h * nn, 0.0f);
+}
+
+// fluidVorticity (fluid.cuh:281-336, in place) followed by fluidAkinciTension
+// (fluid.cuh:436-489) for the same particle: both only change the particle's
+// own velocity, so the fused pass writes v in place.
+__global__ __launch_bounds__(kBlock) void k_post_b(const float4 *__restrict__ xs, float4 *__restrict__ v,
+                                                   const float4 *__restrict__ omega,
+                                                   const float4 *__restrict__ normal,
+                                                   const float *__restrict__ density,
+                                                   const int2 *__restrict__ ranges,
+                                                   const uint32_t *__restrict__ ccell, int n, GridDesc G, SphConst K,
+                                                   FluidParams F, int do_vort, int do_akinci, float dt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = xs[i];
+    const int ph = phase_of(a);
+    if (ph >= 0) return;
+    const f3 pi = xyz(a);
+    int g[3];
+    unpack_cell(ccell[i], g);
+    f3 eta = mk3(0.f, 0.f, 0.f), f = mk3(0.f, 0.f, 0.f);
+    const f3 ni = do_akinci ? xyz(normal[i]) : mk3(0.f, 0.f, 0.f);
+    const float rhoi = do_akinci ? density[i] : 0.0f;
+    const float gam = F.tension_gamma;
+    for_each_run(G, ranges, g, [&](int s, int e) {
+        for (int j = s; j < e; ++j) {
+            if (j == i) continue;
+            const float4 b = xs[j];
+            if (phase_of(b) != ph) continue;
+            const f3 pj = xyz(b);
+            const f3 diff = pi - pj;
+            const float d2 = length2(diff);
+            if (d2 <= K.h2 && d2 > 0.0f) {
+                if (do_vort) eta += omega[j].w * K.grad(diff, d2);
+                if (do_akinci) {
+                    const float dist = sqrtf(d2);
+                    const f3 fcoh = -gam * K.akinci(dist) * (pi - pj) / dist;
+                    const f3 fcur = -gam * (ni - xyz(normal[j]));
+                    const float kij = 2.0f * F.rest_density / (rhoi + density[j]);
+                    f += kij * (fcoh + fcur);
+                }
+            }
+        }
+    });
+    f3 vi = xyz(v[i]);
+    if (do_vort && length2(eta) > 1e-3f) {
+        const f3 nrm = eta * (1.0f / sqrtf(dot(eta, eta)));
+        vi += F.vorticity_eps * cross(nrm, xyz(omega[i])) * dt;
+    }
+    if (do_akinci) vi = vi + f * dt;
+    v[i] = f4(vi, 0.0f);
+}
+
+// fluidXSph (fluid.cuh:338-385).  Non-fluid particles keep their velocity.
+__global__ __launch_bounds__(kBlock) void k_xsph(const float4 *__restrict__ xs, const float4 *__restrict__ v,
+                                                 const int2 *__restrict__ ranges,
+                                                 const uint32_t *__restrict__ ccell, float4 *__restrict__ v_out,
+                                                 int n, GridDesc G, SphConst K, float c, int use_ccell) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = xs[i];
+    const int ph = phase_of(a);
+    const float4 v4 = v[i];
+    if (ph >= 0) {
+        v_out[i] = v4;
+        return;
+    }
+    const f3 pi = xyz(a);
+    int g[3];
+    if (use_ccell)
+        unpack_cell(ccell[i], g);
+    else
+        G.pos(pi, g);
+    const f3 vi = xyz(v4);
+    f3 vn = mk3(0.f, 0.f, 0.f);
+    for_each_run(G, ranges, g, [&](int s, int e) {
+        for (int j = s; j < e; ++j) {
+            if (j == i) continue;
+            const float4 b = xs[j];
+            if (phase_of(b) != ph) continue;
+            const f3 diff = pi - xyz(b);
+            const float d2 = length2(diff);
+            if (d2 <= K.h2) vn += (xyz(v[j]) - vi) * K.poly6(d2);
+        }
+    });
+    v_out[i] = f4(vi + c * vn, 0.0f);
+}
+
+// ------------------------------------------------------------ launchers
+void launch_lambda(const float4 *xs, const float *m, const int2 *ranges, float *lambda, float *density,
+                   uint32_t *ccell, int n, const GridDesc &G, const SphConst &K, const FluidParams &F,
+                   hipStream_t st) {
+    hipLaunchKernelGGL(k_lambda, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, m, ranges, lambda, density, ccell,
+                       n, G, K, F);
+}
+void launch_position(const float4 *xs, const float *m, const float *lambda, const uint32_t *ccell,
+                     const int *build_keys, const int2 *ranges, float4 *xs_out, float4 *x_io, float4 *v_out, int n,
+                     const GridDesc &G, const SphConst &K, const FluidParams &F, const Planes &pl,
+                     const Epilogue &E, hipStream_t st) {
+    hipLaunchKernelGGL(k_position, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, m, lambda, ccell, build_keys,
+                       ranges, xs_out, x_io, v_out, n, G, K, F, pl, E);
+}
+void launch_post_a(const float4 *xs, const float4 *v, const float *density, const int2 *ranges, float4 *omega,
+                   float4 *normal, uint32_t *ccell, int n, const GridDesc &G, const SphConst &K, int do_omega,
+                   int do_normal, hipStream_t st) {
+    hipLaunchKernelGGL(k_post_a, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, v, density, ranges, omega, normal,
+                       ccell, n, G, K, do_omega, do_normal);
+}
+void launch_post_b(const float4 *xs, float4 *v, const float4 *omega, const float4 *normal, const float *density,
+                   const int2 *ranges, const uint32_t *ccell, int n, const GridDesc &G, const SphConst &K,
+                   const FluidParams &F, int do_vort, int do_akinci, float dt, hipStream_t st) {
+    hipLaunchKernelGGL(k_post_b, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, v, omega, normal, density, ranges,
+                       ccell, n, G, K, F, do_vort, do_akinci, dt);
+}
+void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const uint32_t *ccell, float4 *v_out,
+                 int n, const GridDesc &G, const SphConst &K, float c, int use_ccell, hipStream_t st) {
+    hipLaunchKernelGGL(k_xsph, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, v, ranges, ccell, v_out, n, G, K, c,
+                       use_ccell);
+}
+
+}  // namespace bp

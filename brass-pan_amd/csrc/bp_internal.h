@@ -1,0 +1,181 @@
+// bp_internal.h -- shared device helpers and launch interfaces of the gfx950
+// particle solver.  Everything here is HIP C++ for gfx950 only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bp {
+
+constexpr int kMaxPlanes = 16;
+constexpr int kMaxRigidParticles = 64;  // NUM_MAX_PARTICLE_PER_RIGID_BODY (cubrasstower.cuh:22)
+constexpr int kBlock = 256;
+
+// ------------------------------------------------------------ small vectors
+// Own 3-vector type: component-wise operators with the left-to-right order of
+// the reference's float3 operators (ext/helper_math.h), so FP32 results match
+// the host oracle bit-for-bit wherever the same expression is evaluated.
+struct f3 {
+    float x, y, z;
+};
+__host__ __device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+__host__ __device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__host__ __device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__host__ __device__ __forceinline__ f3 operator-(f3 a) { return {-a.x, -a.y, -a.z}; }
+__host__ __device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__host__ __device__ __forceinline__ f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__host__ __device__ __forceinline__ f3 operator/(f3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__host__ __device__ __forceinline__ f3 &operator+=(f3 &a, f3 b) { a = a + b; return a; }
+__host__ __device__ __forceinline__ f3 &operator-=(f3 &a, f3 b) { a = a - b; return a; }
+__host__ __device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ __forceinline__ float length2(f3 a) { return dot(a, a); }
+__device__ __forceinline__ float length(f3 a) { return sqrtf(dot(a, a)); }
+__host__ __device__ __forceinline__ f3 cross(f3 a, f3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ f3 xyz(float4 a) { return f3{a.x, a.y, a.z}; }
+__device__ __forceinline__ float4 f4(f3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+
+// ------------------------------------------------------------ parameters
+struct Planes {
+    int n;
+    float o[kMaxPlanes][3];
+    float nrm[kMaxPlanes][3];
+};
+
+// Uniform hash grid (grid.cuh:6-28), per-axis power-of-two dims.
+struct GridDesc {
+    float cs[3];
+    int dims[3];
+    uint32_t mask[3];
+    uint32_t stride_x;  // dims[1]*dims[2]
+    uint32_t shift_y;   // log2(dims[2])
+    uint32_t shift_x;   // log2(dims[1]*dims[2])
+    // cell address -> packed wrapped coordinates
+    __device__ __forceinline__ uint32_t pack_addr(int a) const {
+        const uint32_t u = (uint32_t)a;
+        return ((u >> shift_x) << 20) | (((u >> shift_y) & mask[1]) << 10) | (u & mask[2]);
+    }
+    __device__ __forceinline__ void pos(f3 p, int g[3]) const {
+        g[0] = (int)((p.x + 50000.0f) / cs[0]);
+        g[1] = (int)((p.y + 50000.0f) / cs[1]);
+        g[2] = (int)((p.z + 50000.0f) / cs[2]);
+    }
+    __device__ __forceinline__ int addr(int x, int y, int z) const {
+        return (int)(((uint32_t)x & mask[0]) * stride_x + ((uint32_t)y & mask[1]) * (uint32_t)dims[2] +
+                     ((uint32_t)z & mask[2]));
+    }
+    // wrapped coordinates packed 10:10:10 (dims <= 1024 per axis)
+    __device__ __forceinline__ uint32_t pack(const int g[3]) const {
+        return (((uint32_t)g[0] & mask[0]) << 20) | (((uint32_t)g[1] & mask[1]) << 10) | ((uint32_t)g[2] & mask[2]);
+    }
+    // true iff packed cell `c` lies in the 27-cell block around packed cell `g`
+    __device__ __forceinline__ bool in_block(uint32_t g, uint32_t c) const {
+        uint32_t dx = (((c >> 20) & 1023u) - ((g >> 20) & 1023u) + 1u) & mask[0];
+        uint32_t dy = (((c >> 10) & 1023u) - ((g >> 10) & 1023u) + 1u) & mask[1];
+        uint32_t dz = ((c & 1023u) - (g & 1023u) + 1u) & mask[2];
+        return dx <= 2u && dy <= 2u && dz <= 2u;
+    }
+};
+
+// SPH kernel constants, SetKernelRadius (fluid.cuh:11-27), computed on the host.
+struct SphConst {
+    float c1, c2, c3, c4, h, h2, halfh, wdq;
+    __device__ __forceinline__ float poly6(float r2) const {
+        float t = h2 - r2;
+        return c1 * t * t * t;
+    }
+    __device__ __forceinline__ f3 grad(f3 v, float r2) const {
+        float r = sqrtf(r2);
+        float t = h - r;
+        float s = c2 * t * t;
+        return (s * v) / r;
+    }
+    __device__ __forceinline__ float akinci(float r) const {
+        float t = (h - r) * r;
+        float t3 = t * t * t;
+        if (r >= halfh) return c3 * t3;
+        return 2.0f * c3 * t3 - c4;
+    }
+};
+
+struct FluidParams {
+    float rest_density, inv_rest_density, solid_density_scale;
+    int lambda_mode;
+    float lambda_denominator, lambda_relaxation;
+    int akinci;
+    float scorr_k, scorr_n, delta_clamp;
+    float vorticity_eps, tension_gamma, xsph_c;
+};
+
+// Epilogue fused into the last full-particle Jacobi pass of an iteration.
+struct Epilogue {
+    int planes_next;      // apply the next iteration's plane collisions
+    int finish;           // last iteration: updateVelocity + updatePositions
+    float inv_dt;
+    float sleep_eps2;
+    float radius, friction_static, friction_dynamic;
+};
+
+// ------------------------------------------------------------ device helpers
+__device__ __forceinline__ int phase_of(float4 a) { return __float_as_int(a.w); }
+
+// particlePlaneCollisionConstraint (particleplane.cuh:21-58) for one particle
+__device__ __forceinline__ f3 plane_collide_one(f3 xs, f3 pos, const Planes &pl, float r, float fs, float fd) {
+    for (int p = 0; p < pl.n; ++p) {
+        f3 o = mk3(pl.o[p][0], pl.o[p][1], pl.o[p][2]);
+        f3 n = mk3(pl.nrm[p][0], pl.nrm[p][1], pl.nrm[p][2]);
+        f3 o2p = o - xs;
+        float d = dot(o2p, n) + r;
+        if (d <= 0.0f) continue;
+        f3 diff = xs - pos;
+        float dn = dot(diff, n);
+        f3 dtan = diff - dn * n;
+        float dtl = length(dtan);
+        f3 resolved = d * n + xs;
+        f3 dX = resolved - pos;
+        f3 tdx = dX - dot(dX, n) * n;
+        if (dtl < fs * dn)
+            xs = resolved - tdx;
+        else
+            xs = resolved - tdx * fminf(fd * -dn / dtl, 1.0f);
+    }
+    return xs;
+}
+
+// Iterates the reference's 27-cell candidate set (fluid.cuh:104-115) around
+// grid position g: each of the 9 (x,y) columns is one or two runs of
+// consecutive cell addresses, whose particles are consecutive slots.
+template <class F>
+__device__ __forceinline__ void for_each_run(const GridDesc &G, const int2 *__restrict__ ranges, const int g[3],
+                                             F &&f) {
+    const uint32_t z0 = (uint32_t)(g[2] - 1) & G.mask[2];
+    const uint32_t z2 = (uint32_t)(g[2] + 1) & G.mask[2];
+    const bool wrap = z0 > z2;
+    for (int dy = -1; dy <= 1; ++dy) {
+        for (int dx = -1; dx <= 1; ++dx) {
+            const uint32_t col = ((uint32_t)(g[0] + dx) & G.mask[0]) * G.stride_x +
+                                 ((uint32_t)(g[1] + dy) & G.mask[1]) * (uint32_t)G.dims[2];
+            uint32_t a = z0, b = wrap ? G.mask[2] : z2;
+#pragma unroll
+            for (int part = 0; part < 2; ++part) {
+                int s = -1, e = -1;
+                for (uint32_t c = a; c <= b; ++c) {
+                    int2 r = ranges[col + c];
+                    if (r.x >= 0) {
+                        if (s < 0) s = r.x;
+                        e = r.y;
+                    }
+                }
+                if (s >= 0) f(s, e);
+                if (!wrap) break;
+                a = 0;
+                b = z2;
+            }
+        }
+    }
+}
+
+inline int grid_blocks(long long n, int block = kBlock) { return (int)((n + block - 1) / block); }
+
+}  // namespace bp

@@ -1,0 +1,960 @@
+// bp_solver.hip -- C ABI (include/brasspan.h) and substep orchestration.
+//
+// Replaces struct ParticleSolver (brasstower/particlesolver.cuh:146-1148).
+// One handle owns all device memory of one scene and one HIP stream; the
+// substep pipeline is ParticleSolver::update (:772-1072) with the projection
+// and wind blocks active, fused into fewer passes:
+//   predict+hash(+reset) | radix sort | ranges | reorder(+inverse map, inverse
+//   scaled mass, stabilisation, first plane collisions) | [wind] |
+//   iterations { [planes] [particle-particle] lambda  position(+next planes |
+//   +velocity/position update) [distance] [bending] [shape matching] [pins] } |
+//   [finish] | [omega+normal] [vorticity+tension] [xsph]
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/brasspan.h"
+#include "bp_kernels.h"
+
+using namespace bp;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+bp_status fail(bp_status st, const std::string &msg) {
+    g_last_error = msg;
+    return st;
+}
+
+#define HIP_TRY(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return fail(BP_ERR_HIP, std::string(#expr) + " failed: " + hipGetErrorString(e_));          \
+    } while (0)
+
+template <class T> struct DBuf {
+    T *b[2] = {nullptr, nullptr};
+    int c = 0;
+    T *cur() const { return b[c]; }
+    T *alt() const { return b[1 - c]; }
+    void flip() { c = 1 - c; }
+};
+
+struct Csr {
+    int nv = 0;
+    int *verts = nullptr, *offs = nullptr, *ents = nullptr;
+};
+
+enum Family {
+    F_PREDICT, F_SORT, F_RANGES, F_REORDER, F_WIND, F_PLANES, F_PP, F_LAMBDA, F_POSITION, F_DISTANCE,
+    F_BENDING, F_SHAPE, F_IMMOVABLE, F_FINISH, F_POST_A, F_POST_B, F_XSPH, F_PICK, F_COUNT
+};
+const char *kFamilyNames[F_COUNT] = {"predict_hash", "radix_sort", "cell_ranges", "reorder", "wind",
+                                     "plane_collide", "particle_particle", "fluid_lambda", "fluid_position",
+                                     "distance", "bending", "shape_matching", "immovable", "finish",
+                                     "omega_normal", "vorticity_tension", "xsph", "pick"};
+
+bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+int ilog2(unsigned v) {
+    int r = 0;
+    while ((1u << r) < v) ++r;
+    return r;
+}
+
+}  // namespace
+
+struct bp_solver {
+    int device = 0;
+    hipStream_t st = nullptr;
+    bp_params P{};
+    int N = 0;
+    GridDesc G{};
+    int num_cells = 0, sort_bits = 0;
+    SphConst K{};
+    FluidParams F{};
+    Planes planes{};
+    float radius = 0.f;
+    bool has_solid = false, has_fluid = false;
+
+    DBuf<float4> xs, x, v;
+    DBuf<float> m;
+    DBuf<int> ph, orig;
+    int *map_to_new = nullptr;
+
+    int *keys[2] = {nullptr, nullptr};
+    int *vals[2] = {nullptr, nullptr};
+    int *counts = nullptr, *totals = nullptr;
+    int sk_idx = -1;  // keys buffer holding the last sorted keys
+    int *sorted_keys = nullptr, *perm = nullptr;
+    int2 *ranges = nullptr;
+
+    float *lambda = nullptr, *density = nullptr;
+    uint32_t *ccell = nullptr;
+    float4 *omega = nullptr, *normal = nullptr;
+
+    int nd = 0;
+    int2 *dpairs = nullptr;
+    float2 *dparams = nullptr;
+    float4 *kbuf = nullptr;
+    Csr dcsr;
+    int nb = 0;
+    int4 *bends = nullptr;
+    float4 *bbuf = nullptr;
+    Csr bcsr;
+    int nf = 0;
+    int4 *faces = nullptr;
+    float4 *fbuf = nullptr;
+    Csr fcsr;
+    int nr = 0;
+    int2 *rranges = nullptr;
+    float4 *rrest = nullptr, *rrot = nullptr, *rcm = nullptr;
+    int ni = 0;
+    int *immov = nullptr;
+
+    std::vector<float> mass_by_orig;
+    std::vector<int> phase_by_orig;
+    std::vector<int> group_by_orig;
+    std::vector<void *> allocs;
+
+    // per-family kernel timing (bp_solver_enable_kernel_timing)
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_next = 0;
+    struct Rec {
+        int fam;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    float fam_ms[F_COUNT] = {};
+    int fam_n[F_COUNT] = {};
+
+    template <class T> bp_status alloc(T **p, size_t count) {
+        *p = nullptr;
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+        if (e != hipSuccess) return fail(BP_ERR_HIP, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        allocs.push_back((void *)*p);
+        return BP_OK;
+    }
+    template <class T> bp_status upload(T **p, const T *host, size_t count) {
+        bp_status s = alloc(p, count);
+        if (s != BP_OK) return s;
+        if (count && host) HIP_TRY(hipMemcpyAsync(*p, host, count * sizeof(T), hipMemcpyHostToDevice, st));
+        return BP_OK;
+    }
+
+    void tbeg(int fam, hipEvent_t *out) {
+        *out = nullptr;
+        if (!timing) return;
+        if (ev_next + 2 > ev_pool.size()) return;
+        *out = ev_pool[ev_next];
+        (void)hipEventRecord(*out, st);
+        ev_next += 1;
+        (void)fam;
+    }
+    void tend(int fam, hipEvent_t a) {
+        if (!timing || !a) return;
+        hipEvent_t b = ev_pool[ev_next++];
+        (void)hipEventRecord(b, st);
+        recs.push_back(Rec{fam, a, b});
+    }
+};
+
+#define TIMED(fam, stmt)      \
+    do {                      \
+        hipEvent_t ev_;       \
+        s->tbeg(fam, &ev_);   \
+        stmt;                 \
+        s->tend(fam, ev_);    \
+    } while (0)
+
+namespace {
+
+bp_status build_csr(bp_solver *s, Csr &c, int nent_per, const int *ids, int count, int N, int div) {
+    // entries (item * nent_per + corner) / div, grouped by particle id, in item order
+    std::vector<int> cnt(N, 0);
+    for (int i = 0; i < count * nent_per; ++i) cnt[ids[i]]++;
+    std::vector<int> verts, offs;
+    std::vector<int> pos(N, -1);
+    for (int p = 0; p < N; ++p)
+        if (cnt[p] > 0) {
+            pos[p] = (int)verts.size();
+            verts.push_back(p);
+        }
+    offs.assign(verts.size() + 1, 0);
+    for (size_t t = 0; t < verts.size(); ++t) offs[t + 1] = offs[t] + cnt[verts[t]];
+    std::vector<int> fill(offs.begin(), offs.end() - 1), ents((size_t)count * nent_per);
+    for (int i = 0; i < count * nent_per; ++i) ents[fill[pos[ids[i]]]++] = i / div;
+    c.nv = (int)verts.size();
+    bp_status st;
+    if ((st = s->upload(&c.verts, verts.data(), verts.size())) != BP_OK) return st;
+    if ((st = s->upload(&c.offs, offs.data(), offs.size())) != BP_OK) return st;
+    if ((st = s->upload(&c.ents, ents.data(), ents.size())) != BP_OK) return st;
+    return BP_OK;
+}
+
+void set_derived_params(bp_solver *s) {
+    const bp_params &P = s->P;
+    s->F.rest_density = P.fluid_rest_density;
+    s->F.inv_rest_density = 1.0f / P.fluid_rest_density;
+    s->F.solid_density_scale = P.solid_density_scale;
+    s->F.lambda_mode = P.lambda_mode;
+    s->F.lambda_denominator = P.lambda_denominator;
+    s->F.lambda_relaxation = P.lambda_relaxation;
+    s->F.akinci = P.akinci;
+    s->F.scorr_k = P.scorr_k;
+    s->F.scorr_n = (float)P.scorr_n;
+    s->F.delta_clamp = P.delta_clamp;
+    s->F.vorticity_eps = P.vorticity_eps;
+    s->F.tension_gamma = P.tension_gamma;
+    s->F.xsph_c = P.xsph_c;
+}
+
+void set_kernel_const(SphConst &K, float h, float scorr_dq) {
+    // SetKernelRadius (fluid.cuh:11-27), host powf
+    K.c1 = 315.f / 64.f / 3.141592f / powf(h, 9.0f);
+    K.c2 = -45.f / 3.141592f / powf(h, 6.f);
+    K.c3 = 32.0f / 3.141592f / powf(h, 9.f);
+    K.c4 = powf(h, 6.0f) / 64.0f;
+    K.h2 = h * h;
+    K.halfh = h * 0.5f;
+    K.h = h;
+    // poly6Kernel(powf(0.03f * KernelRadius, 2.f)) (fluid.cuh:208)
+    const float r2 = powf(scorr_dq * h, 2.0f);
+    const float t = K.h2 - r2;
+    K.wdq = K.c1 * t * t * t;
+}
+
+bp_status check_params(const bp_params *p) {
+    if (!p) return fail(BP_ERR_INVALID_ARG, "params is NULL");
+    if (p->iterations < 0 || p->stabilization_passes < 0)
+        return fail(BP_ERR_INVALID_ARG, "iterations and stabilization_passes must be >= 0");
+    if (!(p->fluid_rest_density > 0.0f)) return fail(BP_ERR_INVALID_ARG, "fluid_rest_density must be > 0");
+    if (p->lambda_mode != 0 && p->lambda_mode != 1) return fail(BP_ERR_INVALID_ARG, "lambda_mode must be 0 or 1");
+    if (p->shape_matching_iterations < 0) return fail(BP_ERR_INVALID_ARG, "shape_matching_iterations < 0");
+    return BP_OK;
+}
+
+// one substep of ParticleSolver::update (particlesolver.cuh:787-1063)
+bp_status substep(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick, int picked, f3 ppos) {
+    const int N = s->N;
+    const bp_params &P = s->P;
+    hipStream_t st = s->st;
+    const f3 zero = mk3(0.f, 0.f, 0.f);
+    const bool wind_on = P.wind_enabled && s->nf > 0;
+    const bool tail_groups = s->nd > 0 || s->nb > 0 || s->nr > 0 || s->ni > 0;
+    const bool need_w = s->has_solid || s->nd > 0 || s->nb > 0;
+    const int iters = P.iterations;
+    const float fs = P.friction_static, fd = P.friction_dynamic;
+
+    if (pick) TIMED(F_PICK, launch_pick(s->x.cur(), s->v.cur(), s->map_to_new, picked, ppos, 0, zero, st));
+
+    // predict + hash, clearing the ranges of the previous substep
+    int kin_idx = (s->sk_idx == 0) ? 1 : 0;
+    int *kin = s->keys[kin_idx], *kother = s->keys[1 - kin_idx];
+    TIMED(F_PREDICT, launch_predict_hash(s->x.cur(), s->v.cur(), kin, N, subdt, gstep, s->G,
+                                         s->sk_idx >= 0 ? s->sorted_keys : nullptr, s->ranges, st));
+    int *sk = nullptr, *pm = nullptr;
+    TIMED(F_SORT, launch_radix_sort(kin, kother, s->vals[0], s->vals[1], s->counts, s->totals, N, s->sort_bits, &sk,
+                                    &pm, st));
+    s->sorted_keys = sk;
+    s->perm = pm;
+    s->sk_idx = (sk == s->keys[0]) ? 0 : 1;
+    TIMED(F_RANGES, launch_find_ranges(sk, N, s->ranges, st));
+
+    const bool fuse_first = s->planes.n > 0 && !wind_on && !pick && iters > 0;
+    TIMED(F_REORDER, launch_reorder(pm, s->x.cur(), s->v.cur(), s->m.cur(), s->ph.cur(), s->orig.cur(), s->xs.cur(),
+                                    s->x.alt(), s->v.alt(), s->m.alt(), s->ph.alt(), s->orig.alt(), s->map_to_new,
+                                    N, subdt, gstep, P.mass_scaling_k, need_w ? 1 : 0, s->planes,
+                                    P.stabilization_passes, s->radius, fuse_first ? 1 : 0, fs, fd, st));
+    s->x.flip();
+    s->v.flip();
+    s->m.flip();
+    s->ph.flip();
+    s->orig.flip();
+
+    if (wind_on)
+        TIMED(F_WIND, launch_wind(s->xs.cur(), s->v.cur(), s->m.cur(), s->faces, s->map_to_new, s->fbuf, s->nf,
+                                  mk3(P.wind[0], P.wind[1], P.wind[2]), P.drag_coeff, P.lift_coeff, subdt,
+                                  s->fcsr.verts, s->fcsr.offs, s->fcsr.ents, s->fcsr.nv, st));
+
+    bool planes_done = fuse_first;
+    bool finished = false;
+    for (int it = 0; it < iters; ++it) {
+        if (pick) TIMED(F_PICK, launch_pick(s->xs.cur(), s->v.cur(), s->map_to_new, picked, ppos, 0, zero, st));
+        if (s->planes.n > 0 && !planes_done)
+            TIMED(F_PLANES, launch_plane_collide(s->xs.cur(), s->x.cur(), N, s->planes, s->radius, fs, fd, st));
+        planes_done = false;
+        if (s->has_solid) {
+            TIMED(F_PP, launch_particle_particle(s->xs.cur(), s->x.cur(), s->ranges, s->xs.alt(), N, s->G, s->radius,
+                                                 fs, fd, st));
+            s->xs.flip();
+        }
+        if (s->has_fluid) {
+            TIMED(F_LAMBDA, launch_lambda(s->xs.cur(), s->m.cur(), s->ranges, s->lambda, s->density, s->ccell, N,
+                                          s->G, s->K, s->F, st));
+            Epilogue E{};
+            E.planes_next = (s->planes.n > 0 && it + 1 < iters && !tail_groups && !pick) ? 1 : 0;
+            E.finish = (it + 1 == iters && !tail_groups) ? 1 : 0;
+            E.inv_dt = inv_dt;
+            E.sleep_eps2 = P.sleep_epsilon * P.sleep_epsilon;
+            E.radius = s->radius;
+            E.friction_static = fs;
+            E.friction_dynamic = fd;
+            TIMED(F_POSITION, launch_position(s->xs.cur(), s->m.cur(), s->lambda, s->ccell, sk, s->ranges,
+                                              s->xs.alt(), s->x.cur(), s->v.cur(), N, s->G, s->K, s->F, s->planes, E,
+                                              st));
+            s->xs.flip();
+            planes_done = E.planes_next;
+            finished = E.finish;
+        }
+        if (s->nd > 0)
+            TIMED(F_DISTANCE, launch_distance(s->xs.cur(), s->x.cur(), s->dpairs, s->dparams, s->map_to_new, s->kbuf,
+                                              s->nd, s->dcsr.verts, s->dcsr.offs, s->dcsr.ents, s->dcsr.nv, st));
+        if (s->nb > 0)
+            TIMED(F_BENDING, launch_bending(s->xs.cur(), s->x.cur(), s->bends, s->map_to_new, s->bbuf, s->nb,
+                                            P.bend_stiffness, s->bcsr.verts, s->bcsr.offs, s->bcsr.ents, s->bcsr.nv,
+                                            st));
+        if (s->nr > 0)
+            TIMED(F_SHAPE, launch_shape_matching(s->xs.cur(), s->map_to_new, s->rrest, s->rranges, s->rrot, s->rcm,
+                                                 s->nr, P.shape_matching_iterations, st));
+        if (s->ni > 0) TIMED(F_IMMOVABLE, launch_immovable(s->xs.cur(), s->x.cur(), s->immov, s->map_to_new, s->ni, st));
+    }
+    if (!finished)
+        TIMED(F_FINISH, launch_finish(s->xs.cur(), s->x.cur(), s->v.cur(), N, inv_dt,
+                                      P.sleep_epsilon * P.sleep_epsilon, st));
+
+    if (s->has_fluid) {
+        const int do_vort = P.vorticity_eps != 0.0f ? 1 : 0;
+        const int do_ak = P.akinci ? 1 : 0;
+        if (do_vort || do_ak) {
+            TIMED(F_POST_A, launch_post_a(s->xs.cur(), s->v.cur(), s->density, s->ranges, s->omega, s->normal,
+                                          s->ccell, N, s->G, s->K, do_vort, do_ak, st));
+            TIMED(F_POST_B, launch_post_b(s->xs.cur(), s->v.cur(), s->omega, s->normal, s->density, s->ranges,
+                                          s->ccell, N, s->G, s->K, s->F, do_vort, do_ak, subdt, st));
+        }
+        if (P.xsph_c != 0.0f) {
+            TIMED(F_XSPH, launch_xsph(s->xs.cur(), s->v.cur(), s->ranges, s->ccell, s->v.alt(), N, s->G, s->K,
+                                      P.xsph_c, (do_vort || do_ak) ? 1 : 0, st));
+            s->v.flip();
+        }
+    }
+    HIP_TRY(hipGetLastError());
+    return BP_OK;
+}
+
+bp_status collect_timing(bp_solver *s) {
+    if (s->recs.empty()) return BP_OK;
+    HIP_TRY(hipStreamSynchronize(s->st));
+    for (const auto &r : s->recs) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, r.a, r.b));
+        s->fam_ms[r.fam] += ms;
+        s->fam_n[r.fam] += 1;
+    }
+    s->recs.clear();
+    s->ev_next = 0;
+    return BP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bp_abi_version(void) { return BP_ABI_VERSION; }
+
+const char *bp_last_error_string(void) { return g_last_error.c_str(); }
+
+bp_status bp_default_params(bp_params *p) {
+    if (!p) return fail(BP_ERR_INVALID_ARG, "out is NULL");
+    std::memset(p, 0, sizeof(*p));
+    p->iterations = 2;
+    p->stabilization_passes = 2;
+    p->gravity[0] = 0.0f;
+    p->gravity[1] = -9.8f;
+    p->gravity[2] = 0.0f;
+    p->fluid_rest_density = 1200.0f;  // main.cu:223 preset value
+    p->solid_density_scale = 2.0f;
+    p->lambda_mode = 0;
+    p->lambda_denominator = 15000.0f;
+    p->lambda_relaxation = 300.0f;
+    p->akinci = 1;
+    p->scorr_k = 0.0001f;
+    p->scorr_n = 4;
+    p->scorr_dq = 0.03f;
+    p->delta_clamp = 5.0f;
+    p->vorticity_eps = 0.001f;
+    p->tension_gamma = 0.5f;
+    p->xsph_c = 0.0002f;
+    p->friction_static = 0.5f;
+    p->friction_dynamic = 0.0f;
+    p->mass_scaling_k = 2.0f;
+    p->sleep_epsilon = 0.0f;
+    p->wind_enabled = 1;
+    p->drag_coeff = 10.0f;
+    p->lift_coeff = 100.0f;
+    p->bend_stiffness = 0.8f;
+    p->shape_matching_iterations = 20;
+    p->nan_guard = 0;
+    return BP_OK;
+}
+
+bp_status bp_solver_create(const bp_scene_desc *sc, const bp_params *params, void *hip_stream, bp_solver **out) {
+    if (!out) return fail(BP_ERR_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    if (!sc) return fail(BP_ERR_INVALID_ARG, "scene is NULL");
+    bp_status st = check_params(params);
+    if (st != BP_OK) return st;
+    const int N = sc->num_particles;
+    if (N < 0) return fail(BP_ERR_INVALID_ARG, "num_particles < 0");
+    if (N > 0 && (!sc->positions || !sc->masses || !sc->phases))
+        return fail(BP_ERR_INVALID_ARG, "positions, masses and phases are required");
+    for (int a = 0; a < 3; ++a) {
+        if (!is_pow2(sc->grid_dims[a]) || sc->grid_dims[a] < 4 || sc->grid_dims[a] > 1024)
+            return fail(BP_ERR_INVALID_ARG, "grid_dims must be powers of two in [4, 1024]");
+        if (!(sc->cell_size[a] > 0.0f)) return fail(BP_ERR_INVALID_ARG, "cell_size must be > 0");
+    }
+    if (!(sc->kernel_radius > 0.0f)) return fail(BP_ERR_INVALID_ARG, "kernel_radius must be > 0");
+    if (sc->num_planes < 0 || sc->num_planes > kMaxPlanes)
+        return fail(BP_ERR_INVALID_ARG, "num_planes must be in [0, 16]");
+    if (sc->num_planes > 0 && (!sc->plane_origins || !sc->plane_normals))
+        return fail(BP_ERR_INVALID_ARG, "plane arrays are NULL");
+    auto ids_ok = [&](const int *ids, long long count) {
+        if (count > 0 && !ids) return false;
+        for (long long i = 0; i < count; ++i)
+            if (ids[i] < 0 || ids[i] >= N) return false;
+        return true;
+    };
+    if (sc->num_distance < 0 || !ids_ok(sc->distance_pairs, 2LL * sc->num_distance) ||
+        (sc->num_distance > 0 && !sc->distance_params))
+        return fail(BP_ERR_INVALID_ARG, "invalid distance constraints");
+    if (sc->num_bendings < 0 || !ids_ok(sc->bendings, 4LL * sc->num_bendings))
+        return fail(BP_ERR_INVALID_ARG, "invalid bending constraints");
+    if (sc->num_faces < 0 || !ids_ok(sc->faces, 3LL * sc->num_faces))
+        return fail(BP_ERR_INVALID_ARG, "invalid wind faces");
+    if (sc->num_immovables < 0 || !ids_ok(sc->immovables, sc->num_immovables))
+        return fail(BP_ERR_INVALID_ARG, "invalid immovable ids");
+    if (sc->num_rigid < 0) return fail(BP_ERR_INVALID_ARG, "num_rigid < 0");
+    for (int b = 0; b < sc->num_rigid; ++b) {
+        const int lo = sc->rigid_ranges[2 * b], hi = sc->rigid_ranges[2 * b + 1];
+        if (lo < 0 || hi <= lo || hi > N || hi > sc->rigid_rest_len || !sc->rigid_rest_positions)
+            return fail(BP_ERR_INVALID_ARG, "invalid rigid range or rest positions");
+        if (hi - lo > kMaxRigidParticles)
+            return fail(BP_ERR_CAPACITY, "rigid body has more than 64 particles "
+                                         "(NUM_MAX_PARTICLE_PER_RIGID_BODY)");
+    }
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(BP_ERR_NO_DEVICE, "no HIP device visible");
+    bp_solver *s = new bp_solver();
+    HIP_TRY(hipGetDevice(&s->device));
+    {
+        hipDeviceProp_t prop;
+        HIP_TRY(hipGetDeviceProperties(&prop, s->device));
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+            delete s;
+            return fail(BP_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", this build targets gfx950");
+        }
+    }
+    s->st = (hipStream_t)hip_stream;
+    s->P = *params;
+    s->N = N;
+    for (int a = 0; a < 3; ++a) {
+        s->G.cs[a] = sc->cell_size[a];
+        s->G.dims[a] = sc->grid_dims[a];
+        s->G.mask[a] = (uint32_t)sc->grid_dims[a] - 1u;
+    }
+    s->G.stride_x = (uint32_t)(sc->grid_dims[1] * sc->grid_dims[2]);
+    s->G.shift_y = (uint32_t)ilog2((unsigned)sc->grid_dims[2]);
+    s->G.shift_x = (uint32_t)ilog2((unsigned)(sc->grid_dims[1] * sc->grid_dims[2]));
+    s->num_cells = sc->grid_dims[0] * sc->grid_dims[1] * sc->grid_dims[2];
+    s->sort_bits = ilog2((unsigned)s->num_cells);
+    s->radius = sc->particle_radius;
+    set_kernel_const(s->K, sc->kernel_radius, params->scorr_dq);
+    set_derived_params(s);
+    s->planes.n = sc->num_planes;
+    for (int p = 0; p < sc->num_planes; ++p)
+        for (int a = 0; a < 3; ++a) {
+            s->planes.o[p][a] = sc->plane_origins[3 * p + a];
+            s->planes.nrm[p][a] = sc->plane_normals[3 * p + a];
+        }
+
+    s->mass_by_orig.assign(sc->masses, sc->masses + N);
+    s->phase_by_orig.assign(sc->phases, sc->phases + N);
+    s->group_by_orig.assign(N, 0);
+    if (sc->group_ids) s->group_by_orig.assign(sc->group_ids, sc->group_ids + N);
+    for (int i = 0; i < N; ++i) {
+        if (sc->phases[i] > 0) s->has_solid = true; else s->has_fluid = true;
+    }
+
+    // particle state, original order = initial slot order (initOrder_int, util.cuh:312-328)
+    std::vector<float4> hx(N), hv(N);
+    std::vector<int> iota(N);
+    for (int i = 0; i < N; ++i) {
+        hx[i] = make_float4(sc->positions[3 * i], sc->positions[3 * i + 1], sc->positions[3 * i + 2], 0.f);
+        if (sc->velocities)
+            hv[i] = make_float4(sc->velocities[3 * i], sc->velocities[3 * i + 1], sc->velocities[3 * i + 2], 0.f);
+        else
+            hv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        iota[i] = i;
+    }
+#define TRY(x)                 \
+    do {                       \
+        bp_status r_ = (x);    \
+        if (r_ != BP_OK) {     \
+            bp_solver_destroy(s); \
+            return r_;         \
+        }                      \
+    } while (0)
+    for (int k = 0; k < 2; ++k) {
+        TRY(s->alloc(&s->xs.b[k], N));
+        TRY(s->alloc(&s->x.b[k], N));
+        TRY(s->alloc(&s->v.b[k], N));
+        TRY(s->alloc(&s->m.b[k], N));
+        TRY(s->alloc(&s->ph.b[k], N));
+        TRY(s->alloc(&s->orig.b[k], N));
+        TRY(s->alloc(&s->keys[k], N));
+        TRY(s->alloc(&s->vals[k], N));
+    }
+    TRY(s->alloc(&s->map_to_new, N));
+    TRY(s->alloc(&s->counts, radix_counts_len(N)));
+    TRY(s->alloc(&s->totals, 256));
+    TRY(s->alloc(&s->ranges, s->num_cells));
+    TRY(s->alloc(&s->lambda, N));
+    TRY(s->alloc(&s->density, N));
+    TRY(s->alloc(&s->ccell, N));
+    TRY(s->alloc(&s->omega, N));
+    TRY(s->alloc(&s->normal, N));
+    if (N > 0) {
+        HIP_TRY(hipMemcpyAsync(s->x.cur(), hx.data(), N * sizeof(float4), hipMemcpyHostToDevice, s->st));
+        HIP_TRY(hipMemcpyAsync(s->v.cur(), hv.data(), N * sizeof(float4), hipMemcpyHostToDevice, s->st));
+        HIP_TRY(hipMemcpyAsync(s->m.cur(), sc->masses, N * sizeof(float), hipMemcpyHostToDevice, s->st));
+        HIP_TRY(hipMemcpyAsync(s->ph.cur(), sc->phases, N * sizeof(int), hipMemcpyHostToDevice, s->st));
+        HIP_TRY(hipMemcpyAsync(s->orig.cur(), iota.data(), N * sizeof(int), hipMemcpyHostToDevice, s->st));
+        HIP_TRY(hipMemcpyAsync(s->map_to_new, iota.data(), N * sizeof(int), hipMemcpyHostToDevice, s->st));
+        HIP_TRY(hipMemsetAsync(s->lambda, 0, N * sizeof(float), s->st));
+        HIP_TRY(hipMemsetAsync(s->density, 0, N * sizeof(float), s->st));
+        HIP_TRY(hipMemsetAsync(s->omega, 0, N * sizeof(float4), s->st));
+        HIP_TRY(hipMemsetAsync(s->normal, 0, N * sizeof(float4), s->st));
+    }
+    HIP_TRY(hipMemsetAsync(s->ranges, 0xFF, (size_t)s->num_cells * sizeof(int2), s->st));
+
+    // constraints (original ids)
+    s->nd = sc->num_distance;
+    if (s->nd > 0) {
+        TRY(s->upload(&s->dpairs, (const int2 *)sc->distance_pairs, s->nd));
+        TRY(s->upload(&s->dparams, (const float2 *)sc->distance_params, s->nd));
+        TRY(s->alloc(&s->kbuf, s->nd));
+        TRY(build_csr(s, s->dcsr, 2, sc->distance_pairs, s->nd, N, 1));
+    }
+    s->nb = sc->num_bendings;
+    if (s->nb > 0) {
+        TRY(s->upload(&s->bends, (const int4 *)sc->bendings, s->nb));
+        TRY(s->alloc(&s->bbuf, 4 * (size_t)s->nb));
+        TRY(build_csr(s, s->bcsr, 4, sc->bendings, s->nb, N, 1));
+    }
+    s->nf = sc->num_faces;
+    if (s->nf > 0) {
+        std::vector<int4> f4v(s->nf);
+        for (int f = 0; f < s->nf; ++f)
+            f4v[f] = make_int4(sc->faces[3 * f], sc->faces[3 * f + 1], sc->faces[3 * f + 2], 0);
+        TRY(s->upload(&s->faces, f4v.data(), s->nf));
+        TRY(s->alloc(&s->fbuf, s->nf));
+        TRY(build_csr(s, s->fcsr, 3, sc->faces, s->nf, N, 3));
+    }
+    s->nr = sc->num_rigid;
+    if (s->nr > 0) {
+        TRY(s->upload(&s->rranges, (const int2 *)sc->rigid_ranges, s->nr));
+        std::vector<float4> rest(sc->rigid_rest_len);
+        for (int i = 0; i < sc->rigid_rest_len; ++i)
+            rest[i] = make_float4(sc->rigid_rest_positions[3 * i], sc->rigid_rest_positions[3 * i + 1],
+                                  sc->rigid_rest_positions[3 * i + 2], 0.f);
+        TRY(s->upload(&s->rrest, rest.data(), rest.size()));
+        std::vector<float4> q(s->nr, make_float4(0.f, 0.f, 0.f, 1.f));
+        TRY(s->upload(&s->rrot, q.data(), q.size()));
+        TRY(s->alloc(&s->rcm, s->nr));
+        HIP_TRY(hipMemsetAsync(s->rcm, 0, s->nr * sizeof(float4), s->st));
+    }
+    s->ni = sc->num_immovables;
+    if (s->ni > 0) TRY(s->upload(&s->immov, sc->immovables, s->ni));
+#undef TRY
+    HIP_TRY(hipStreamSynchronize(s->st));
+    *out = s;
+    return BP_OK;
+}
+
+bp_status bp_solver_destroy(bp_solver *s) {
+    if (!s) return BP_OK;
+    if (s->st) (void)hipStreamSynchronize(s->st); else (void)hipDeviceSynchronize();
+    for (void *p : s->allocs) (void)hipFree(p);
+    for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
+    delete s;
+    return BP_OK;
+}
+
+bp_status bp_solver_set_params(bp_solver *s, const bp_params *p) {
+    if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
+    bp_status st = check_params(p);
+    if (st != BP_OK) return st;
+    s->P = *p;
+    set_kernel_const(s->K, s->K.h, p->scorr_dq);
+    set_derived_params(s);
+    return BP_OK;
+}
+
+bp_status bp_solver_step(bp_solver *s, int num_substeps, float dt, int picked, const float *ppos,
+                         const float *pvel) {
+    if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
+    if (num_substeps <= 0) return fail(BP_ERR_INVALID_ARG, "num_substeps must be > 0");
+    if (s->N <= 0) return BP_OK;  // particlesolver.cuh:778
+    const float subdt = dt / (float)num_substeps;  // :780
+    const float inv_dt = 1.0f / subdt;             // :1000
+    // make_float3(0,-9.8,0) * dt * dt (particlesolver.cuh:56), per component
+    const f3 gstep = mk3(s->P.gravity[0] * subdt * subdt, s->P.gravity[1] * subdt * subdt,
+                         s->P.gravity[2] * subdt * subdt);
+    const bool pick = picked >= 0 && picked < s->N;  // :790
+    if (pick && !ppos) return fail(BP_ERR_INVALID_ARG, "picked position is NULL");
+    const f3 pp = pick ? mk3(ppos[0], ppos[1], ppos[2]) : mk3(0.f, 0.f, 0.f);
+    for (int k = 0; k < num_substeps; ++k) {
+        bp_status st = substep(s, subdt, inv_dt, gstep, pick, picked, pp);
+        if (st != BP_OK) return st;
+    }
+    if (pick) {  // :1066-1071: keep the solved position, set the picked velocity
+        const f3 pv = pvel ? mk3(pvel[0], pvel[1], pvel[2]) : mk3(0.f, 0.f, 0.f);
+        TIMED(F_PICK, launch_pick(s->x.cur(), s->v.cur(), s->map_to_new, picked, pp, 1, pv, s->st));
+    }
+    HIP_TRY(hipGetLastError());
+    if (s->P.nan_guard) {
+        std::vector<float4> hx(s->N);
+        HIP_TRY(hipMemcpyAsync(hx.data(), s->x.cur(), s->N * sizeof(float4), hipMemcpyDeviceToHost, s->st));
+        HIP_TRY(hipStreamSynchronize(s->st));
+        for (int i = 0; i < s->N; ++i)
+            if (!std::isfinite(hx[i].x) || !std::isfinite(hx[i].y) || !std::isfinite(hx[i].z))
+                return fail(BP_ERR_NOT_FINITE, "non-finite position at slot " + std::to_string(i));
+    }
+    return BP_OK;
+}
+
+bp_status bp_solver_sync(bp_solver *s) {
+    if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return BP_OK;
+}
+
+namespace {
+bp_status read_int(bp_solver *s, const int *dev, int idx, int *out) {
+    HIP_TRY(hipMemcpyAsync(out, dev + idx, sizeof(int), hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return BP_OK;
+}
+}  // namespace
+
+bp_status bp_query_original_id(bp_solver *s, int slot, int *out) {
+    if (!s || !out) return fail(BP_ERR_INVALID_ARG, "NULL argument");
+    if (slot < 0 || slot >= s->N) return fail(BP_ERR_INVALID_ARG, "slot out of range");
+    return read_int(s, s->orig.cur(), slot, out);
+}
+
+bp_status bp_query_slot(bp_solver *s, int original_id, int *out) {
+    if (!s || !out) return fail(BP_ERR_INVALID_ARG, "NULL argument");
+    if (original_id < 0 || original_id >= s->N) return fail(BP_ERR_INVALID_ARG, "original id out of range");
+    return read_int(s, s->map_to_new, original_id, out);
+}
+
+bp_status bp_get_particle(bp_solver *s, int slot, float out_pos[3]) {
+    if (!s || !out_pos) return fail(BP_ERR_INVALID_ARG, "NULL argument");
+    if (slot < 0 || slot >= s->N) {  // particlesolver.cuh:380 returns vec3(0)
+        out_pos[0] = out_pos[1] = out_pos[2] = 0.f;
+        return BP_OK;
+    }
+    float4 a;
+    HIP_TRY(hipMemcpyAsync(&a, s->x.cur() + slot, sizeof(float4), hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    out_pos[0] = a.x; out_pos[1] = a.y; out_pos[2] = a.z;
+    return BP_OK;
+}
+
+bp_status bp_set_particle(bp_solver *s, int slot, const float pos[3], const float vel[3]) {
+    if (!s || !pos || !vel) return fail(BP_ERR_INVALID_ARG, "NULL argument");
+    if (slot < 0 || slot >= s->N) return BP_OK;  // particlesolver.cuh:390
+    float4 a, b;
+    HIP_TRY(hipMemcpyAsync(&a, s->x.cur() + slot, sizeof(float4), hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    a.x = pos[0]; a.y = pos[1]; a.z = pos[2];
+    b = make_float4(vel[0], vel[1], vel[2], 0.f);
+    HIP_TRY(hipMemcpyAsync(s->x.cur() + slot, &a, sizeof(float4), hipMemcpyHostToDevice, s->st));
+    HIP_TRY(hipMemcpyAsync(s->v.cur() + slot, &b, sizeof(float4), hipMemcpyHostToDevice, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return BP_OK;
+}
+
+bp_status bp_get_device_arrays(bp_solver *s, bp_device_view *o) {
+    if (!s || !o) return fail(BP_ERR_INVALID_ARG, "NULL argument");
+    o->num_particles = s->N;
+    o->new_positions = (float *)s->xs.cur();
+    o->positions = (float *)s->x.cur();
+    o->velocities = (float *)s->v.cur();
+    o->phases = s->ph.cur();
+    o->map_to_original = s->orig.cur();
+    o->map_to_new = s->map_to_new;
+    o->rigid_rotations = (float *)s->rrot;
+    o->rigid_cms = (float *)s->rcm;
+    return BP_OK;
+}
+
+bp_status bp_get_state(bp_solver *s, float *pos, float *vel, int *map) {
+    if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
+    const int N = s->N;
+    std::vector<float4> hx(N), hv(N);
+    if (N > 0) {
+        HIP_TRY(hipMemcpyAsync(hx.data(), s->x.cur(), N * sizeof(float4), hipMemcpyDeviceToHost, s->st));
+        HIP_TRY(hipMemcpyAsync(hv.data(), s->v.cur(), N * sizeof(float4), hipMemcpyDeviceToHost, s->st));
+        if (map) HIP_TRY(hipMemcpyAsync(map, s->orig.cur(), N * sizeof(int), hipMemcpyDeviceToHost, s->st));
+    }
+    HIP_TRY(hipStreamSynchronize(s->st));
+    for (int i = 0; i < N; ++i) {
+        if (pos) { pos[3 * i] = hx[i].x; pos[3 * i + 1] = hx[i].y; pos[3 * i + 2] = hx[i].z; }
+        if (vel) { vel[3 * i] = hv[i].x; vel[3 * i + 1] = hv[i].y; vel[3 * i + 2] = hv[i].z; }
+    }
+    return BP_OK;
+}
+
+bp_status bp_copy_state_original_order(bp_solver *s, float *pos, float *vel) {
+    if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
+    const int N = s->N;
+    std::vector<float> p(3 * (size_t)N), v(3 * (size_t)N);
+    std::vector<int> map(N);
+    bp_status st = bp_get_state(s, p.data(), v.data(), map.data());
+    if (st != BP_OK) return st;
+    for (int i = 0; i < N; ++i) {
+        const int o = map[i];
+        for (int a = 0; a < 3; ++a) {
+            if (pos) pos[3 * o + a] = p[3 * i + a];
+            if (vel) vel[3 * o + a] = v[3 * i + a];
+        }
+    }
+    return BP_OK;
+}
+
+bp_status bp_set_state(bp_solver *s, const float *pos, const float *vel, const int *map) {
+    if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
+    const int N = s->N;
+    if (N == 0) return BP_OK;
+    std::vector<float4> hx(N), hv(N);
+    std::vector<int> o(N), inv(N, -1);
+    std::vector<float> hm(N);
+    std::vector<int> hp(N);
+    HIP_TRY(hipMemcpyAsync(hx.data(), s->x.cur(), N * sizeof(float4), hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipMemcpyAsync(hv.data(), s->v.cur(), N * sizeof(float4), hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipMemcpyAsync(o.data(), s->orig.cur(), N * sizeof(int), hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    if (map) {
+        for (int i = 0; i < N; ++i) {
+            if (map[i] < 0 || map[i] >= N || inv[map[i]] >= 0)
+                return fail(BP_ERR_INVALID_ARG, "map_to_original is not a permutation");
+            inv[map[i]] = i;
+            o[i] = map[i];
+        }
+    }
+    for (int i = 0; i < N; ++i) {
+        if (pos) hx[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], hx[i].w);
+        if (vel) hv[i] = make_float4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 0.f);
+        hm[i] = s->mass_by_orig[o[i]];
+        hp[i] = s->phase_by_orig[o[i]];
+        inv[o[i]] = i;
+    }
+    HIP_TRY(hipMemcpyAsync(s->x.cur(), hx.data(), N * sizeof(float4), hipMemcpyHostToDevice, s->st));
+    HIP_TRY(hipMemcpyAsync(s->v.cur(), hv.data(), N * sizeof(float4), hipMemcpyHostToDevice, s->st));
+    HIP_TRY(hipMemcpyAsync(s->orig.cur(), o.data(), N * sizeof(int), hipMemcpyHostToDevice, s->st));
+    HIP_TRY(hipMemcpyAsync(s->map_to_new, inv.data(), N * sizeof(int), hipMemcpyHostToDevice, s->st));
+    HIP_TRY(hipMemcpyAsync(s->m.cur(), hm.data(), N * sizeof(float), hipMemcpyHostToDevice, s->st));
+    HIP_TRY(hipMemcpyAsync(s->ph.cur(), hp.data(), N * sizeof(int), hipMemcpyHostToDevice, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return BP_OK;
+}
+
+bp_status bp_get_rigid_state(bp_solver *s, float *quats, float *cms) {
+    if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
+    if (s->nr == 0) return BP_OK;
+    std::vector<float4> q(s->nr), c(s->nr);
+    HIP_TRY(hipMemcpyAsync(q.data(), s->rrot, s->nr * sizeof(float4), hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipMemcpyAsync(c.data(), s->rcm, s->nr * sizeof(float4), hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    for (int b = 0; b < s->nr; ++b) {
+        if (quats) { quats[4 * b] = q[b].x; quats[4 * b + 1] = q[b].y; quats[4 * b + 2] = q[b].z; quats[4 * b + 3] = q[b].w; }
+        if (cms) { cms[3 * b] = c[b].x; cms[3 * b + 1] = c[b].y; cms[3 * b + 2] = c[b].z; }
+    }
+    return BP_OK;
+}
+
+bp_status bp_set_rigid_state(bp_solver *s, const float *quats) {
+    if (!s || !quats) return fail(BP_ERR_INVALID_ARG, "NULL argument");
+    if (s->nr == 0) return BP_OK;
+    std::vector<float4> q(s->nr);
+    for (int b = 0; b < s->nr; ++b) q[b] = make_float4(quats[4 * b], quats[4 * b + 1], quats[4 * b + 2], quats[4 * b + 3]);
+    HIP_TRY(hipMemcpyAsync(s->rrot, q.data(), s->nr * sizeof(float4), hipMemcpyHostToDevice, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return BP_OK;
+}
+
+bp_status bp_debug_fetch(bp_solver *s, int which, void *out) {
+    if (!s || !out) return fail(BP_ERR_INVALID_ARG, "NULL argument");
+    const int N = s->N;
+    if (N == 0) return BP_OK;
+    switch (which) {
+    case 0: HIP_TRY(hipMemcpyAsync(out, s->density, N * sizeof(float), hipMemcpyDeviceToHost, s->st)); break;
+    case 1: HIP_TRY(hipMemcpyAsync(out, s->lambda, N * sizeof(float), hipMemcpyDeviceToHost, s->st)); break;
+    case 2:
+    case 3: {
+        const int *src = (which == 2) ? s->sorted_keys : s->perm;
+        if (!src) return fail(BP_ERR_INVALID_ARG, "no substep has run yet");
+        HIP_TRY(hipMemcpyAsync(out, src, N * sizeof(int), hipMemcpyDeviceToHost, s->st));
+        break;
+    }
+    case 4:
+    case 5: {
+        std::vector<float4> t(N);
+        HIP_TRY(hipMemcpyAsync(t.data(), which == 4 ? s->omega : s->normal, N * sizeof(float4),
+                               hipMemcpyDeviceToHost, s->st));
+        HIP_TRY(hipStreamSynchronize(s->st));
+        float *o = (float *)out;
+        for (int i = 0; i < N; ++i) { o[3 * i] = t[i].x; o[3 * i + 1] = t[i].y; o[3 * i + 2] = t[i].z; }
+        break;
+    }
+    default: return fail(BP_ERR_INVALID_ARG, "unknown debug buffer");
+    }
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return BP_OK;
+}
+
+bp_status bp_neighbor_grid(const float *xs, int n, const float cell_size[3], const int grid_dims[3], int *keys,
+                           int *perm, int *cell_start, int *cell_end) {
+    if (n < 0 || (n > 0 && !xs) || !cell_size || !grid_dims) return fail(BP_ERR_INVALID_ARG, "invalid arguments");
+    GridDesc G{};
+    for (int a = 0; a < 3; ++a) {
+        if (!is_pow2(grid_dims[a]) || grid_dims[a] < 4 || grid_dims[a] > 1024)
+            return fail(BP_ERR_INVALID_ARG, "grid_dims must be powers of two in [4, 1024]");
+        G.cs[a] = cell_size[a];
+        G.dims[a] = grid_dims[a];
+        G.mask[a] = (uint32_t)grid_dims[a] - 1u;
+    }
+    G.stride_x = (uint32_t)(grid_dims[1] * grid_dims[2]);
+    G.shift_y = (uint32_t)ilog2((unsigned)grid_dims[2]);
+    G.shift_x = (uint32_t)ilog2((unsigned)(grid_dims[1] * grid_dims[2]));
+    const int cells = grid_dims[0] * grid_dims[1] * grid_dims[2];
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(BP_ERR_NO_DEVICE, "no HIP device visible");
+    std::vector<float4> hx(n);
+    for (int i = 0; i < n; ++i) hx[i] = make_float4(xs[3 * i], xs[3 * i + 1], xs[3 * i + 2], 0.f);
+    float4 *dx = nullptr, *dv = nullptr;
+    int *k[2] = {nullptr, nullptr}, *vv[2] = {nullptr, nullptr}, *counts = nullptr, *totals = nullptr;
+    int2 *ranges = nullptr;
+    const size_t nn = n > 0 ? (size_t)n : 1;
+    hipError_t e = hipSuccess;
+    auto A = [&](void **p, size_t bytes) { if (e == hipSuccess) e = hipMalloc(p, bytes); };
+    A((void **)&dx, nn * sizeof(float4));
+    A((void **)&dv, nn * sizeof(float4));
+    A((void **)&k[0], nn * sizeof(int));
+    A((void **)&k[1], nn * sizeof(int));
+    A((void **)&vv[0], nn * sizeof(int));
+    A((void **)&vv[1], nn * sizeof(int));
+    A((void **)&counts, radix_counts_len(n > 0 ? n : 1) * sizeof(int));
+    A((void **)&totals, 256 * sizeof(int));
+    A((void **)&ranges, (size_t)cells * sizeof(int2));
+    bp_status result = BP_OK;
+    auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
+    if (e == hipSuccess) {
+        hipStream_t st = nullptr;
+        int *sk = nullptr, *pm = nullptr;
+        if (n > 0) {
+            chk(hipMemcpyAsync(dx, hx.data(), n * sizeof(float4), hipMemcpyHostToDevice, st));
+            chk(hipMemsetAsync(dv, 0, n * sizeof(float4), st));
+        }
+        chk(hipMemsetAsync(ranges, 0xFF, (size_t)cells * sizeof(int2), st));
+        if (n > 0) {
+            // x* = x + 0*0 + 0 is x exactly: the hash sees the given positions
+            launch_predict_hash(dx, dv, k[0], n, 0.0f, mk3(0.f, 0.f, 0.f), G, nullptr, ranges, st);
+            launch_radix_sort(k[0], k[1], vv[0], vv[1], counts, totals, n, ilog2((unsigned)cells), &sk, &pm, st);
+            launch_find_ranges(sk, n, ranges, st);
+            if (keys) chk(hipMemcpyAsync(keys, sk, n * sizeof(int), hipMemcpyDeviceToHost, st));
+            if (perm) chk(hipMemcpyAsync(perm, pm, n * sizeof(int), hipMemcpyDeviceToHost, st));
+        }
+        std::vector<int2> hr;
+        if (cell_start || cell_end) {
+            hr.resize(cells);
+            chk(hipMemcpyAsync(hr.data(), ranges, (size_t)cells * sizeof(int2), hipMemcpyDeviceToHost, st));
+        }
+        chk(hipGetLastError());
+        chk(hipStreamSynchronize(st));
+        for (int c = 0; c < (int)hr.size(); ++c) {
+            if (cell_start) cell_start[c] = hr[c].x;
+            if (cell_end) cell_end[c] = hr[c].y;
+        }
+    }
+    if (e != hipSuccess) result = fail(BP_ERR_HIP, std::string("neighbor grid: ") + hipGetErrorString(e));
+    for (void *p : {(void *)dx, (void *)dv, (void *)k[0], (void *)k[1], (void *)vv[0], (void *)vv[1],
+                    (void *)counts, (void *)totals, (void *)ranges})
+        if (p) (void)hipFree(p);
+    return result;
+}
+
+bp_status bp_solver_timed_substeps(bp_solver *s, int substeps, float dt, float *out_ms) {
+    if (!s || !out_ms) return fail(BP_ERR_INVALID_ARG, "NULL argument");
+    if (substeps <= 0) return fail(BP_ERR_INVALID_ARG, "substeps must be > 0");
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    const f3 gstep = mk3(s->P.gravity[0] * dt * dt, s->P.gravity[1] * dt * dt, s->P.gravity[2] * dt * dt);
+    const float inv_dt = 1.0f / dt;
+    HIP_TRY(hipEventRecord(a, s->st));
+    for (int k = 0; k < substeps && s->N > 0; ++k) {
+        bp_status st = substep(s, dt, inv_dt, gstep, false, -1, mk3(0.f, 0.f, 0.f));
+        if (st != BP_OK) return st;
+    }
+    HIP_TRY(hipEventRecord(b, s->st));
+    HIP_TRY(hipEventSynchronize(b));
+    HIP_TRY(hipEventElapsedTime(out_ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return collect_timing(s);
+}
+
+bp_status bp_solver_enable_kernel_timing(bp_solver *s, int enable) {
+    if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
+    s->timing = enable != 0;
+    if (s->timing && s->ev_pool.empty()) {
+        s->ev_pool.resize(8192);
+        for (auto &e : s->ev_pool) HIP_TRY(hipEventCreate(&e));
+    }
+    for (int f = 0; f < F_COUNT; ++f) { s->fam_ms[f] = 0.f; s->fam_n[f] = 0; }
+    s->recs.clear();
+    s->ev_next = 0;
+    return BP_OK;
+}
+
+bp_status bp_solver_kernel_times(bp_solver *s, int max, char *names, float *total_ms, int *launches, int *count) {
+    if (!s || !count) return fail(BP_ERR_INVALID_ARG, "NULL argument");
+    bp_status st = collect_timing(s);
+    if (st != BP_OK) return st;
+    int c = 0;
+    for (int f = 0; f < F_COUNT; ++f) {
+        if (s->fam_n[f] == 0) continue;
+        if (c < max) {
+            if (names) std::snprintf(names + 32 * c, 32, "%s", kFamilyNames[f]);
+            if (total_ms) total_ms[c] = s->fam_ms[f];
+            if (launches) launches[c] = s->fam_n[f];
+        }
+        ++c;
+    }
+    *count = c;
+    return BP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,204 @@
+/*
+ * brasspan.h -- C ABI of the MI355X (gfx950) particle solver substep.
+ *
+ * This is the drop-in boundary for brass-pan's `ParticleSolver` / `Scene`
+ * pair.  Every entry point below replaces one member of the reference's
+ * `struct ParticleSolver` (brasstower/particlesolver.cuh) and is called by
+ * the C++ shim in brass-pan_amd/cpp/brasspan/particlesolver.h, which keeps the
+ * reference's class and method names.  Plain pointers and sizes only: no HIP,
+ * torch or glm types cross this boundary (the stream is passed as void*).
+ *
+ * Conventions
+ *  - Vectors are packed float[3] (or int[2] / int[3] / int[4]) in host memory.
+ *  - "slot" = index of a particle in the solver's current sorted order (the
+ *    reference's "new id"); "original id" = index in the scene description.
+ *  - Every call returns bp_status; on failure bp_last_error_string() gives a
+ *    message.  The reference prints and continues (cuda/helper.cuh:282-290)
+ *    or throws (particlesolver.cuh:398-402); this ABI reports instead.
+ *  - One host thread per solver handle.  Different handles are independent.
+ */
+#ifndef BRASSPAN_H_
+#define BRASSPAN_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BP_ABI_VERSION 1
+
+typedef enum bp_status {
+    BP_OK = 0,
+    BP_ERR_INVALID_ARG = 1,   /* bad pointer, size or parameter           */
+    BP_ERR_CAPACITY = 2,      /* e.g. rigid body with more than 64 particles */
+    BP_ERR_HIP = 3,           /* a HIP runtime call failed                */
+    BP_ERR_NOT_FINITE = 4,    /* NaN/Inf found by the optional guard      */
+    BP_ERR_NO_DEVICE = 5,     /* no gfx950 device visible                 */
+    BP_ERR_UNSUPPORTED = 6
+} bp_status;
+
+/*
+ * Solver parameters.  bp_default_params() fills every field with the
+ * literal the reference hard-codes; the comment gives where.
+ */
+typedef struct bp_params {
+    int   iterations;              /* 2: particlesolver.cuh:862                  */
+    int   stabilization_passes;    /* 2: particlesolver.cuh:822                  */
+    float gravity[3];              /* (0,-9.8,0): particlesolver.cuh:56          */
+    float fluid_rest_density;      /* OldSceneFormat::fluidRestDensity (:338)    */
+    float solid_density_scale;     /* 2.0: particlesolver.cuh:909                */
+    int   lambda_mode;             /* 0 = constant denominator (fluid.cuh:76,153),
+                                      1 = -C/(sum|grad C|^2 + eps) (fluid.cuh:155) */
+    float lambda_denominator;      /* 15000: fluid.cuh:153                       */
+    float lambda_relaxation;       /* 300 (eps of mode 1): particlesolver.cuh:910 */
+    int   akinci;                  /* 1: particlesolver.cuh:785 (clamps C>=0, turns
+                                      sCorr off, runs normal + tension passes)   */
+    float scorr_k;                 /* 1e-4: particlesolver.cuh:926               */
+    int   scorr_n;                 /* 4:    particlesolver.cuh:927               */
+    float scorr_dq;                /* 0.03 (x h): fluid.cuh:208                  */
+    float delta_clamp;             /* 5.0: fluid.cuh:214                         */
+    float vorticity_eps;           /* 0.001: particlesolver.cuh:1017 (0 = skip)  */
+    float tension_gamma;           /* 0.5: particlesolver.cuh:1043               */
+    float xsph_c;                  /* 0.0002: particlesolver.cuh:1055 (0 = skip) */
+    float friction_static;         /* 0.5: cubrasstower.cuh:18                   */
+    float friction_dynamic;        /* 0.0: cubrasstower.cuh:19                   */
+    float mass_scaling_k;          /* 2: cubrasstower.cuh:20                     */
+    float sleep_epsilon;           /* 0: cubrasstower.cuh:21                     */
+    int   wind_enabled;            /* 1: particlesolver.cuh:838-858              */
+    float wind[3];                 /* (0,0,0): the reference has no wind vector  */
+    float drag_coeff;              /* 10: wind.cuh:40                            */
+    float lift_coeff;              /* 100: wind.cuh:43                           */
+    float bend_stiffness;          /* 0.8: bending.cuh:65                        */
+    int   shape_matching_iterations; /* 20: shapematching.cuh:72                */
+    int   nan_guard;               /* 0: optional non-finite check after a step
+                                      (reference CHECK_NAN, cubrasstower.cuh:15) */
+} bp_params;
+
+/*
+ * Scene description.  Replaces `struct Scene` (scene.h:455-536) on the C side.
+ * Host pointers; the solver copies everything at create time and keeps no
+ * pointer afterwards (the reference copies once too, particlesolver.cuh:158-214).
+ */
+typedef struct bp_scene_desc {
+    int          num_particles;
+    const float *positions;        /* [3N] scene.h:507                          */
+    const float *velocities;       /* [3N] or NULL = zero (particlesolver.cuh:167) */
+    const float *masses;           /* [N]  scene.h:508                          */
+    const int   *phases;           /* [N]  scene.h:509: fluid < 0, solid > 0    */
+    const int   *group_ids;        /* [N] or NULL (render-only)                 */
+    float        particle_radius;  /* solidRadius, scene.h:500                  */
+    float        kernel_radius;    /* h, scene.h:501                            */
+    float        cell_size[3];     /* particlesolver.cuh:150                    */
+    int          grid_dims[3];     /* per-axis powers of two (reference: 512^3) */
+    int          num_planes;       /* scene.h:513                               */
+    const float *plane_origins;    /* [3*num_planes]                            */
+    const float *plane_normals;    /* [3*num_planes]                            */
+    int          num_distance;     /* scene.h:525-526                           */
+    const int   *distance_pairs;   /* [2*nd] original ids                       */
+    const float *distance_params;  /* [2*nd] (rest length, stiffness)           */
+    int          num_bendings;     /* scene.h:528                               */
+    const int   *bendings;         /* [4*nb] original ids                       */
+    int          num_faces;        /* scene.h:516                               */
+    const int   *faces;            /* [3*nf] original ids                        */
+    int          num_rigid;        /* scene.h:522                               */
+    const int   *rigid_ranges;     /* [2*nr] [begin,end) original ids, <=64 each */
+    int          rigid_rest_len;   /* number of entries in rigid_rest_positions */
+    const float *rigid_rest_positions; /* [3*rigid_rest_len] indexed by original
+                                      id, centre of mass at 0 (scene.h:521)     */
+    int          num_immovables;   /* scene.h:530                               */
+    const int   *immovables;       /* [ni] original ids                         */
+} bp_scene_desc;
+
+/* Device view of the solver state, sorted order (bp_get_device_arrays). */
+typedef struct bp_device_view {
+    int    num_particles;
+    float *new_positions;  /* float4[N]: x* (xyz, w = mass)                    */
+    float *positions;      /* float4[N]: x  (xyz, w = inverse scaled mass)     */
+    float *velocities;     /* float4[N]: v  (xyz, w = 0)                       */
+    int   *phases;         /* int[N]                                           */
+    int   *map_to_original;/* int[N]: slot -> original id                      */
+    int   *map_to_new;     /* int[N]: original id -> slot                      */
+    float *rigid_rotations;/* float4[nr] quaternions (x,y,z,w)                 */
+    float *rigid_cms;      /* float4[nr]                                       */
+} bp_device_view;
+
+typedef struct bp_solver bp_solver;
+
+/* Library + parameter helpers ------------------------------------------- */
+int         bp_abi_version(void);
+const char *bp_last_error_string(void);
+/* Fills the reference literals (see bp_params). */
+bp_status   bp_default_params(bp_params *out);
+
+/* Lifetime ----------------------------------------------------------------- */
+/* Replaces ParticleSolver(const std::shared_ptr<Scene>&), particlesolver.cuh:148-215.
+ * `hip_stream` is a hipStream_t or NULL (the null stream). */
+bp_status bp_solver_create(const bp_scene_desc *scene, const bp_params *params,
+                           void *hip_stream, bp_solver **out);
+/* The reference has no destructor (particlesolver.cuh:1074). */
+bp_status bp_solver_destroy(bp_solver *s);
+
+/* Step --------------------------------------------------------------------- */
+/* Replaces ParticleSolver::update(numSubTimeStep, deltaTime, pickedOriginalParticleId,
+ * pickedParticlePosition, pickedParticleVelocity), particlesolver.cuh:772-1072.
+ * `picked_pos` / `picked_vel` may be NULL when picked_original_id < 0.
+ * Asynchronous on the solver's stream. */
+bp_status bp_solver_step(bp_solver *s, int num_substeps, float dt,
+                         int picked_original_id, const float *picked_pos,
+                         const float *picked_vel);
+/* Waits for the solver's stream. */
+bp_status bp_solver_sync(bp_solver *s);
+/* Changes parameters between steps (same scene). */
+bp_status bp_solver_set_params(bp_solver *s, const bp_params *params);
+
+/* Queries ------------------------------------------------------------------ */
+/* queryOriginalParticleId, particlesolver.cuh:352-358 */
+bp_status bp_query_original_id(bp_solver *s, int slot, int *out_original_id);
+/* queryNewParticleId, particlesolver.cuh:360-366 */
+bp_status bp_query_slot(bp_solver *s, int original_id, int *out_slot);
+/* getParticlePosition, particlesolver.cuh:378-386 (slot index) */
+bp_status bp_get_particle(bp_solver *s, int slot, float out_pos[3]);
+/* setParticle, particlesolver.cuh:388-393 (slot index; sets x and v) */
+bp_status bp_set_particle(bp_solver *s, int slot, const float pos[3], const float vel[3]);
+/* Public device members devPositions etc., particlesolver.cuh:1075-1148 */
+bp_status bp_get_device_arrays(bp_solver *s, bp_device_view *out);
+/* Headless export in original-id order: pos/vel [3N] host, either may be NULL. */
+bp_status bp_copy_state_original_order(bp_solver *s, float *pos, float *vel);
+/* Full state in slot order, host buffers.  pos/vel [3N]; map [N] slot -> original id. */
+bp_status bp_get_state(bp_solver *s, float *pos, float *vel, int *map_to_original);
+bp_status bp_set_state(bp_solver *s, const float *pos, const float *vel,
+                       const int *map_to_original);
+/* Rigid-body warm-start state: quats [4*nr] (x,y,z,w), cms [3*nr]; either may be NULL. */
+bp_status bp_get_rigid_state(bp_solver *s, float *quats, float *cms);
+bp_status bp_set_rigid_state(bp_solver *s, const float *quats);
+
+/* Intermediate buffers of the last substep, slot order, for parity checks.
+ * which: 0 = densities [N], 1 = lambdas [N], 2 = sorted cell keys [N],
+ *        3 = sort permutation (previous slot of each new slot) [N],
+ *        4 = omegas [3N], 5 = normals [3N]. */
+bp_status bp_debug_fetch(bp_solver *s, int which, void *host_out);
+
+/* Stand-alone neighbour grid (updateGrid, particlesolver.cuh:712-765):
+ * hash (grid.cuh:30-47), stable sort (particlesolver.cuh:721-738) and cell
+ * ranges (grid.cuh:70-99) of n positions xs[3n] (host).  Outputs (host):
+ * keys[n] sorted cell ids, perm[n] sorted slot values, cell_start/cell_end
+ * [dims product] (-1 for empty cells).  Any output may be NULL. */
+bp_status bp_neighbor_grid(const float *xs, int n, const float cell_size[3],
+                           const int grid_dims[3], int *keys, int *perm,
+                           int *cell_start, int *cell_end);
+
+/* Timing helper for benchmarks: runs `substeps` substeps of size dt and
+ * returns the elapsed GPU time in ms measured with HIP events on the
+ * solver's stream (bracketing all launches). */
+bp_status bp_solver_timed_substeps(bp_solver *s, int substeps, float dt, float *out_ms);
+
+/* Per-kernel timing over the last bp_solver_timed_substeps call when
+ * profiling is enabled with bp_solver_enable_kernel_timing(s, 1): names and
+ * total ms per kernel family, `max` entries.  Returns the count in *count. */
+bp_status bp_solver_enable_kernel_timing(bp_solver *s, int enable);
+bp_status bp_solver_kernel_times(bp_solver *s, int max, char *names /*max*32*/,
+                                 float *total_ms, int *launches, int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BRASSPAN_H_ */

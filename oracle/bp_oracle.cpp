@@ -1,0 +1,1008 @@
+// bp_oracle.cpp -- CPU oracle of the brass-pan particle-solver substep.
+//
+// TEST INFRASTRUCTURE ONLY (tests/, __graft_entry__.smoke(), bench.py
+// cpu_baseline).  Never linked into the product library.
+//
+// Each function cites the reference file:line it restates.  Operation order
+// follows the reference source (left-to-right float3 operators of
+// ext/helper_math.h); build with -ffp-contract=off so no FMA contraction
+// changes the FP32 bits.
+#include "bp_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+namespace {
+
+template <class R> struct V3 {
+    R x, y, z;
+};
+template <class R> inline V3<R> mk(R x, R y, R z) { return V3<R>{x, y, z}; }
+template <class R> inline V3<R> operator+(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class R> inline V3<R> operator-(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class R> inline V3<R> operator-(V3<R> a) { return {-a.x, -a.y, -a.z}; }
+template <class R> inline V3<R> operator*(V3<R> a, R s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class R> inline V3<R> operator*(R s, V3<R> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class R> inline V3<R> operator/(V3<R> a, R s) { return {a.x / s, a.y / s, a.z / s}; }
+template <class R> inline V3<R> &operator+=(V3<R> &a, V3<R> b) { a = a + b; return a; }
+template <class R> inline V3<R> &operator-=(V3<R> &a, V3<R> b) { a = a - b; return a; }
+// helper_math.h dot/length2/length/cross semantics (:1253,1308,1296,1437)
+template <class R> inline R dot(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class R> inline R length2(V3<R> a) { return dot(a, a); }
+template <class R> inline R length(V3<R> a) { return std::sqrt(dot(a, a)); }
+template <class R> inline V3<R> cross(V3<R> a, V3<R> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// device normalize = v * rsqrt(dot(v,v)) (helper_math.h:1326-1330); host uses 1/sqrt.
+template <class R> inline V3<R> normalize(V3<R> v) { R inv = R(1) / std::sqrt(dot(v, v)); return v * inv; }
+
+template <class R> struct Q4 {
+    R x, y, z, w;
+};
+
+// fminf / fmaxf semantics for the reference's float min/max (Q11).
+template <class R> inline R fmin_(R a, R b) { return std::fmin(a, b); }
+template <class R> inline R fmax_(R a, R b) { return std::fmax(a, b); }
+
+// ---------------------------------------------------------------- grid hash
+// calcGridPos (grid.cuh:9-13): int3((p + 50000) / cellSize), FP32, truncation.
+struct Grid {
+    float cs[3];
+    int dims[3];
+    uint32_t mask[3];
+    int num_cells() const { return dims[0] * dims[1] * dims[2]; }
+    inline void pos(const float p[3], int g[3]) const {
+        for (int a = 0; a < 3; ++a) g[a] = (int)((p[a] + 50000.0f) / cs[a]);
+    }
+    // calcGridAddress (grid.cuh:15-22), x-major, extended to per-axis dims
+    inline int addr(int x, int y, int z) const {
+        uint32_t cx = (uint32_t)x & mask[0], cy = (uint32_t)y & mask[1], cz = (uint32_t)z & mask[2];
+        return (int)(cx * (uint32_t)(dims[1] * dims[2]) + cy * (uint32_t)dims[2] + cz);
+    }
+    inline void decompose(int a, int c[3]) const {
+        c[2] = a % dims[2];
+        c[1] = (a / dims[2]) % dims[1];
+        c[0] = a / (dims[1] * dims[2]);
+    }
+};
+
+template <class R> inline void to_f(const V3<R> &v, float p[3]) {
+    p[0] = (float)v.x; p[1] = (float)v.y; p[2] = (float)v.z;
+}
+
+// --------------------------------------------------------------- SPH kernels
+// SetKernelRadius (fluid.cuh:11-27), computed in the real type.
+template <class R> struct KConst {
+    R c1, c2, c3, c4, h, h2, halfh;
+    void set(R hh) {
+        const R pi = R(3.141592f);
+        c1 = R(315) / R(64) / pi / std::pow(hh, R(9));
+        c2 = R(-45) / pi / std::pow(hh, R(6));
+        c3 = R(32) / pi / std::pow(hh, R(9));
+        c4 = std::pow(hh, R(6)) / R(64);
+        h2 = hh * hh;
+        halfh = hh * R(0.5f);
+        h = hh;
+    }
+    // poly6Kernel (fluid.cuh:31-42), range check off (USE_KERNEL_DISTANCE_CHECK undefined)
+    inline R poly6(R r2) const { R t = h2 - r2; return c1 * t * t * t; }
+    // gradientSpikyKernel (fluid.cuh:45-56)
+    inline V3<R> grad(V3<R> v, R r2) const {
+        R r = std::sqrt(r2);
+        R t = h - r;
+        R s = c2 * t * t;
+        return (s * v) / r;
+    }
+    // akinciSplineC (fluid.cuh:59-73)
+    inline R akinci(R r) const {
+        R t = (h - r) * r;
+        R t3 = t * t * t;
+        if (r >= halfh) return c3 * t3;
+        return R(2) * c3 * t3 - c4;
+    }
+};
+
+// ------------------------------------------------------------- quaternions
+// cudamatrix.cuh:112-172
+template <class R> inline Q4<R> qmul(Q4<R> a, Q4<R> b) {
+    Q4<R> r;
+    r.x = a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y;
+    r.y = a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x;
+    r.z = a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w;
+    r.w = a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z;
+    return r;
+}
+template <class R> inline Q4<R> angle_axis(V3<R> d, R angle) {
+    R s = std::sin(angle * R(0.5f)), c = std::cos(angle * R(0.5f));
+    return Q4<R>{d.x * s, d.y * s, d.z * s, c};
+}
+template <class R> inline Q4<R> qnormalize(Q4<R> q) {
+    R inv = R(1) / std::sqrt(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    return Q4<R>{q.x * inv, q.y * inv, q.z * inv, q.w * inv};
+}
+template <class R> struct M3 {
+    V3<R> c[3];
+};
+template <class R> inline M3<R> quat_to_mat(Q4<R> q) {
+    R x2 = q.x * q.x, y2 = q.y * q.y, z2 = q.z * q.z;
+    M3<R> m;
+    m.c[0].x = R(1) - R(2) * y2 - R(2) * z2;
+    m.c[0].y = R(2) * q.x * q.y + R(2) * q.w * q.z;
+    m.c[0].z = R(2) * q.x * q.z - R(2) * q.w * q.y;
+    m.c[1].x = R(2) * q.x * q.y - R(2) * q.w * q.z;
+    m.c[1].y = R(1) - R(2) * x2 - R(2) * z2;
+    m.c[1].z = R(2) * q.y * q.z + R(2) * q.w * q.x;
+    m.c[2].x = R(2) * q.x * q.z + R(2) * q.w * q.y;
+    m.c[2].y = R(2) * q.y * q.z - R(2) * q.w * q.x;
+    m.c[2].z = R(1) - R(2) * x2 - R(2) * y2;
+    return m;
+}
+// matrix3 * float3 (cudamatrix.cuh:87-95)
+template <class R> inline V3<R> mat_vec(const M3<R> &m, V3<R> a) {
+    V3<R> o = m.c[0] * a.x;
+    o += m.c[1] * a.y;
+    o += m.c[2] * a.z;
+    return o;
+}
+// Rotation extraction loop of shapeMatchingAlphaOne (shapematching.cuh:70-88)
+template <class R> Q4<R> extract_rotation(const V3<R> A[3], Q4<R> q, int iters) {
+    for (int it = 0; it < iters; ++it) {
+        M3<R> Rm = quat_to_mat(q);
+        V3<R> num = cross(Rm.c[0], A[0]) + cross(Rm.c[1], A[1]) + cross(Rm.c[2], A[2]);
+        R den = R(1) / std::fabs(dot(Rm.c[0], A[0]) + dot(Rm.c[1], A[1]) + dot(Rm.c[2], A[2])) + R(1e-9f);
+        V3<R> om = num * den;
+        R w2 = length2(om);
+        if (w2 <= R(1e-9f)) break;
+        R w = std::sqrt(w2);
+        q = qmul(angle_axis(om / w, w), q);
+        q = qnormalize(q);
+    }
+    return q;
+}
+
+// ------------------------------------------------------- stable key sort
+// cub::DeviceRadixSort::SortPairs over all 32 bits of non-negative int keys
+// (particlesolver.cuh:721-738): stable ascending.  A counting sort over the
+// cell range gives the identical permutation.
+void stable_sort_keys(const std::vector<int> &keys, int num_cells, std::vector<int> &sorted_keys,
+                      std::vector<int> &perm) {
+    const int n = (int)keys.size();
+    std::vector<int> count((size_t)num_cells + 1, 0);
+    for (int i = 0; i < n; ++i) count[(size_t)keys[i] + 1]++;
+    for (int c = 0; c < num_cells; ++c) count[(size_t)c + 1] += count[(size_t)c];
+    sorted_keys.assign(n, 0);
+    perm.assign(n, 0);
+    for (int i = 0; i < n; ++i) {
+        int dst = count[(size_t)keys[i]]++;
+        sorted_keys[dst] = keys[i];
+        perm[dst] = i;
+    }
+}
+
+// findStartEndId (grid.cuh:70-99)
+void find_ranges(const std::vector<int> &sk, std::vector<int> &start, std::vector<int> &end) {
+    const int n = (int)sk.size();
+    for (int i = 0; i < n; ++i) {
+        int u = sk[i];
+        if (i > 0) {
+            int v = sk[i - 1];
+            if (u != v) { start[u] = i; end[v] = i; }
+        } else {
+            start[u] = i;
+        }
+        if (i == n - 1) end[u] = n;
+    }
+}
+// resetStartEndId (grid.cuh:101-132)
+void reset_ranges(const std::vector<int> &sk, std::vector<int> &start, std::vector<int> &end) {
+    const int n = (int)sk.size();
+    for (int i = 0; i < n; ++i) {
+        int u = sk[i];
+        if (i > 0) {
+            int v = sk[i - 1];
+            if (u != v) { start[u] = -1; end[v] = -1; }
+        } else {
+            start[u] = -1;
+        }
+        if (i == n - 1) end[u] = -1;
+    }
+}
+
+struct SolverBase {
+    virtual ~SolverBase() {}
+    virtual int n() const = 0;
+    virtual int step(int substeps, float dt, int picked, const float *pp, const float *pv) = 0;
+    virtual void set_params(const bp_params &p) = 0;
+    virtual void get_state(double *pos, double *vel, int *map) = 0;
+    virtual void set_state(const double *pos, const double *vel, const int *map) = 0;
+    virtual void get_rigid(double *q, double *cm) = 0;
+    virtual void set_rigid(const double *q) = 0;
+    virtual void fetch(int which, double *out) = 0;
+    virtual void fetch_int(int which, int *out) = 0;
+};
+
+template <class R> struct Solver : SolverBase {
+    bp_params P;
+    Grid grid;
+    KConst<R> K;
+    int N = 0;
+    int mode = 0;
+    R radius = 0, rest_density = 1;
+    // per-slot state (sorted order, ParticleSolver dev* members :1078-1091)
+    std::vector<V3<R>> xs, x, v;
+    std::vector<R> mass;
+    std::vector<int> phase, orig, group;
+    std::vector<int> to_new;
+    // scratch
+    std::vector<V3<R>> dx, omega, normal, vtmp;
+    std::vector<R> lambda, density, invw;
+    std::vector<int> keys, sorted_keys, perm, cstart, cend;
+    // scene (original ids)
+    std::vector<V3<R>> plane_o, plane_n;
+    std::vector<int> dpairs;
+    std::vector<R> dparams;
+    std::vector<int> bends, faces, rranges, immov;
+    std::vector<V3<R>> rrest;
+    std::vector<Q4<R>> rrot;
+    std::vector<V3<R>> rcm;
+    bool has_solid = false, has_fluid = false;
+
+    int n() const override { return N; }
+
+    void init(const bp_scene_desc &s, const bp_params &p, int m) {
+        P = p;
+        mode = m;
+        N = s.num_particles;
+        for (int a = 0; a < 3; ++a) {
+            grid.cs[a] = s.cell_size[a];
+            grid.dims[a] = s.grid_dims[a];
+            grid.mask[a] = (uint32_t)s.grid_dims[a] - 1u;
+        }
+        K.set(R(s.kernel_radius));
+        radius = R(s.particle_radius);
+        rest_density = R(p.fluid_rest_density);
+        xs.assign(N, {});
+        x.resize(N);
+        v.assign(N, V3<R>{0, 0, 0});
+        mass.resize(N);
+        phase.resize(N);
+        orig.resize(N);
+        group.assign(N, 0);
+        to_new.resize(N);
+        for (int i = 0; i < N; ++i) {
+            x[i] = mk(R(s.positions[3 * i]), R(s.positions[3 * i + 1]), R(s.positions[3 * i + 2]));
+            if (s.velocities)
+                v[i] = mk(R(s.velocities[3 * i]), R(s.velocities[3 * i + 1]), R(s.velocities[3 * i + 2]));
+            mass[i] = R(s.masses[i]);
+            phase[i] = s.phases[i];
+            if (s.group_ids) group[i] = s.group_ids[i];
+            orig[i] = i;       // initOrder_int (util.cuh:312-328)
+            to_new[i] = i;
+            if (phase[i] > 0) has_solid = true; else has_fluid = true;
+        }
+        dx.assign(N, {}); omega.assign(N, {}); normal.assign(N, {}); vtmp.assign(N, {});
+        lambda.assign(N, R(0)); density.assign(N, R(0)); invw.assign(N, R(0));
+        cstart.assign(grid.num_cells(), -1);
+        cend.assign(grid.num_cells(), -1);
+        for (int i = 0; i < s.num_planes; ++i) {
+            plane_o.push_back(mk(R(s.plane_origins[3 * i]), R(s.plane_origins[3 * i + 1]), R(s.plane_origins[3 * i + 2])));
+            plane_n.push_back(mk(R(s.plane_normals[3 * i]), R(s.plane_normals[3 * i + 1]), R(s.plane_normals[3 * i + 2])));
+        }
+        dpairs.assign(s.distance_pairs, s.distance_pairs + 2 * s.num_distance);
+        for (int i = 0; i < 2 * s.num_distance; ++i) dparams.push_back(R(s.distance_params[i]));
+        bends.assign(s.bendings, s.bendings + 4 * s.num_bendings);
+        faces.assign(s.faces, s.faces + 3 * s.num_faces);
+        rranges.assign(s.rigid_ranges, s.rigid_ranges + 2 * s.num_rigid);
+        for (int i = 0; i < s.rigid_rest_len; ++i)
+            rrest.push_back(mk(R(s.rigid_rest_positions[3 * i]), R(s.rigid_rest_positions[3 * i + 1]),
+                               R(s.rigid_rest_positions[3 * i + 2])));
+        rrot.assign(s.num_rigid, Q4<R>{0, 0, 0, 1});
+        rcm.assign(s.num_rigid, V3<R>{0, 0, 0});
+        immov.assign(s.immovables, s.immovables + s.num_immovables);
+    }
+
+    void set_params(const bp_params &p) override {
+        P = p;
+        rest_density = R(p.fluid_rest_density);
+    }
+
+    inline int cur_cell_coords(const V3<R> &p, int g[3]) const {
+        float f[3];
+        to_f(p, f);
+        grid.pos(f, g);
+        return grid.addr(g[0], g[1], g[2]);
+    }
+
+    // ------------------------------------------------------------ grid
+    // updateGrid (particlesolver.cuh:712-765) + inverseMapping (util.cuh:348-359)
+    void update_grid() {
+        keys.resize(N);
+        for (int i = 0; i < N; ++i) {  // updateGridId (grid.cuh:30-47)
+            int g[3];
+            keys[i] = cur_cell_coords(xs[i], g);
+        }
+        stable_sort_keys(keys, grid.num_cells(), sorted_keys, perm);
+        find_ranges(sorted_keys, cstart, cend);
+        // reorderParticlesData (grid.cuh:134-164) + the swaps (:758-764)
+        std::vector<V3<R>> nxs(N), nx(N), nv(N);
+        std::vector<R> nm(N);
+        std::vector<int> np(N), no(N), ng(N);
+        for (int i = 0; i < N; ++i) {
+            int o = perm[i];
+            nxs[i] = xs[o]; nx[i] = x[o]; nv[i] = v[o]; nm[i] = mass[o];
+            np[i] = phase[o]; no[i] = orig[o]; ng[i] = group[o];
+        }
+        xs.swap(nxs); x.swap(nx); v.swap(nv); mass.swap(nm);
+        phase.swap(np); orig.swap(no); group.swap(ng);
+        for (int i = 0; i < N; ++i) to_new[orig[i]] = i;
+    }
+
+    // ------------------------------------------------------------ neighbours
+    template <class F> inline void for_candidates(const V3<R> &center, F &&f) const {
+        int g[3];
+        float c[3];
+        to_f(center, c);
+        grid.pos(c, g);
+        if (mode == 0) {
+            // loop order z, y, x as in every fluid kernel (fluid.cuh:108-110)
+            for (int z = g[2] - 1; z <= g[2] + 1; ++z)
+                for (int y = g[1] - 1; y <= g[1] + 1; ++y)
+                    for (int xx = g[0] - 1; xx <= g[0] + 1; ++xx) {
+                        int a = grid.addr(xx, y, z);
+                        int s = cstart[a], e = cend[a];
+                        for (int j = s; j < e; ++j) f(j);
+                    }
+        } else {
+            // gather mode: the GPU kernels' order -- y, x, then the z run
+            for (int y = g[1] - 1; y <= g[1] + 1; ++y)
+                for (int xx = g[0] - 1; xx <= g[0] + 1; ++xx)
+                    for (int z = g[2] - 1; z <= g[2] + 1; ++z) {
+                        int a = grid.addr(xx, y, z);
+                        int s = cstart[a], e = cend[a];
+                        for (int j = s; j < e; ++j) f(j);
+                    }
+        }
+    }
+    // true iff wrapped cell `cell` lies in the 27-cell block around grid position g
+    inline bool in_block(const int g[3], int cell) const {
+        int c[3];
+        grid.decompose(cell, c);
+        for (int a = 0; a < 3; ++a) {
+            bool ok = false;
+            for (int d = -1; d <= 1; ++d)
+                if ((int)(((uint32_t)(g[a] + d)) & grid.mask[a]) == c[a]) ok = true;
+            if (!ok) return false;
+        }
+        return true;
+    }
+
+    // ---------------------------------------------------------- passes
+    // computeInvScaledMasses (particlesolver.cuh:84-98)
+    void inv_scaled_masses() {
+        const R e = R(2.7182818284f);
+        const R k = R(P.mass_scaling_k);
+        for (int i = 0; i < N; ++i) {
+            R scale = std::pow(e, -k * x[i].y);
+            invw[i] = R(1) / (scale * mass[i]);
+        }
+    }
+    // planeStabilize (particleplane.cuh:3-19)
+    void plane_stabilize(V3<R> o, V3<R> nrm) {
+        for (int i = 0; i < N; ++i) {
+            V3<R> o2p = o - x[i];
+            R d = dot(o2p, nrm) + radius;
+            if (d <= R(0)) continue;
+            x[i] += d * nrm;
+            xs[i] += d * nrm;
+        }
+    }
+    // particlePlaneCollisionConstraint (particleplane.cuh:21-58)
+    void plane_collide(V3<R> o, V3<R> nrm) {
+        const R fs = R(P.friction_static), fd = R(P.friction_dynamic);
+        for (int i = 0; i < N; ++i) {
+            V3<R> o2p = o - xs[i];
+            R d = dot(o2p, nrm) + radius;
+            if (d <= R(0)) continue;
+            V3<R> pos = x[i];
+            V3<R> diff = xs[i] - pos;
+            R dn = dot(diff, nrm);
+            V3<R> dtan = diff - dn * nrm;
+            R dtl = length(dtan);
+            V3<R> resolved = d * nrm + xs[i];
+            V3<R> dX = resolved - pos;
+            V3<R> tdx = dX - dot(dX, nrm) * nrm;
+            if (dtl < fs * dn)
+                xs[i] = resolved - tdx;
+            else
+                xs[i] = resolved - tdx * fmin_(fd * -dn / dtl, R(1));
+        }
+    }
+    // particleParticleCollisionConstraint (particleparticle.cuh:5-101) + Δx apply
+    void particle_particle() {
+        const R fs = R(P.friction_static), fd = R(P.friction_dynamic);
+        const R r = radius;
+        for (int i = 0; i < N; ++i) dx[i] = V3<R>{0, 0, 0};
+        for (int i = 0; i < N; ++i) {
+            const V3<R> xi = x[i], xip = xs[i];
+            V3<R> sumD{0, 0, 0}, sumF{0, 0, 0};
+            const R wi = invw[i];
+            const int pi = phase[i];
+            int cnt = 0;
+            for_candidates(xip, [&](int j) {
+                int pj = phase[j];
+                if (i != j && pi != pj && (pi > 0 || pj > 0)) {
+                    V3<R> xjp = xs[j];
+                    V3<R> diff = xip - xjp;
+                    R d2 = length2(diff);
+                    if (d2 < r * r * R(4) && d2 > R(1e-5f)) {
+                        R dist = std::sqrt(d2);
+                        R wj = invw[j];
+                        R w1 = wi / (wi + wj);
+                        R w2 = wj / (wi + wj);
+                        V3<R> proj = diff * (R(2) * r / dist - R(1));
+                        V3<R> dxi = w1 * proj;
+                        V3<R> xistar = dxi + xip;
+                        sumD += dxi;
+                        V3<R> dxj = -w2 * proj;
+                        R l2 = length2(dxi);
+                        if (l2 > r * r * R(0.001f) * R(0.001f)) {
+                            cnt += 1;
+                            V3<R> xj = x[j];
+                            V3<R> xjstar = dxj + xjp;
+                            V3<R> term1 = (xistar - xi) - (xjstar - xj);
+                            V3<R> n = diff / dist;
+                            V3<R> tan = term1 - dot(term1, n) * n;
+                            R tl2 = length2(tan);
+                            if (tl2 <= (fs * fs) * l2)
+                                sumF -= w1 * tan;
+                            else
+                                sumF -= w1 * tan * fmin_(fd * std::sqrt(l2 / tl2), R(1));
+                        }
+                    }
+                }
+            });
+            dx[i] += (cnt == 0) ? sumD : sumD + sumF / R(cnt);
+        }
+        for (int i = 0; i < N; ++i) xs[i] += dx[i];  // accDevArr_float3 (util.cuh:261-269)
+    }
+    // fluidLambda (fluid.cuh:78-159)
+    void fluid_lambda() {
+        const R sds = R(P.solid_density_scale);
+        for (int i = 0; i < N; ++i) {
+            if (phase[i] > 0) continue;
+            const V3<R> pi = xs[i];
+            R dens = 0, sumg2 = 0;
+            V3<R> gi{0, 0, 0};
+            for_candidates(pi, [&](int j) {
+                if (i <= j) {  // Q1: j >= i only (fluid.cuh:117)
+                    V3<R> pj = xs[j];
+                    V3<R> diff = pi - pj;
+                    R d2 = length2(pi - pj);
+                    if (d2 <= K.h2) {
+                        R mj = mass[j];
+                        R ld = mj * K.poly6(d2);
+                        dens += (phase[j] < 0) ? ld : sds * ld;
+                        if (P.lambda_mode == 1 && d2 > R(0)) {
+                            V3<R> g = (-mj * K.grad(diff, d2)) / rest_density;
+                            sumg2 += dot(g, g);
+                            gi -= g;
+                        }
+                    }
+                }
+            });
+            if (P.lambda_mode == 1) sumg2 += dot(gi, gi);
+            R C = dens / rest_density - R(1);
+            if (P.akinci) C = fmax_(C, R(0));
+            R lam = (P.lambda_mode == 1) ? -C / (sumg2 + R(P.lambda_relaxation))
+                                         : -C / R(P.lambda_denominator);
+            lambda[i] = lam;
+            density[i] = dens;
+        }
+    }
+    // fluidPosition pair term (fluid.cuh:199-221) for the pair (a,b) seen from a
+    inline V3<R> pair_delta(const V3<R> &diff, R d2, R lam_sum, R wdq) const {
+        R sl = lam_sum;
+        if (!P.akinci) sl += -R(P.scorr_k) * std::pow(K.poly6(d2) / wdq, R(P.scorr_n));
+        V3<R> delta = sl * K.grad(diff, d2);
+        const R clamp = R(P.delta_clamp);
+        R len = length(delta);
+        return (len < clamp) ? delta : delta * fmin_(len, clamp) / len;
+    }
+    // fluidPosition (fluid.cuh:161-228) + Δx zero/apply (particlesolver.cuh:919-932)
+    long long missed_pairs = 0;
+    void fluid_position() {
+        const R inv_rho = R(1) / rest_density;
+        const R wdq = K.poly6(std::pow(R(P.scorr_dq) * K.h, R(2)));
+        for (int i = 0; i < N; ++i) dx[i] = V3<R>{0, 0, 0};
+        if (mode == 0) {
+            for (int i = 0; i < N; ++i) {
+                if (phase[i] > 0) continue;
+                const V3<R> pi = xs[i];
+                V3<R> sum{0, 0, 0};
+                const R mi = mass[i], li = lambda[i];
+                for_candidates(pi, [&](int j) {
+                    if (i < j) {
+                        V3<R> pj = xs[j];
+                        V3<R> diff = pi - pj;
+                        R d2 = length2(diff);
+                        if (d2 <= K.h2 && d2 > R(0)) {
+                            R mj = mass[j];
+                            V3<R> delta = pair_delta(diff, d2, li + lambda[j], wdq);
+                            sum += mj * delta;
+                            dx[j] += -mi * delta * inv_rho;
+                        }
+                    }
+                });
+                dx[i] += sum * inv_rho;
+            }
+        } else {
+            // gather: each k sums m_j δ(k,j) over the pairs the reference forms
+            for (int k = 0; k < N; ++k) {
+                const V3<R> pk = xs[k];
+                const int bk = sorted_keys[k];
+                const bool k_fluid = phase[k] <= 0;
+                V3<R> sum{0, 0, 0};
+                const R lk = lambda[k];
+                for_candidates(pk, [&](int j) {
+                    if (j == k) return;
+                    if (j > k && !k_fluid) return;      // pair owned by solid k: none
+                    if (j < k && phase[j] > 0) return;  // pair owned by solid j: none
+                    V3<R> pj = xs[j];
+                    V3<R> diff = pk - pj;
+                    R d2 = length2(diff);
+                    if (!(d2 <= K.h2 && d2 > R(0))) return;
+                    if (j < k) {
+                        // owner j searched around its current cell
+                        int g[3];
+                        cur_cell_coords(pj, g);
+                        if (!in_block(g, bk)) return;
+                    }
+                    V3<R> delta = pair_delta(diff, d2, lk + lambda[j], wdq);
+                    sum += mass[j] * delta;
+                });
+                dx[k] = sum * inv_rho;
+            }
+        }
+        for (int i = 0; i < N; ++i) xs[i] += dx[i];
+    }
+    // distanceConstraints (distance.cuh:3-41) + Δx apply
+    void distance() {
+        for (int i = 0; i < N; ++i) dx[i] = V3<R>{0, 0, 0};
+        const int nd = (int)dpairs.size() / 2;
+        for (int c = 0; c < nd; ++c) {
+            int id1 = to_new[dpairs[2 * c]], id2 = to_new[dpairs[2 * c + 1]];
+            R L = dparams[2 * c], stiff = dparams[2 * c + 1];
+            V3<R> x12 = xs[id1] - xs[id2];
+            R d = length(x12);
+            if (d <= R(1e-5f)) continue;
+            V3<R> n = x12 / d;
+            R w1 = invw[id1], w2 = invw[id2];
+            R sw = w1 + w2;
+            V3<R> k = n * ((d - L) / sw) * stiff;
+            dx[id1] += -w1 * k;
+            dx[id2] += w2 * k;
+        }
+        for (int i = 0; i < N; ++i) xs[i] += dx[i];
+    }
+    // bendingConstraints (bending.cuh:3-76) + Δx apply
+    void bending() {
+        for (int i = 0; i < N; ++i) dx[i] = V3<R>{0, 0, 0};
+        const int nb = (int)bends.size() / 4;
+        const R kb = R(P.bend_stiffness);
+        for (int c = 0; c < nb; ++c) {
+            int id1 = to_new[bends[4 * c]], id2 = to_new[bends[4 * c + 1]];
+            int id3 = to_new[bends[4 * c + 2]], id4 = to_new[bends[4 * c + 3]];
+            V3<R> P1 = xs[id1];
+            V3<R> P2 = xs[id2] - P1, P3 = xs[id3] - P1, P4 = xs[id4] - P1;
+            V3<R> c23 = cross(P2, P3), c24 = cross(P2, P4);
+            R l23 = length(c23);
+            if (l23 <= R(1e-5f)) continue;
+            R l24 = length(c24);
+            if (l24 <= R(1e-5f)) continue;
+            V3<R> N1 = c23 * (R(1) / l23);
+            V3<R> N2 = c24 * (R(1) / l24);
+            R d = dot(N1, N2);
+            d = fmin_(fmax_(d, R(-1)), R(1));
+            V3<R> c2n1 = cross(P2, N1), c2n2 = cross(P2, N2);
+            V3<R> c3n1 = cross(P3, N1), c3n2 = cross(P3, N2);
+            V3<R> c4n1 = cross(P4, N1), c4n2 = cross(P4, N2);
+            V3<R> Q3 = (c2n2 - c2n1 * d) / l23;
+            V3<R> Q4v = (c2n1 - c2n2 * d) / l24;
+            V3<R> Q2 = (-(c3n2 - c3n1 * d)) / l23 - (c4n1 - c4n2 * d) / l24;
+            V3<R> Q1 = -Q2 - Q3 - Q4v;
+            R w1 = invw[id1], w2 = invw[id2], w3 = invw[id3], w4 = invw[id4];
+            R den = w1 * length2(Q1) + w2 * length2(Q2) + w3 * length2(Q3) + w4 * length2(Q4v);
+            R num = std::sqrt(R(1) - d * d) * (std::acos(d) - R(3.141592f));
+            if (den <= R(1e-5f)) continue;
+            R sc = num / den;
+            dx[id1] += -w1 * Q1 * sc * kb;
+            dx[id2] += -w2 * Q2 * sc * kb;
+            dx[id3] += -w3 * Q3 * sc * kb;
+            dx[id4] += -w4 * Q4v * sc * kb;
+        }
+        for (int i = 0; i < N; ++i) xs[i] += dx[i];
+    }
+    // shapeMatchingAlphaOne (shapematching.cuh:8-99), one body at a time
+    void shape_matching() {
+        const int nr = (int)rranges.size() / 2;
+        for (int b = 0; b < nr; ++b) {
+            int s = rranges[2 * b], e = rranges[2 * b + 1];
+            int np = e - s;
+            V3<R> sum{0, 0, 0};
+            for (int t = 0; t < np; ++t) sum += xs[to_new[s + t]];
+            V3<R> cm = sum / R(np);
+            rcm[b] = cm;
+            V3<R> A[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            for (int t = 0; t < np; ++t) {
+                V3<R> q = rrest[s + t];
+                V3<R> p = xs[to_new[s + t]] - cm;
+                A[0] += p * q.x;
+                A[1] += p * q.y;
+                A[2] += p * q.z;
+            }
+            Q4<R> q = extract_rotation(A, rrot[b], P.shape_matching_iterations);
+            M3<R> Rm = quat_to_mat(q);
+            rrot[b] = q;
+            for (int t = 0; t < np; ++t) xs[to_new[s + t]] = mat_vec(Rm, rrest[s + t]) + cm;
+        }
+    }
+    // applyWindForce (wind.cuh:3-50) + Δx apply (particlesolver.cuh:838-858)
+    void wind(R dt) {
+        for (int i = 0; i < N; ++i) dx[i] = V3<R>{0, 0, 0};
+        const int nf = (int)faces.size() / 3;
+        const V3<R> W = mk(R(P.wind[0]), R(P.wind[1]), R(P.wind[2]));
+        const R cd = R(P.drag_coeff), cl = R(P.lift_coeff);
+        for (int f = 0; f < nf; ++f) {
+            int id1 = to_new[faces[3 * f]], id2 = to_new[faces[3 * f + 1]], id3 = to_new[faces[3 * f + 2]];
+            V3<R> p1 = xs[id1], p2 = xs[id2], p3 = xs[id3];
+            V3<R> a = p2 - p1, b = p3 - p1;
+            V3<R> vel = (v[id1] + v[id2] + v[id3]) / R(3);
+            vel = vel - W;  // relative air velocity; W = 0 is the reference
+            R lv = length(vel);
+            if (lv <= R(1e-5f)) continue;
+            V3<R> vhat = vel / lv;
+            V3<R> fn = cross(a, b);
+            R lfn = length(fn);
+            if (lfn <= R(1e-5f)) continue;
+            fn = fn / lfn;
+            fn = (dot(fn, vel) > R(0)) ? fn : -fn;
+            R area = R(0.5f) * lfn;
+            V3<R> drag = cd * (lv * lv) * area * dot(fn, vhat) * (-vhat);
+            V3<R> uhat = cross(cross(fn, vhat), vhat);
+            V3<R> lift = cl * (lv * lv) * area * dot(fn, vhat) * uhat;
+            V3<R> force = drag + lift;
+            V3<R> move = force * dt * dt;
+            dx[id1] += move / mass[id1];
+            dx[id2] += move / mass[id2];
+            dx[id3] += move / mass[id3];
+        }
+        for (int i = 0; i < N; ++i) xs[i] += dx[i];
+    }
+    // fluidOmega (fluid.cuh:231-279)
+    void fluid_omega() {
+        for (int i = 0; i < N; ++i) {
+            const int pi = phase[i];
+            if (pi >= 0) continue;
+            const V3<R> p = xs[i], vi = v[i];
+            V3<R> om{0, 0, 0};
+            for_candidates(p, [&](int j) {
+                if (i != j && pi == phase[j]) {
+                    V3<R> diff = p - xs[j];
+                    R d2 = length2(diff);
+                    if (d2 <= K.h2 && d2 > R(0)) om += cross(v[j] - vi, K.grad(diff, d2));
+                }
+            });
+            omega[i] = om;
+        }
+    }
+    // fluidVorticity (fluid.cuh:281-336), in place on v
+    void fluid_vorticity(R dt) {
+        const R eps = R(P.vorticity_eps);
+        for (int i = 0; i < N; ++i) {
+            const int pi = phase[i];
+            if (pi >= 0) continue;
+            const V3<R> p = xs[i];
+            V3<R> eta{0, 0, 0};
+            for_candidates(p, [&](int j) {
+                if (i != j && pi == phase[j]) {
+                    V3<R> diff = p - xs[j];
+                    R d2 = length2(diff);
+                    if (d2 <= K.h2 && d2 > R(0)) eta += length(omega[j]) * K.grad(diff, d2);
+                }
+            });
+            if (length2(eta) > R(1e-3f)) {
+                V3<R> nrm = normalize(eta);
+                v[i] += eps * cross(nrm, omega[i]) * dt;
+            }
+        }
+    }
+    // fluidNormal (fluid.cuh:387-434)
+    void fluid_normal() {
+        for (int i = 0; i < N; ++i) {
+            const int pi = phase[i];
+            if (pi >= 0) continue;
+            const V3<R> p = xs[i];
+            V3<R> nn{0, 0, 0};
+            for_candidates(p, [&](int j) {
+                if (i != j && pi == phase[j]) {
+                    V3<R> diff = p - xs[j];
+                    R d2 = length2(diff);
+                    if (d2 <= K.h2 && d2 > R(0)) nn += R(1) / density[j] * K.grad(diff, d2);
+                }
+            });
+            normal[i] = K.h * nn;
+        }
+    }
+    // fluidAkinciTension (fluid.cuh:436-489) -> vtmp, then swap (particlesolver.cuh:1048).
+    // Particles the kernel skips keep their velocity (see DESIGN: the reference
+    // leaves stale values there).
+    void fluid_akinci(R dt) {
+        const R g = R(P.tension_gamma);
+        for (int i = 0; i < N; ++i) {
+            const int pi = phase[i];
+            vtmp[i] = v[i];
+            if (pi >= 0) continue;
+            const V3<R> p = xs[i];
+            V3<R> f{0, 0, 0};
+            for_candidates(p, [&](int j) {
+                if (i != j && pi == phase[j]) {
+                    V3<R> pj = xs[j];
+                    V3<R> diff = p - pj;
+                    R d2 = length2(diff);
+                    if (d2 <= K.h2 && d2 > R(0)) {
+                        R dist = std::sqrt(d2);
+                        V3<R> fcoh = -g * K.akinci(dist) * (p - pj) / dist;
+                        V3<R> fcur = -g * (normal[i] - normal[j]);
+                        R kij = R(2) * rest_density / (density[i] + density[j]);
+                        f += kij * (fcoh + fcur);
+                    }
+                }
+            });
+            vtmp[i] = v[i] + f * dt;
+        }
+        v.swap(vtmp);
+    }
+    // fluidXSph (fluid.cuh:338-385) -> vtmp, then swap (particlesolver.cuh:1060)
+    void fluid_xsph() {
+        const R c = R(P.xsph_c);
+        for (int i = 0; i < N; ++i) {
+            const int pi = phase[i];
+            vtmp[i] = v[i];
+            if (pi >= 0) continue;
+            const V3<R> p = xs[i];
+            const V3<R> vi = v[i];
+            V3<R> vn{0, 0, 0};
+            for_candidates(p, [&](int j) {
+                if (i != j && pi == phase[j]) {
+                    V3<R> diff = p - xs[j];
+                    R d2 = length2(diff);
+                    if (d2 <= K.h2) vn += (v[j] - vi) * K.poly6(d2);
+                }
+            });
+            vtmp[i] = vi + c * vn;
+        }
+        v.swap(vtmp);
+    }
+
+    // ------------------------------------------------------------ update
+    // ParticleSolver::update (particlesolver.cuh:772-1072)
+    int step(int substeps, float dtf, int picked, const float *pp, const float *pv) override {
+        if (N <= 0) return 0;
+        const float subdt_f = dtf / (float)substeps;
+        const R subdt = R(subdt_f);
+        const R inv_dt = R(1.0f / subdt_f);
+        const bool pick = picked >= 0 && picked < N;
+        V3<R> ppos{0, 0, 0}, pvel{0, 0, 0};
+        if (pick) {
+            ppos = mk(R(pp[0]), R(pp[1]), R(pp[2]));
+            if (pv) pvel = mk(R(pv[0]), R(pv[1]), R(pv[2]));
+        }
+        const V3<R> grav = mk(R(P.gravity[0]), R(P.gravity[1]), R(P.gravity[2]));
+        for (int s = 0; s < substeps; ++s) {
+            if (pick) {  // :790-794
+                int slot = to_new[picked];
+                x[slot] = ppos;
+                v[slot] = V3<R>{0, 0, 0};
+            }
+            // predictPositions + applyGravity (:796-804; particlesolver.cuh:49-69)
+            for (int i = 0; i < N; ++i) xs[i] = x[i] + v[i] * subdt;
+            const V3<R> gstep = grav * subdt * subdt;
+            for (int i = 0; i < N; ++i) xs[i] += gstep;
+            update_grid();        // :807
+            inv_scaled_masses();  // :812-816
+            for (int it = 0; it < P.stabilization_passes; ++it)  // :822-833
+                for (size_t pl = 0; pl < plane_o.size(); ++pl) plane_stabilize(plane_o[pl], plane_n[pl]);
+            if (P.wind_enabled && !faces.empty()) wind(subdt);  // :838-858
+            for (int it = 0; it < P.iterations; ++it) {         // :862
+                if (pick) xs[to_new[picked]] = ppos, v[to_new[picked]] = V3<R>{0, 0, 0};
+                for (size_t pl = 0; pl < plane_o.size(); ++pl) plane_collide(plane_o[pl], plane_n[pl]);
+                if (has_solid) particle_particle();
+                if (has_fluid) {
+                    fluid_lambda();
+                    fluid_position();
+                }
+                if (!dpairs.empty()) distance();
+                if (!bends.empty()) bending();
+                if (!rranges.empty()) shape_matching();
+                for (size_t m = 0; m < immov.size(); ++m) {  // immovableConstraints (immovable.cuh:3-14)
+                    int id = to_new[immov[m]];
+                    xs[id] = x[id];
+                }
+            }
+            // updateVelocity + updatePositions (:996-1002)
+            for (int i = 0; i < N; ++i) v[i] = (xs[i] - x[i]) * inv_dt;
+            const R thr = R(P.sleep_epsilon);
+            for (int i = 0; i < N; ++i) {
+                R d2 = length2(xs[i] - x[i]);
+                x[i] = (d2 >= thr * thr || phase[i] < 0) ? xs[i] : x[i];
+            }
+            if (has_fluid) {
+                if (P.vorticity_eps != 0.0f) {
+                    fluid_omega();
+                    fluid_vorticity(subdt);
+                }
+                if (P.akinci) {
+                    fluid_normal();
+                    fluid_akinci(subdt);
+                }
+                if (P.xsph_c != 0.0f) fluid_xsph();
+            }
+            reset_ranges(sorted_keys, cstart, cend);  // :1062
+        }
+        if (pick) {  // :1066-1071
+            int slot = to_new[picked];
+            v[slot] = pvel;
+        }
+        return 0;
+    }
+
+    void get_state(double *pos, double *vel, int *map) override {
+        for (int i = 0; i < N; ++i) {
+            if (pos) { pos[3 * i] = (double)x[i].x; pos[3 * i + 1] = (double)x[i].y; pos[3 * i + 2] = (double)x[i].z; }
+            if (vel) { vel[3 * i] = (double)v[i].x; vel[3 * i + 1] = (double)v[i].y; vel[3 * i + 2] = (double)v[i].z; }
+            if (map) map[i] = orig[i];
+        }
+    }
+    // Sets the slot-order state; masses / phases / group ids follow the map.
+    void set_state(const double *pos, const double *vel, const int *map) override {
+        std::vector<R> m0(N);
+        std::vector<int> p0(N), g0(N);
+        for (int i = 0; i < N; ++i) { m0[orig[i]] = mass[i]; p0[orig[i]] = phase[i]; g0[orig[i]] = group[i]; }
+        for (int i = 0; i < N; ++i) {
+            if (pos) x[i] = mk(R(pos[3 * i]), R(pos[3 * i + 1]), R(pos[3 * i + 2]));
+            if (vel) v[i] = mk(R(vel[3 * i]), R(vel[3 * i + 1]), R(vel[3 * i + 2]));
+            if (map) orig[i] = map[i];
+        }
+        for (int i = 0; i < N; ++i) {
+            mass[i] = m0[orig[i]]; phase[i] = p0[orig[i]]; group[i] = g0[orig[i]];
+            to_new[orig[i]] = i;
+        }
+    }
+    void get_rigid(double *q, double *cm) override {
+        for (size_t b = 0; b < rrot.size(); ++b) {
+            if (q) { q[4 * b] = (double)rrot[b].x; q[4 * b + 1] = (double)rrot[b].y; q[4 * b + 2] = (double)rrot[b].z; q[4 * b + 3] = (double)rrot[b].w; }
+            if (cm) { cm[3 * b] = (double)rcm[b].x; cm[3 * b + 1] = (double)rcm[b].y; cm[3 * b + 2] = (double)rcm[b].z; }
+        }
+    }
+    void set_rigid(const double *q) override {
+        for (size_t b = 0; b < rrot.size(); ++b)
+            rrot[b] = Q4<R>{R(q[4 * b]), R(q[4 * b + 1]), R(q[4 * b + 2]), R(q[4 * b + 3])};
+    }
+    void fetch(int which, double *out) override {
+        for (int i = 0; i < N; ++i) {
+            switch (which) {
+            case 0: out[i] = (double)density[i]; break;
+            case 1: out[i] = (double)lambda[i]; break;
+            case 4: out[3 * i] = (double)omega[i].x; out[3 * i + 1] = (double)omega[i].y; out[3 * i + 2] = (double)omega[i].z; break;
+            case 5: out[3 * i] = (double)normal[i].x; out[3 * i + 1] = (double)normal[i].y; out[3 * i + 2] = (double)normal[i].z; break;
+            default: break;
+            }
+        }
+    }
+    void fetch_int(int which, int *out) override {
+        for (int i = 0; i < N; ++i) out[i] = (which == 2) ? sorted_keys[i] : perm[i];
+    }
+};
+
+}  // namespace
+
+struct or_solver {
+    std::unique_ptr<SolverBase> impl;
+};
+
+extern "C" {
+
+or_solver *or_create(const bp_scene_desc *scene, const bp_params *params, int fp64, int mode) {
+    if (!scene || !params) return nullptr;
+    or_solver *s = new or_solver;
+    if (fp64) {
+        auto *p = new Solver<double>();
+        p->init(*scene, *params, mode);
+        s->impl.reset(p);
+    } else {
+        auto *p = new Solver<float>();
+        p->init(*scene, *params, mode);
+        s->impl.reset(p);
+    }
+    return s;
+}
+void or_destroy(or_solver *s) { delete s; }
+int or_num_particles(or_solver *s) { return s->impl->n(); }
+int or_step(or_solver *s, int substeps, float dt, int picked, const float *pp, const float *pv) {
+    return s->impl->step(substeps, dt, picked, pp, pv);
+}
+void or_set_params(or_solver *s, const bp_params *p) { s->impl->set_params(*p); }
+void or_get_state(or_solver *s, double *pos, double *vel, int *map) { s->impl->get_state(pos, vel, map); }
+void or_set_state(or_solver *s, const double *pos, const double *vel, const int *map) {
+    s->impl->set_state(pos, vel, map);
+}
+void or_get_rigid(or_solver *s, double *q, double *cm) { s->impl->get_rigid(q, cm); }
+void or_set_rigid(or_solver *s, const double *q) { s->impl->set_rigid(q); }
+void or_debug_fetch(or_solver *s, int which, double *out) { s->impl->fetch(which, out); }
+void or_debug_fetch_int(or_solver *s, int which, int *out) { s->impl->fetch_int(which, out); }
+
+void or_neighbor_grid(const float *xs, int n, const float cs[3], const int dims[3], int *keys, int *perm,
+                      int *cell_start, int *cell_end) {
+    Grid g;
+    for (int a = 0; a < 3; ++a) { g.cs[a] = cs[a]; g.dims[a] = dims[a]; g.mask[a] = (uint32_t)dims[a] - 1u; }
+    std::vector<int> k(n), sk, pm;
+    for (int i = 0; i < n; ++i) {
+        int gp[3];
+        g.pos(xs + 3 * i, gp);
+        k[i] = g.addr(gp[0], gp[1], gp[2]);
+    }
+    stable_sort_keys(k, g.num_cells(), sk, pm);
+    std::vector<int> st(g.num_cells(), -1), en(g.num_cells(), -1);
+    find_ranges(sk, st, en);
+    if (keys) std::copy(sk.begin(), sk.end(), keys);
+    if (perm) std::copy(pm.begin(), pm.end(), perm);
+    if (cell_start) std::copy(st.begin(), st.end(), cell_start);
+    if (cell_end) std::copy(en.begin(), en.end(), cell_end);
+}
+
+int or_cell_address(const float p[3], const float cs[3], const int dims[3]) {
+    Grid g;
+    for (int a = 0; a < 3; ++a) { g.cs[a] = cs[a]; g.dims[a] = dims[a]; g.mask[a] = (uint32_t)dims[a] - 1u; }
+    int gp[3];
+    g.pos(p, gp);
+    return g.addr(gp[0], gp[1], gp[2]);
+}
+
+double or_poly6(double r2, double h, int fp64) {
+    if (fp64) { KConst<double> k; k.set(h); return k.poly6(r2); }
+    KConst<float> k; k.set((float)h); return (double)k.poly6((float)r2);
+}
+void or_grad_spiky(const double v[3], double r2, double h, int fp64, double out[3]) {
+    if (fp64) {
+        KConst<double> k; k.set(h);
+        V3<double> g = k.grad(V3<double>{v[0], v[1], v[2]}, r2);
+        out[0] = g.x; out[1] = g.y; out[2] = g.z;
+    } else {
+        KConst<float> k; k.set((float)h);
+        V3<float> g = k.grad(V3<float>{(float)v[0], (float)v[1], (float)v[2]}, (float)r2);
+        out[0] = g.x; out[1] = g.y; out[2] = g.z;
+    }
+}
+double or_akinci_c(double r, double h, int fp64) {
+    if (fp64) { KConst<double> k; k.set(h); return k.akinci(r); }
+    KConst<float> k; k.set((float)h); return (double)k.akinci((float)r);
+}
+void or_extract_rotation(const double A[9], double q[4], int iterations, int fp64) {
+    if (fp64) {
+        V3<double> a[3] = {{A[0], A[1], A[2]}, {A[3], A[4], A[5]}, {A[6], A[7], A[8]}};
+        Q4<double> r = extract_rotation(a, Q4<double>{q[0], q[1], q[2], q[3]}, iterations);
+        q[0] = r.x; q[1] = r.y; q[2] = r.z; q[3] = r.w;
+    } else {
+        V3<float> a[3] = {{(float)A[0], (float)A[1], (float)A[2]}, {(float)A[3], (float)A[4], (float)A[5]},
+                          {(float)A[6], (float)A[7], (float)A[8]}};
+        Q4<float> r = extract_rotation(a, Q4<float>{(float)q[0], (float)q[1], (float)q[2], (float)q[3]}, iterations);
+        q[0] = r.x; q[1] = r.y; q[2] = r.z; q[3] = r.w;
+    }
+}
+
+}  // extern "C"

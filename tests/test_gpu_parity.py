@@ -1,0 +1,201 @@
+"""GPU parity: the gfx950 solver (through the C ABI) against the CPU oracle.
+
+Integer work (cell keys, sort permutation, cell ranges, slot maps) must match
+bit-for-bit on identical inputs.  FP32 positions after one substep from an
+identical state must match the oracle in the GPU's summation order within
+POS_TOL_GATHER, and the oracle in the reference's order (scatter) within
+POS_TOL_SCATTER; velocities are (x* - x)/dt, so their tolerance is the
+position tolerance times 1/dt.  Multi-substep errors against FP32 and FP64
+are reported."""
+import numpy as np
+import pytest
+
+import brasspan as bp
+import oracle_py as orc
+import scenes
+
+pytestmark = pytest.mark.gpu
+
+# per-substep position tolerances (units: scene length; h = 0.1)
+POS_TOL_GATHER = 2e-5    # same summation order: only powf/sinf/cosf/acosf ulps differ
+POS_TOL_SCATTER = 2e-3   # reference order + scatter pair ownership; borderline pairs may flip
+DT = 1.0 / 120.0
+
+
+def case(msg):
+    print(f"[gpu case] {msg}", flush=True)
+
+
+def set_same_state(gpu, ora):
+    p, v, m = ora.get_state()
+    gpu.set_state(p.astype(np.float32), v.astype(np.float32), m)
+    if ora.num_rigid:
+        q, _ = ora.rigid_state()
+        gpu.set_rigid_state(q.astype(np.float32))
+
+
+def per_substep_parity(sc, prm, substeps, label, tol_g=POS_TOL_GATHER, tol_s=POS_TOL_SCATTER, warm=3):
+    arr = bp.SceneArrays(sc)
+    og = orc.Oracle(arr, prm, fp64=False, mode=orc.GATHER)
+    os_ = orc.Oracle(arr, prm, fp64=False, mode=orc.SCATTER)
+    case(f"{label}: create solver n={arr.n}")
+    gpu = bp.Solver(sc, prm)
+    for _ in range(warm):  # move off the initial lattice so the sort does real work
+        og.step(1, DT)
+    worst_g = worst_s = 0.0
+    for k in range(substeps):
+        os_.set_state(*og.get_state())
+        if og.num_rigid:
+            os_.set_rigid_state(og.rigid_state()[0])
+        set_same_state(gpu, og)
+        case(f"{label}: substep {k}")
+        gpu.step(1, DT)
+        og.step(1, DT)
+        os_.step(1, DT)
+        assert np.array_equal(gpu.debug(2), og.debug(2)), "cell keys differ"
+        assert np.array_equal(gpu.debug(3), og.debug(3)), "sort permutation differs"
+        gp, gv, gm = gpu.get_state()
+        rp, rv, rm = og.get_state()
+        sp, sv, sm = os_.get_state()
+        assert np.array_equal(gm, rm) and np.array_equal(gm, sm)
+        assert np.all(np.isfinite(gp))
+        eg = float(np.abs(gp - rp).max())
+        es = float(np.abs(gp - sp).max())
+        worst_g, worst_s = max(worst_g, eg), max(worst_s, es)
+        assert eg <= tol_g, f"substep {k}: |x_gpu - x_oracle(gather)| = {eg}"
+        assert es <= tol_s, f"substep {k}: |x_gpu - x_oracle(scatter)| = {es}"
+        assert float(np.abs(gv - rv).max()) <= tol_g / DT * 2
+        if og.num_rigid:
+            gq, gcm = gpu.rigid_state()
+            rq, rcm = og.rigid_state()
+            assert np.abs(gcm - rcm).max() <= tol_g
+    print(f"{label}: worst per-substep |dx| vs gather {worst_g:.3e}, vs scatter {worst_s:.3e}")
+    gpu.close()
+    return worst_g, worst_s
+
+
+@pytest.mark.parametrize("n,dims,cs", [(100_000, (64, 64, 64), (0.1, 0.1, 0.1)),
+                                       (65_536, (256, 128, 4), (0.05, 0.1, 0.3)),
+                                       (1, (4, 4, 4), (0.1, 0.1, 0.1)),
+                                       (4097, (8, 8, 8), (0.2, 0.2, 0.2))])
+def test_neighbor_grid_bitexact(n, dims, cs):
+    rng = np.random.default_rng(n)
+    xs = rng.uniform(-2.0, 5.0, size=(n, 3)).astype(np.float32)
+    if n > 100:
+        xs[: n // 10] = xs[n // 10: 2 * (n // 10)]  # duplicated cells stress stability
+    case(f"neighbor grid n={n} dims={dims}")
+    g = bp.neighbor_grid(xs, cs, dims)
+    o = orc.neighbor_grid(xs, cs, dims)
+    for a, b, name in zip(g, o, ("keys", "perm", "cell_start", "cell_end")):
+        assert np.array_equal(a, b), name
+
+
+def test_neighbor_grid_bitexact_4m_lattice():
+    sc = scenes.large_block()
+    xs = sc.positions
+    case(f"neighbor grid 4M lattice n={len(xs)}")
+    g = bp.neighbor_grid(xs, sc.cell_size, sc.grid_dims)
+    o = orc.neighbor_grid(xs, sc.cell_size, sc.grid_dims)
+    for a, b, name in zip(g, o, ("keys", "perm", "cell_start", "cell_end")):
+        assert np.array_equal(a, b), name
+
+
+def test_substep_parity_dam_break_small():
+    sc = scenes.dam_break(dims=(16, 16, 8), grid=(64, 64, 64))
+    per_substep_parity(sc, scenes.dam_break_params(), 6, "dam break 2k")
+
+
+def test_substep_parity_dam_break_16k():
+    sc = scenes.dam_break()
+    per_substep_parity(sc, scenes.dam_break_params(), 4, "dam break 16k")
+
+
+def test_substep_parity_akinci_vorticity():
+    sc = scenes.large_block(dims=(32, 16, 16), grid=(256, 128, 128))
+    per_substep_parity(sc, scenes.large_block_params(), 4, "large block 8k akinci+vorticity")
+
+
+def test_substep_parity_lambda_mode1_and_solid_layer():
+    sc = scenes.Scene(particle_radius=0.025, fluid_kernel_radius=0.1, cell_size=0.1, grid_dims=(64, 64, 64))
+    sc.add_granulars_block((12, 1, 12), (0.3, 0.03, 0.3), (0.05, 0.05, 0.05), 1.0)
+    sc.add_fluid_block((10, 10, 10), (0.35, 0.1, 0.35), 0.05, 1.0, jitter=0.002, seed=9)
+    sc.add_box_planes((0, 0, 0), (6.4, 6.4, 6.4))
+    prm = scenes.dam_break_params(lambda_mode=1, xsph_c=0.0002, vorticity_eps=1e-3, akinci=1)
+    per_substep_parity(sc, prm, 4, "fluid on granular layer, lambda mode 1")
+
+
+def test_substep_parity_mixed_phase_small():
+    sc = scenes.mixed_phase(fluid_dims=(16, 16, 16), num_boxes=6, cloth_n=16, grid=(128, 128, 128))
+    per_substep_parity(sc, scenes.mixed_phase_params(), 4, "mixed phase small")
+
+
+def test_horizon_errors_reported():
+    sc = scenes.large_block(dims=(32, 16, 16), grid=(256, 128, 128))
+    prm = scenes.large_block_params()
+    arr = bp.SceneArrays(sc)
+    o32 = orc.Oracle(arr, prm, fp64=False, mode=orc.GATHER)
+    o64 = orc.Oracle(arr, prm, fp64=True, mode=orc.SCATTER)
+    case("10-substep horizon")
+    gpu = bp.Solver(sc, prm)
+    rows = []
+    for k in range(10):
+        gpu.step(1, DT)
+        o32.step(1, DT)
+        o64.step(1, DT)
+        g, _ = gpu.state_original_order()
+        a, _ = o32.state_original_order()
+        b, _ = o64.state_original_order()
+        rows.append((k, float(np.abs(g - a).max()), float(np.sqrt(((g - a) ** 2).mean())),
+                     float(np.abs(g - b).max()), float(np.sqrt(((g - b) ** 2).mean()))))
+        assert np.all(np.isfinite(g))
+    for r in rows:
+        print("substep %d: vs FP32 max %.3e rms %.3e | vs FP64 max %.3e rms %.3e" % r)
+    gpu.close()
+
+
+def test_picking_and_queries():
+    sc = scenes.dam_break(dims=(8, 8, 8), grid=(64, 64, 64))
+    prm = scenes.dam_break_params()
+    arr = bp.SceneArrays(sc)
+    og = orc.Oracle(arr, prm, fp64=False, mode=orc.GATHER)
+    case("picking")
+    gpu = bp.Solver(sc, prm)
+    gpu.step(2, 1 / 60, picked=17, pos=(1.0, 1.0, 1.0), vel=(0.5, 0.0, 0.0))
+    og.step(2, 1 / 60, picked=17, pos=(1.0, 1.0, 1.0), vel=(0.5, 0.0, 0.0))
+    gp, gv = gpu.state_original_order()
+    rp, rv = og.state_original_order()
+    assert np.allclose(gp[17], (1.0, 1.0, 1.0)) and np.allclose(gv[17], (0.5, 0.0, 0.0))
+    assert np.abs(gp - rp).max() <= 4 * POS_TOL_GATHER
+    _, _, mp = gpu.get_state()
+    for slot in (0, 5, len(mp) - 1):
+        assert gpu.query_original_id(slot) == mp[slot]
+        assert gpu.query_slot(int(mp[slot])) == slot
+    assert np.allclose(gpu.get_particle(3), gpu.get_state()[0][3])
+    gpu.set_particle(3, (2.0, 2.0, 2.0), (0.0, 1.0, 0.0))
+    assert np.allclose(gpu.get_particle(3), (2.0, 2.0, 2.0))
+    v = gpu.device_arrays()
+    assert v.num_particles == gpu.n and v.positions and v.map_to_new
+    gpu.close()
+
+
+def test_full_size_invariants_large_block():
+    sc = scenes.large_block()
+    prm = scenes.large_block_params()
+    case(f"large block 4M invariants n={sc.num_particles()}")
+    gpu = bp.Solver(sc, prm)
+    gpu.step(1, 1 / 60)
+    keys = gpu.debug(2)
+    perm = gpu.debug(3)
+    pos, vel, mp = gpu.get_state()
+    n = gpu.n
+    assert np.all(np.diff(keys) >= 0)
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    assert np.array_equal(np.sort(mp), np.arange(n))
+    # first substep starts from slot = original id, so the permutation is the
+    # stable sort of the keys of x* = x + v dt + g dt^2 of the scene positions
+    x0 = sc.positions
+    xs = (x0 + np.float32(0.0)) + (np.float32(-9.8) * np.float32(1 / 60) * np.float32(1 / 60)) * np.array([0, 1, 0], np.float32)
+    ok, op, _, _ = orc.neighbor_grid(xs.astype(np.float32), sc.cell_size, sc.grid_dims)
+    assert np.array_equal(perm, op) and np.array_equal(keys, ok)
+    assert np.all(np.isfinite(pos)) and np.all(np.isfinite(vel))
+    gpu.close()

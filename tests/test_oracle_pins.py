@@ -1,0 +1,429 @@
+"""Pins of the CPU oracle (oracle/bp_oracle.cpp) against closed forms,
+invariants and brute force.  The reference ships no tests or golden vectors
+(its only check is the opt-in CHECK_NAN macro, kernel/cubrasstower.cuh:15),
+so every oracle function is pinned here by mathematics that a plausible
+mistake (dropped term, wrong sign or index, transposed operand) would break.
+CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+import brasspan as bp
+import oracle_py as orc
+import scenes
+
+PI_REF = float(np.float32(3.141592))  # the reference's pi literal as an FP32 constant (fluid.cuh:12-16)
+F32 = lambda v: float(np.float32(v))  # scene and parameter values enter as FP32
+
+
+def quiet_params(**kw):
+    """No gravity, planes off by scene, no post passes unless asked."""
+    p = dict(gravity=(0.0, 0.0, 0.0), iterations=1, stabilization_passes=0, vorticity_eps=0.0, xsph_c=0.0,
+             akinci=0, fluid_rest_density=1000.0, mass_scaling_k=0.0)
+    p.update(kw)
+    return bp.default_params(**p)
+
+
+def make_scene(grid=(16, 16, 16), h=0.1):
+    return scenes.Scene(particle_radius=0.025, fluid_kernel_radius=h, cell_size=h, grid_dims=grid)
+
+
+def run_one(sc, prm, fp64=True, mode=orc.SCATTER, substeps=1, dt=1.0 / 120.0):
+    o = orc.Oracle(bp.SceneArrays(sc), prm, fp64=fp64, mode=mode)
+    x0, v0 = o.state_original_order()
+    o.step(substeps, dt)
+    return o, x0, v0
+
+
+# ---------------------------------------------------------------- SPH kernels
+def test_poly6_closed_form(have_oracle):
+    for h in (0.1, 0.115, 0.23):
+        w0 = 315.0 / (64.0 * PI_REF * h ** 3)
+        assert orc.poly6(0.0, h) == pytest.approx(w0, rel=1e-12)
+        assert orc.poly6(h * h, h) == 0.0
+        assert orc.poly6(0.0, h, fp64=False) == pytest.approx(w0, rel=2e-6)
+        r2 = 0.3 * h * h
+        assert orc.poly6(r2, h) == pytest.approx(315.0 / (64 * PI_REF * h ** 9) * (h * h - r2) ** 3, rel=1e-12)
+
+
+def test_poly6_integrates_to_one(have_oracle):
+    h = 0.1
+    r = np.linspace(0.0, h, 4001)
+    w = np.array([orc.poly6(x * x, h) for x in r])
+    integral = 4.0 * math.pi * np.trapezoid(w * r * r, r)
+    assert integral == pytest.approx(math.pi / PI_REF, rel=1e-6)
+
+
+def test_spiky_gradient_is_derivative_of_spiky(have_oracle):
+    h = 0.1
+    spiky = lambda r: 15.0 / (PI_REF * h ** 6) * (h - r) ** 3
+    for r in (0.01, 0.05, 0.08, 0.0999):
+        v = np.array([r, 0.0, 0.0])
+        g = orc.grad_spiky(v, r * r, h)
+        e = 1e-7
+        fd = (spiky(r + e) - spiky(r - e)) / (2 * e)
+        assert g[0] == pytest.approx(fd, rel=1e-6)
+        assert g[1] == 0.0 and g[2] == 0.0
+    # direction along v, zero at the support radius
+    d = np.array([0.03, -0.04, 0.0])
+    g = orc.grad_spiky(d, float(d @ d), h)
+    assert np.allclose(np.cross(g, d), 0.0, atol=1e-9) and g @ d < 0
+    assert np.allclose(orc.grad_spiky(np.array([h, 0, 0]), h * h, h), 0.0)
+
+
+def test_akinci_spline_reference_form(have_oracle):
+    h = 0.1
+    c3 = 32.0 / (PI_REF * h ** 9)
+    for r in (0.01, 0.03, 0.049, 0.05, 0.07, 0.0999):
+        t3 = ((h - r) * r) ** 3
+        # akinciSplineC (fluid.cuh:59-73): the h^6/64 term is not scaled by 32/(pi h^9)
+        expect = c3 * t3 if r >= h / 2 else 2.0 * c3 * t3 - h ** 6 / 64.0
+        assert orc.akinci_c(r, h) == pytest.approx(expect, rel=1e-12)
+
+
+# ---------------------------------------------------------------- neighbour grid
+def numpy_keys(xs, cs, dims):
+    xs = np.asarray(xs, np.float32)
+    g = ((xs + np.float32(50000.0)) / np.asarray(cs, np.float32)).astype(np.int64)
+    m = np.asarray(dims) - 1
+    g = g & m
+    return (g[:, 0] * dims[1] * dims[2] + g[:, 1] * dims[2] + g[:, 2]).astype(np.int32)
+
+
+@pytest.mark.parametrize("dims", [(16, 16, 16), (64, 8, 4), (4, 4, 4)])
+def test_neighbor_grid_brute_force(have_oracle, dims):
+    rng = np.random.default_rng(7)
+    n = 5000
+    cs = (0.1, 0.07, 0.13)
+    xs = rng.uniform(-1.0, 3.0, size=(n, 3)).astype(np.float32)
+    xs[:50] = xs[50:100]  # duplicates test stability
+    keys, perm, st, en = orc.neighbor_grid(xs, cs, dims)
+    k_np = numpy_keys(xs, cs, dims)
+    p_np = np.argsort(k_np, kind="stable")
+    assert np.array_equal(perm, p_np)
+    assert np.array_equal(keys, k_np[p_np])
+    assert np.all(np.diff(keys) >= 0)
+    # ranges partition [0, n): every occupied cell maps to its run, others -1
+    occ = np.unique(keys)
+    assert np.all(st[occ] >= 0)
+    assert np.sum(en[occ] - st[occ]) == n
+    for c in occ[:200]:
+        assert np.all(keys[st[c]:en[c]] == c)
+    empty = np.setdiff1d(np.arange(np.prod(dims)), occ)
+    assert np.all(st[empty] == -1) and np.all(en[empty] == -1)
+
+
+def test_grid_hash_quantisation_quirk(have_oracle):
+    # (p + 50000) is rounded to 2^-8 in FP32 before the division (grid.cuh:12):
+    # a point 0.001 below a cell boundary hashes into the next cell.
+    cs, dims = (0.1, 0.1, 0.1), (64, 64, 64)
+    p = np.array([0.3 - 0.001, 0.0, 0.0], np.float32)
+    exact = int((50000.0 + 0.299) / 0.1)
+    a = orc.cell_address(p, cs, dims)
+    assert (a // (64 * 64)) == (numpy_keys(p[None], cs, dims)[0] // (64 * 64))
+    assert (a // (64 * 64)) == ((exact + 1) & 63)
+
+
+# ---------------------------------------------------------------- fluid
+def lattice_scene(n=6, jitter=0.004, seed=11, h=0.1, grid=(16, 16, 16), solid_layer=False):
+    sc = make_scene(grid, h)
+    sc.add_fluid_block((n, n, n), (0.5, 0.5, 0.5), 0.05, 1.0, jitter=jitter, seed=seed)
+    if solid_layer:  # solids 0.06 apart and 0.065 below the fluid: inside h, outside contact 2r
+        sc.add_granulars_block((n, 1, n), (0.5, 0.435, 0.5), (0.06, 0.06, 0.06), 1.0)
+    return sc
+
+
+def brute_density(xs_sorted, phases_sorted, masses_sorted, cs, dims, h, sds):
+    """fluidLambda density (fluid.cuh:78-159) by brute force over all pairs."""
+    keys = numpy_keys(xs_sorted, cs, dims)
+    d = np.asarray(dims)
+    coords = np.stack([keys // (d[1] * d[2]), (keys // d[2]) % d[1], keys % d[2]], 1)
+    g = ((xs_sorted.astype(np.float32) + np.float32(50000)) / np.float32(cs[0])).astype(np.int64) & (d - 1)
+    n = len(xs_sorted)
+    out = np.zeros(n)
+    for i in range(n):
+        diffc = (coords - g[i] + 1) % d
+        cand = np.all(diffc <= 2, axis=1)
+        cand[:i] = False  # j >= i only
+        r2 = np.sum((xs_sorted[i] - xs_sorted) ** 2, axis=1)
+        sel = cand & (r2 <= h * h)
+        w = 315.0 / (64 * PI_REF * h ** 9) * (h * h - r2[sel]) ** 3 * masses_sorted[sel]
+        w = np.where(phases_sorted[sel] < 0, w, sds * w)
+        out[i] = w.sum()
+    return out
+
+
+def test_density_half_sum_brute_force(have_oracle):
+    sc = lattice_scene(solid_layer=True)
+    prm = quiet_params()
+    o, x0, _ = run_one(sc, prm, dt=1e-30)  # x* = x: the lambda pass sees the initial positions
+    pos, _, mp = o.get_state()
+    dens = o.debug(0)
+    ph = sc.phases[mp]
+    ms = sc.masses[mp].astype(np.float64)
+    xs_sorted = x0[mp]
+    ref = brute_density(xs_sorted, ph, ms, sc.cell_size, sc.grid_dims, F32(0.1), 2.0)
+    fl = ph < 0
+    assert np.allclose(dens[fl], ref[fl], rtol=1e-12, atol=0)
+    lam = o.debug(1)
+    assert np.allclose(lam[fl], -(ref[fl] / 1000.0 - 1.0) / 15000.0, rtol=1e-12, atol=1e-18)
+
+
+def test_lambda_akinci_clamp_and_mode1(have_oracle):
+    sc = lattice_scene(jitter=0.0)
+    o, _, _ = run_one(sc, quiet_params(akinci=1, fluid_rest_density=1e9), dt=1e-30)
+    assert np.all(o.debug(1) == 0.0)  # C clamped at 0 (fluid.cuh:150)
+    o2, _, _ = run_one(sc, quiet_params(lambda_mode=1), dt=1e-30)
+    o1, _, _ = run_one(sc, quiet_params(), dt=1e-30)
+    c = o1.debug(0) / 1000.0 - 1.0
+    lam2 = o2.debug(1)
+    # mode 1 (fluid.cuh:155): -C/(sum|grad C|^2 + eps) has the sign of -C and |.| < |C|/eps
+    assert np.all(np.sign(lam2[np.abs(c) > 1e-9]) == -np.sign(c[np.abs(c) > 1e-9]))
+    assert np.all(np.abs(lam2) <= np.abs(c) / 300.0 + 1e-18)
+
+
+def test_position_pass_conserves_momentum(have_oracle):
+    sc = lattice_scene(jitter=0.006)
+    for mode in (orc.SCATTER, orc.GATHER):
+        o, x0, _ = run_one(sc, quiet_params(), mode=mode, dt=1e-30)
+        x1, _ = o.state_original_order()
+        dx = x1 - x0
+        assert np.abs(dx).max() > 1e-6  # non-trivial correction
+        p = (sc.masses[:, None] * dx).sum(0)
+        assert np.abs(p).max() < 1e-12 * max(1.0, np.abs(dx).sum())
+
+
+def test_scatter_and_gather_agree_without_cell_changes(have_oracle):
+    sc = lattice_scene(jitter=0.006, solid_layer=True)
+    prm = quiet_params(iterations=1)
+    a, _, _ = run_one(sc, prm, mode=orc.SCATTER, dt=1e-30)
+    b, _, _ = run_one(sc, prm, mode=orc.GATHER, dt=1e-30)
+    xa, _ = a.state_original_order()
+    xb, _ = b.state_original_order()
+    assert np.abs(xa - xb).max() < 1e-14
+
+
+def test_two_particle_position_correction(have_oracle):
+    h, m, rho0, r = F32(0.1), 1.0, 1000.0, F32(0.04)
+    sc = make_scene()
+    sc.add_fluid_block((2, 1, 1), (0.5, 0.5, 0.5), (r, 0, 0), m)
+    prm = quiet_params(fluid_rest_density=rho0)
+    o, x0, _ = run_one(sc, prm, dt=1e-30)
+    x1, _ = o.state_original_order()
+    w = lambda r2: 315.0 / (64 * PI_REF * h ** 9) * (h * h - r2) ** 3
+    dens = np.array([m * (w(0) + w(r * r)), m * w(0)])  # slot 0 sees both (j >= i)
+    lam = -(dens / rho0 - 1.0) / 15000.0
+    wdq = w((0.03 * h) ** 2)
+    scorr = -1e-4 * (w(r * r) / wdq) ** 4
+    diff = np.array([-r, 0.0, 0.0])  # x0 - x1
+    grad = -45.0 / (PI_REF * h ** 6) * (h - r) ** 2 * diff / r
+    delta = (lam[0] + lam[1] + scorr) * grad
+    ln = np.linalg.norm(delta)
+    if ln >= 5.0:  # clamp (fluid.cuh:214-216)
+        delta = delta * 5.0 / ln
+    assert np.allclose(x1[0] - x0[0], m * delta / rho0, rtol=1e-10, atol=1e-16)
+    assert np.allclose(x1[1] - x0[1], -m * delta / rho0, rtol=1e-10, atol=1e-16)
+
+
+def test_xsph_uniform_velocity_unchanged(have_oracle):
+    sc = lattice_scene(jitter=0.004)
+    v = np.tile(np.array([0.3, -0.2, 0.1], np.float32), (sc.num_particles(), 1))
+    sc._vel = [v]
+    sc._has_vel = True
+    prm = quiet_params(xsph_c=0.01, lambda_denominator=1e30, scorr_k=0.0)
+    o, _, _ = run_one(sc, prm, fp64=True)
+    _, v1 = o.state_original_order()
+    assert np.abs(v1 - v).max() < 1e-8  # (x+v dt - x)/dt with an FP32 1/dt only
+
+
+def test_vorticity_rigid_rotation_closed_form(have_oracle):
+    h = 0.1
+    sc = lattice_scene(n=7, jitter=0.0)
+    x = sc.positions.astype(np.float64)
+    c = x.mean(0)
+    omega = np.array([0.0, 0.0, 2.0])
+    v = np.cross(omega, x - c)
+    sc._vel = [v.astype(np.float32)]
+    sc._has_vel = True
+    prm = quiet_params(vorticity_eps=1e-4, lambda_denominator=1e30, scorr_k=0.0)
+    o, _, _ = run_one(sc, prm, dt=1e-6)
+    om = o.debug(4)
+    _, _, mp = o.get_state()
+    centre = np.argmin(np.linalg.norm(x[mp] - c, axis=1))
+    # interior lattice particle: omega_i = (2/3) Omega sum_j c(r_j) r_j^2, c(r) = C2 (h-r)^2 / r
+    offs = np.array([(a, b, d) for a in range(-2, 3) for b in range(-2, 3) for d in range(-2, 3)
+                     if 0 < a * a + b * b + d * d <= 4], float) * 0.05
+    rr = np.linalg.norm(offs, axis=1)
+    c2 = -45.0 / (PI_REF * h ** 6)
+    expect = (2.0 / 3.0) * omega * np.sum(c2 * (h - rr) ** 2 / rr * rr * rr)
+    assert np.allclose(om[centre], expect, rtol=1e-4, atol=1e-6 * np.abs(expect).max())
+
+
+# ---------------------------------------------------------------- solids, cloth, boundary
+def test_plane_stabilization_pushes_to_radius(have_oracle):
+    sc = make_scene()
+    sc.add_granulars_block((1, 1, 1), (0.5, -0.1, 0.5), (0.05, 0.05, 0.05), 1.0)
+    sc.add_plane((0, 0, 0), (0, 1, 0))
+    prm = quiet_params(iterations=0, stabilization_passes=2)
+    o, _, _ = run_one(sc, prm)
+    x1, _ = o.state_original_order()
+    assert x1[0, 1] == pytest.approx(F32(0.025), abs=1e-15)
+
+
+def test_plane_collision_static_friction(have_oracle):
+    sc = make_scene()
+    sc.add_granulars_block((1, 1, 1), (0.5, 0.03, 0.5), (0.05, 0.05, 0.05), 1.0)
+    sc._vel = [np.array([[0.3, -6.0, 0.0]], np.float32)]
+    sc._has_vel = True
+    sc.add_plane((0, 0, 0), (0, 1, 0))
+    dt = F32(1.0 / 60.0)
+    # moving into the plane, diffNormal < 0, so the static test (particleplane.cuh:51)
+    # never passes; with dynamic friction 0 the tangential move is kept in full
+    o, x0, _ = run_one(sc, quiet_params(iterations=1, friction_static=0.5, friction_dynamic=0.0), dt=dt)
+    x1, _ = o.state_original_order()
+    assert x1[0, 1] == pytest.approx(F32(0.025), abs=1e-12)
+    assert x1[0, 0] == pytest.approx(x0[0, 0] + F32(0.3) * dt, abs=1e-12)
+    # dynamic friction saturating at 1 (min(mu_k |dn| / |dt|, 1)) removes it
+    o, x0, _ = run_one(sc, quiet_params(iterations=1, friction_static=0.5, friction_dynamic=10.0), dt=dt)
+    x1, _ = o.state_original_order()
+    assert x1[0, 1] == pytest.approx(F32(0.025), abs=1e-12)
+    assert x1[0, 0] == pytest.approx(x0[0, 0], abs=1e-12)
+
+
+def rot_quat(axis, ang):
+    axis = np.asarray(axis, float) / np.linalg.norm(axis)
+    return np.array([*(axis * math.sin(ang / 2)), math.cos(ang / 2)])
+
+
+def test_shape_matching_recovers_rigid_motion(have_oracle):
+    q = rot_quat((1, 2, 3), 0.7)
+    sc = make_scene()
+    sc.add_rigid_box((3, 3, 3), (1.0, 1.0, 1.0), 0.05, 1.0, rotation=q)
+    o, x0, _ = run_one(sc, quiet_params(iterations=1))
+    x1, _ = o.state_original_order()
+    qq, cm = o.rigid_state()
+    # the extraction stops once |omega|^2 <= 1e-9 (shapematching.cuh:83): ~3e-5 rad
+    assert np.abs(x1 - x0).max() < 5e-6
+    assert min(np.abs(qq[0] - q).max(), np.abs(qq[0] + q).max()) < 1e-4
+    assert np.allclose(cm[0], x0.mean(0), atol=1e-12)
+
+
+def test_rotation_extraction_matches_polar_decomposition(have_oracle):
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        R0 = scenes.quat_to_matrix(rot_quat(rng.normal(size=3), rng.uniform(0, 1.2)))
+        S = np.eye(3) + 0.2 * rng.normal(size=(3, 3))
+        S = 0.5 * (S + S.T) + 0.5 * np.eye(3)
+        A = R0 @ S
+        U, _, Vt = np.linalg.svd(A)
+        Rp = U @ Vt
+        q = orc.extract_rotation(A.T.reshape(9), np.array([0, 0, 0, 1.0]), iterations=400)
+        # converged to the stopping threshold |omega|^2 <= 1e-9 (shapematching.cuh:83)
+        assert np.allclose(scenes.quat_to_matrix(q), Rp, atol=1e-4)
+        assert not np.allclose(scenes.quat_to_matrix(q), R0, atol=1e-3) or np.allclose(Rp, R0, atol=1e-3)
+
+
+def test_distance_constraint_restores_rest_length(have_oracle):
+    sc = make_scene()
+    s, e = sc.add_rope((0.5, 0.5, 0.5), (0.2, 0.0, 0.0), 2, 1.0)
+    sc.distance_params[-1] = (0.15, 1.0)
+    o, x0, _ = run_one(sc, quiet_params(iterations=1))
+    x1, _ = o.state_original_order()
+    assert np.linalg.norm(x1[1] - x1[0]) == pytest.approx(F32(0.15), abs=1e-12)
+    assert np.allclose(x1.mean(0), x0.mean(0), atol=1e-12)  # equal masses: centre kept
+
+
+def bending_scene(angle):
+    sc = make_scene()
+    # hinge along x through p1 p2, wings p3, p4 at dihedral `angle`
+    pts = np.array([[0.5, 0.5, 0.5], [0.6, 0.5, 0.5],
+                    [0.55, 0.5 + 0.1, 0.5],
+                    [0.55, 0.5 + 0.1 * math.cos(angle), 0.5 + 0.1 * math.sin(angle)]])
+    for p in pts:
+        sc.add_granulars_block((1, 1, 1), p, (0.05, 0.05, 0.05), 1.0)
+    sc.bending_constraints.append((0, 1, 2, 3))
+    return sc, pts
+
+
+def dihedral(p):
+    n1 = np.cross(p[1] - p[0], p[2] - p[0])
+    n2 = np.cross(p[1] - p[0], p[3] - p[0])
+    return math.acos(np.clip(n1 @ n2 / np.linalg.norm(n1) / np.linalg.norm(n2), -1, 1))
+
+
+def test_bending_flat_is_fixed_point_and_bent_relaxes(have_oracle):
+    prm = quiet_params(iterations=1, friction_static=0.0)
+    sc, pts = bending_scene(math.pi)
+    o, x0, _ = run_one(sc, prm)
+    x1, _ = o.state_original_order()
+    assert np.abs(x1 - x0).max() < 1e-12
+    # bent hinge: dihedral angle between normals (acos d, bending.cuh:61) moves toward pi
+    sc, pts = bending_scene(1.2)
+    o, x0, _ = run_one(sc, prm)
+    x1, _ = o.state_original_order()
+    assert dihedral(x1) > dihedral(x0) + 1e-3
+
+
+def wind_scene(normal_along_flow: bool, v=(0.0, 0.0, -4.0)):
+    sc = make_scene()
+    if normal_along_flow:
+        tri = np.array([[0.5, 0.5, 0.5], [0.75, 0.5, 0.5], [0.5, 0.75, 0.5]])
+    else:
+        tri = np.array([[0.5, 0.5, 0.5], [0.75, 0.5, 0.5], [0.5, 0.5, 0.75]])
+    for p in tri:
+        sc.add_granulars_block((1, 1, 1), p, (0.05, 0.05, 0.05), 2.0)
+    sc.faces.append((0, 1, 2))
+    sc._vel = [np.tile(np.asarray(v, np.float32), (3, 1))]
+    sc._has_vel = True
+    return sc, tri
+
+
+def test_wind_drag_perpendicular_face(have_oracle):
+    sc, tri = wind_scene(True)
+    dt = F32(1.0 / 120.0)
+    o, x0, v0 = run_one(sc, quiet_params(iterations=0, drag_coeff=10.0, lift_coeff=100.0), dt=dt)
+    x1, _ = o.state_original_order()
+    move = x1 - x0 - v0 * dt
+    area = 0.5 * 0.25 * 0.25
+    vv = np.array([0.0, 0.0, -4.0])
+    force = -10.0 * (vv @ vv) * area * vv / np.linalg.norm(vv)  # lift is zero: n parallel to v
+    assert np.allclose(move, np.tile(force * dt * dt / 2.0, (3, 1)), rtol=1e-9, atol=1e-15)
+
+
+def test_wind_parallel_face_and_wind_vector(have_oracle):
+    sc, _ = wind_scene(False)
+    dt = F32(1.0 / 120.0)
+    o, x0, v0 = run_one(sc, quiet_params(iterations=0), dt=dt)
+    x1, _ = o.state_original_order()
+    assert np.abs(x1 - x0 - v0 * dt).max() < 1e-15
+    # air moving with the particles: zero relative velocity, zero force
+    sc, _ = wind_scene(True)
+    o, x0, v0 = run_one(sc, quiet_params(iterations=0, wind=(0.0, 0.0, -4.0)), dt=dt)
+    x1, _ = o.state_original_order()
+    assert np.abs(x1 - x0 - v0 * dt).max() < 1e-15
+
+
+def test_particle_particle_contact_separates_to_two_radii(have_oracle):
+    sc = make_scene()
+    sc.add_granulars_block((2, 1, 1), (0.5, 0.5, 0.5), (0.03, 0.05, 0.05), 1.0)
+    o, x0, _ = run_one(sc, quiet_params(iterations=1))
+    x1, _ = o.state_original_order()
+    assert np.linalg.norm(x1[1] - x1[0]) == pytest.approx(2 * F32(0.025), abs=1e-12)
+
+
+def test_immovable_and_pick(have_oracle):
+    sc = make_scene()
+    sc.add_granulars_block((3, 1, 1), (0.5, 0.5, 0.5), (0.2, 0.05, 0.05), 1.0)
+    sc.make_immovable(1)
+    sc._vel = [np.tile(np.array([1.0, 0, 0], np.float32), (3, 1))]
+    sc._has_vel = True
+    o, x0, _ = run_one(sc, quiet_params(iterations=2))
+    x1, v1 = o.state_original_order()
+    assert np.array_equal(x1[1], x0[1]) and np.all(v1[1] == 0.0)
+    assert x1[0, 0] > x0[0, 0]
+    # picking (particlesolver.cuh:790-794,1066-1071): pinned at the picked position, given velocity
+    o = orc.Oracle(bp.SceneArrays(sc), quiet_params(iterations=2), fp64=True)
+    o.step(2, 1 / 60, picked=2, pos=(1.0, 1.0, 1.0), vel=(0.5, 0.0, 0.0))
+    x2, v2 = o.state_original_order()
+    assert np.allclose(x2[2], (1.0, 1.0, 1.0)) and np.allclose(v2[2], (0.5, 0.0, 0.0))
