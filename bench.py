@@ -117,7 +117,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the solver shards by spatial slab only in design, see DESIGN.md)", file=sys.stderr)
+              "(multi-GPU slab sharding is a design, not part of this build)", file=sys.stderr)
         sys.exit(2)
 
     sc, prm, subdt = build_workload(args.workload)

@@ -28,7 +28,7 @@ __global__ __launch_bounds__(kBlock) void k_lambda(const float4 *__restrict__ xs
     int g[3];
     G.pos(pi, g);
     ccell[i] = G.pack(g);
-    if (phase_of(a) > 0) {  // solids carry no lambda (see DESIGN)
+    if (phase_of(a) > 0) {  // solids: lambda 0 (the reference never writes theirs)
         lambda[i] = 0.0f;
         density[i] = 0.0f;
         return;

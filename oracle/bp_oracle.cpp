@@ -739,8 +739,8 @@ template <class R> struct Solver : SolverBase {
         }
     }
     // fluidAkinciTension (fluid.cuh:436-489) -> vtmp, then swap (particlesolver.cuh:1048).
-    // Particles the kernel skips keep their velocity (see DESIGN: the reference
-    // leaves stale values there).
+    // Particles the kernel skips (non-fluids) keep their velocity; the
+    // reference leaves whatever the swap buffer held in those slots.
     void fluid_akinci(R dt) {
         const R g = R(P.tension_gamma);
         for (int i = 0; i < N; ++i) {
