@@ -474,7 +474,11 @@ template <class R> struct Solver : SolverBase {
     void fluid_lambda() {
         const R sds = R(P.solid_density_scale);
         for (int i = 0; i < N; ++i) {
-            if (phase[i] > 0) continue;
+            if (phase[i] > 0) {  // solids carry lambda 0 (the reference leaves stale values)
+                lambda[i] = R(0);
+                density[i] = R(0);
+                continue;
+            }
             const V3<R> pi = xs[i];
             R dens = 0, sumg2 = 0;
             V3<R> gi{0, 0, 0};
