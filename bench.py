@@ -139,11 +139,16 @@ def main():
         "kernel_ms": fam_times,
     }
     if not args.no_large and args.workload != "large_block_4m":
+        # The large block's Akinci tension (gamma 0.5, no mass weighting,
+        # fluid.cuh:436-489) drives it unstable after ~20 substeps; time
+        # substeps 3-12 from the lattice.
         lsc, lprm, lsub = build_workload("large_block_4m")
-        ls, lwall = time_plain(lsc, lprm, lsub, 20, 5)
-        lroof, ltimes = kernel_roofline(ls, lsub)
-        out["large_block_4m"] = {"value": round(ls.n * 20 / lwall, 1), "ms_per_step": round(lwall / 20 * 1e3, 4),
-                                 "steps": 20, "warmup": 5, "roofline": lroof, "kernel_ms": ltimes}
+        lsteps, lwarm = 10, 2
+        ls, lwall = time_plain(lsc, lprm, lsub, lsteps, lwarm)
+        lroof, ltimes = kernel_roofline(ls, lsub, substeps=2)
+        out["large_block_4m"] = {"value": round(ls.n * lsteps / lwall, 1),
+                                 "ms_per_step": round(lwall / lsteps * 1e3, 4), "steps": lsteps,
+                                 "warmup": lwarm, "roofline": lroof, "kernel_ms": ltimes}
         ls.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, prm, subdt)

@@ -40,42 +40,43 @@ __global__ __launch_bounds__(kBlock) void k_particle_particle(const float4 *__re
     G.pos(xip, g);
     const float r2x4 = r * r * 4.0f;
     const float thr = r * r * 0.001f * 0.001f;
-    for_each_run(G, ranges, g, [&](int s, int e) {
-        for (int j = s; j < e; ++j) {
-            const float4 b = xs[j];
+    for_each_neighbor(
+        G, ranges, g, xs, 0,
+        [&](int j, float4 b) {
             const int pj = phase_of(b);
-            if (i != j && pi != pj && (pi > 0 || pj > 0)) {
-                const f3 xjp = xyz(b);
-                const f3 diff = xip - xjp;
-                const float d2 = length2(diff);
-                if (d2 < r2x4 && d2 > 1e-5f) {
-                    const float dist = sqrtf(d2);
-                    const float4 xb = x[j];
-                    const float wj = xb.w;
-                    const float w1 = wi / (wi + wj);
-                    const float w2 = wj / (wi + wj);
-                    const f3 proj = diff * (2.0f * r / dist - 1.0f);
-                    const f3 dxi = w1 * proj;
-                    const f3 xistar = dxi + xip;
-                    sumD += dxi;
-                    const f3 dxj = -w2 * proj;
-                    const float l2 = length2(dxi);
-                    if (l2 > thr) {
-                        cnt += 1;
-                        const f3 xjstar = dxj + xjp;
-                        const f3 term1 = (xistar - xi) - (xjstar - xyz(xb));
-                        const f3 nn = diff / dist;
-                        const f3 tan = term1 - dot(term1, nn) * nn;
-                        const float tl2 = length2(tan);
-                        if (tl2 <= (fs * fs) * l2)
-                            sumF -= w1 * tan;
-                        else
-                            sumF -= w1 * tan * fminf(fd * sqrtf(l2 / tl2), 1.0f);
-                    }
-                }
+            if (!(i != j && pi != pj && (pi > 0 || pj > 0))) return false;
+            const float d2 = dist2(xip, xyz(b));
+            return d2 < r2x4 && d2 > 1e-5f;
+        },
+        [&](int j, float4 b) {
+            const f3 xjp = xyz(b);
+            const f3 diff = xip - xjp;
+            const float dist = fsqrt(dist2(xip, xjp));
+            const float idist = frcp(dist);
+            const float4 xb = x[j];
+            const float wj = xb.w;
+            const float iw = frcp(wi + wj);
+            const float w1 = wi * iw;
+            const float w2 = wj * iw;
+            const f3 proj = diff * (2.0f * r * idist - 1.0f);
+            const f3 dxi = w1 * proj;
+            const f3 xistar = dxi + xip;
+            sumD += dxi;
+            const f3 dxj = -w2 * proj;
+            const float l2 = length2(dxi);
+            if (l2 > thr) {
+                cnt += 1;
+                const f3 xjstar = dxj + xjp;
+                const f3 term1 = (xistar - xi) - (xjstar - xyz(xb));
+                const f3 nn = diff * idist;
+                const f3 tan = term1 - dot(term1, nn) * nn;
+                const float tl2 = length2(tan);
+                if (tl2 <= (fs * fs) * l2)
+                    sumF -= w1 * tan;
+                else
+                    sumF -= w1 * tan * fminf(fd * fsqrt(l2 * frcp(tl2)), 1.0f);
             }
-        }
-    });
+        });
     const f3 d = (cnt == 0) ? sumD : sumD + sumF / (float)cnt;
     xs_out[i] = f4(xip + d, a.w);
 }

@@ -36,23 +36,24 @@ __global__ __launch_bounds__(kBlock) void k_lambda(const float4 *__restrict__ xs
     float dens = 0.0f, sumg2 = 0.0f;
     f3 gi = mk3(0.f, 0.f, 0.f);
     const float sds = F.solid_density_scale;
-    for_each_run(G, ranges, g, [&](int s, int e) {
-        for (int j = s > i ? s : i; j < e; ++j) {  // Q1: only j >= i (fluid.cuh:117)
-            const float4 b = xs[j];
-            const f3 diff = pi - xyz(b);
-            const float d2 = length2(diff);
-            if (d2 <= K.h2) {
-                const float mj = m[j];
-                const float ld = mj * K.poly6(d2);
-                dens += (phase_of(b) < 0) ? ld : sds * ld;
-                if (F.lambda_mode == 1 && d2 > 0.0f) {
-                    const f3 gr = (-mj * K.grad(diff, d2)) / F.rest_density;
-                    sumg2 += dot(gr, gr);
-                    gi -= gr;
-                }
+    // Q1: only j >= i (fluid.cuh:117)
+    const float inv_rho0 = F.inv_rest_density;
+    for_each_neighbor(
+        G, ranges, g, xs, i, [&](int, float4 b) { return dist2(pi, xyz(b)) <= K.h2; },
+        [&](int j, float4 b) {
+            const float d2 = dist2(pi, xyz(b));
+            const float mj = m[j];
+            const float ld = mj * K.poly6(d2);
+            dens += (phase_of(b) < 0) ? ld : sds * ld;
+            if (F.lambda_mode == 1 && d2 > 0.0f) {
+                // grad C_j = -m_j gradW / rho0 (fluid.cuh:132-137)
+                const float r = fsqrt(d2);
+                const float gc = -mj * K.grad_coef(r, frcp(r)) * inv_rho0;
+                const f3 gr = gc * (pi - xyz(b));
+                sumg2 += gc * gc * d2;
+                gi -= gr;
             }
-        }
-    });
+        });
     if (F.lambda_mode == 1) sumg2 += dot(gi, gi);
     float C = dens / F.rest_density - 1.0f;
     if (F.akinci) C = fmaxf(C, 0.0f);
@@ -60,13 +61,17 @@ __global__ __launch_bounds__(kBlock) void k_lambda(const float4 *__restrict__ xs
     density[i] = dens;
 }
 
-// fluidPosition pair term (fluid.cuh:199-221): delta of pair (k,j) seen from k.
-__device__ __forceinline__ f3 pair_delta(f3 diff, float d2, float lam_sum, const SphConst &K, const FluidParams &F) {
+// fluidPosition pair term (fluid.cuh:199-221) of pair (k,j) seen from k, as a
+// multiple of diff = x_k - x_j: delta = c diff with c = lambda_sum * gradW / r,
+// so |delta| = |c| r and the clamp |delta| <= 5 becomes |c| <= 5 / r.
+__device__ __forceinline__ float pair_coef(float d2, float lam_sum, const SphConst &K, const FluidParams &F) {
     float sl = lam_sum;
-    if (!F.akinci) sl += -F.scorr_k * powf(K.poly6(d2) / K.wdq, F.scorr_n);
-    f3 delta = sl * K.grad(diff, d2);
-    const float len = length(delta);
-    return (len < F.delta_clamp) ? delta : delta * fminf(len, F.delta_clamp) / len;
+    if (!F.akinci) sl += -F.scorr_k * ipowf(K.poly6(d2) / K.wdq, F.scorr_n);
+    const float r = fsqrt(d2);
+    const float ir = frcp(r);
+    float c = sl * K.grad_coef(r, ir);
+    if (fabsf(c) * r >= F.delta_clamp) c = copysignf(F.delta_clamp * ir, c);
+    return c;
 }
 
 // fluidPosition (fluid.cuh:161-228) + Δx apply (particlesolver.cuh:919-932)
@@ -94,20 +99,21 @@ __global__ __launch_bounds__(kBlock) void k_position(const float4 *__restrict__ 
     int g[3];
     unpack_cell(ck, g);
     f3 sum = mk3(0.f, 0.f, 0.f);
-    for_each_run(G, ranges, g, [&](int s, int e) {
-        const int e2 = kf ? e : (e < k ? e : k);
-        for (int j = s; j < e2; ++j) {
-            if (j == k) continue;
-            const float4 b = xs[j];
-            if (j < k && phase_of(b) > 0) continue;
+    for_each_neighbor(
+        G, ranges, g, xs, 0,
+        [&](int j, float4 b) {
+            if (j == k || (j > k && !kf) || (j < k && phase_of(b) > 0)) return false;
+            const float d2 = dist2(pk, xyz(b));
+            return d2 <= K.h2 && d2 > 0.0f;
+        },
+        [&](int j, float4 b) {
+            if (j < k && !G.in_block(ccell[j], bk)) return;
             const f3 diff = pk - xyz(b);
-            const float d2 = length2(diff);
-            if (!(d2 <= K.h2 && d2 > 0.0f)) continue;
-            if (j < k && !G.in_block(ccell[j], bk)) continue;
-            const f3 delta = pair_delta(diff, d2, lk + lambda[j], K, F);
-            sum += m[j] * delta;
-        }
-    });
+            const float c = pair_coef(dist2(pk, xyz(b)), lk + lambda[j], K, F) * m[j];
+            sum.x = fmaf(c, diff.x, sum.x);
+            sum.y = fmaf(c, diff.y, sum.y);
+            sum.z = fmaf(c, diff.z, sum.z);
+        });
     f3 out = pk + sum * F.inv_rest_density;
     if (E.planes_next || E.finish) {
         float4 x4 = x_io[k];
@@ -140,21 +146,21 @@ __global__ __launch_bounds__(kBlock) void k_post_a(const float4 *__restrict__ xs
     ccell[i] = G.pack(g);
     const f3 vi = xyz(v[i]);
     f3 om = mk3(0.f, 0.f, 0.f), nn = mk3(0.f, 0.f, 0.f);
-    for_each_run(G, ranges, g, [&](int s, int e) {
-        for (int j = s; j < e; ++j) {
-            if (j == i) continue;
-            const float4 b = xs[j];
-            if (phase_of(b) != ph) continue;
+    for_each_neighbor(
+        G, ranges, g, xs, 0,
+        [&](int j, float4 b) {
+            if (j == i || phase_of(b) != ph) return false;
+            const float d2 = dist2(pi, xyz(b));
+            return d2 <= K.h2 && d2 > 0.0f;
+        },
+        [&](int j, float4 b) {
             const f3 diff = pi - xyz(b);
-            const float d2 = length2(diff);
-            if (d2 <= K.h2 && d2 > 0.0f) {
-                const f3 gr = K.grad(diff, d2);
-                if (do_omega) om += cross(xyz(v[j]) - vi, gr);
-                if (do_normal) nn += (1.0f / density[j]) * gr;
-            }
-        }
-    });
-    if (do_omega) omega[i] = f4(om, length(om));
+            const float r = fsqrt(dist2(pi, xyz(b)));
+            const f3 gr = K.grad_coef(r, frcp(r)) * diff;
+            if (do_omega) om += cross(xyz(v[j]) - vi, gr);
+            if (do_normal) nn += frcp(density[j]) * gr;
+        });
+    if (do_omega) omega[i] = f4(om, fsqrt(dot(om, om)));
     if (do_normal) normal[i] = f4(K.h * nn, 0.0f);
 }
 
@@ -180,26 +186,26 @@ __global__ __launch_bounds__(kBlock) void k_post_b(const float4 *__restrict__ xs
     const f3 ni = do_akinci ? xyz(normal[i]) : mk3(0.f, 0.f, 0.f);
     const float rhoi = do_akinci ? density[i] : 0.0f;
     const float gam = F.tension_gamma;
-    for_each_run(G, ranges, g, [&](int s, int e) {
-        for (int j = s; j < e; ++j) {
-            if (j == i) continue;
-            const float4 b = xs[j];
-            if (phase_of(b) != ph) continue;
-            const f3 pj = xyz(b);
-            const f3 diff = pi - pj;
-            const float d2 = length2(diff);
-            if (d2 <= K.h2 && d2 > 0.0f) {
-                if (do_vort) eta += omega[j].w * K.grad(diff, d2);
-                if (do_akinci) {
-                    const float dist = sqrtf(d2);
-                    const f3 fcoh = -gam * K.akinci(dist) * (pi - pj) / dist;
-                    const f3 fcur = -gam * (ni - xyz(normal[j]));
-                    const float kij = 2.0f * F.rest_density / (rhoi + density[j]);
-                    f += kij * (fcoh + fcur);
-                }
+    for_each_neighbor(
+        G, ranges, g, xs, 0,
+        [&](int j, float4 b) {
+            if (j == i || phase_of(b) != ph) return false;
+            const float d2 = dist2(pi, xyz(b));
+            return d2 <= K.h2 && d2 > 0.0f;
+        },
+        [&](int j, float4 b) {
+            const f3 diff = pi - xyz(b);
+            const float r = fsqrt(dist2(pi, xyz(b)));
+            const float ir = frcp(r);
+            if (do_vort) eta += (omega[j].w * K.grad_coef(r, ir)) * diff;
+            if (do_akinci) {
+                // k_ij (-gamma C(r) diff / r - gamma (n_i - n_j)) (fluid.cuh:476-483)
+                const float kij = 2.0f * F.rest_density * frcp(rhoi + density[j]);
+                const f3 fcoh = (-gam * K.akinci(r) * ir) * diff;
+                const f3 fcur = -gam * (ni - xyz(normal[j]));
+                f += kij * (fcoh + fcur);
             }
-        }
-    });
+        });
     f3 vi = xyz(v[i]);
     if (do_vort && length2(eta) > 1e-3f) {
         const f3 nrm = eta * (1.0f / sqrtf(dot(eta, eta)));
@@ -231,16 +237,10 @@ __global__ __launch_bounds__(kBlock) void k_xsph(const float4 *__restrict__ xs, 
         G.pos(pi, g);
     const f3 vi = xyz(v4);
     f3 vn = mk3(0.f, 0.f, 0.f);
-    for_each_run(G, ranges, g, [&](int s, int e) {
-        for (int j = s; j < e; ++j) {
-            if (j == i) continue;
-            const float4 b = xs[j];
-            if (phase_of(b) != ph) continue;
-            const f3 diff = pi - xyz(b);
-            const float d2 = length2(diff);
-            if (d2 <= K.h2) vn += (xyz(v[j]) - vi) * K.poly6(d2);
-        }
-    });
+    for_each_neighbor(
+        G, ranges, g, xs, 0,
+        [&](int j, float4 b) { return j != i && phase_of(b) == ph && dist2(pi, xyz(b)) <= K.h2; },
+        [&](int j, float4 b) { vn += (xyz(v[j]) - vi) * K.poly6(dist2(pi, xyz(b))); });
     v_out[i] = f4(vi + c * vn, 0.0f);
 }
 

@@ -34,6 +34,26 @@ __host__ __device__ __forceinline__ f3 cross(f3 a, f3 b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 __device__ __forceinline__ f3 xyz(float4 a) { return f3{a.x, a.y, a.z}; }
+
+// Pair-term arithmetic.  Squared distances use FMA; square roots and
+// reciprocals in per-pair terms use the 1-ulp hardware instructions
+// (v_sqrt_f32 / v_rcp_f32).  The grid hash keeps IEEE division (GridDesc::pos).
+__device__ __forceinline__ float dist2(f3 a, f3 b) {
+    const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+}
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// x^n for a small non-negative integer n (sCorr exponent, fluid.cuh:208)
+__device__ __forceinline__ float ipowf(float x, int n) {
+    float r = 1.0f;
+    while (n > 0) {
+        if (n & 1) r *= x;
+        x *= x;
+        n >>= 1;
+    }
+    return r;
+}
 __device__ __forceinline__ float4 f4(f3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
 
 // ------------------------------------------------------------ parameters
@@ -91,6 +111,11 @@ struct SphConst {
         float s = c2 * t * t;
         return (s * v) / r;
     }
+    // gradientSpikyKernel(v, r^2) = grad_coef(r, 1/r) * v
+    __device__ __forceinline__ float grad_coef(float r, float ir) const {
+        const float t = h - r;
+        return c2 * t * t * ir;
+    }
     __device__ __forceinline__ float akinci(float r) const {
         float t = (h - r) * r;
         float t3 = t * t * t;
@@ -104,7 +129,8 @@ struct FluidParams {
     int lambda_mode;
     float lambda_denominator, lambda_relaxation;
     int akinci;
-    float scorr_k, scorr_n, delta_clamp;
+    float scorr_k, delta_clamp;
+    int scorr_n;
     float vorticity_eps, tension_gamma, xsph_c;
 };
 
@@ -174,6 +200,33 @@ __device__ __forceinline__ void for_each_run(const GridDesc &G, const int2 *__re
             }
         }
     }
+}
+
+// Two-phase neighbour walk: for every run, chunks of up to 32 candidates are
+// first screened with the cheap test `pick(j, b)` (b = xs[j]) into a bitmask,
+// then `body(j, b)` runs only for the picked candidates, in ascending j.  The
+// expensive pair math therefore executes max(picked) times per wave instead
+// of once per candidate, and the visiting order is unchanged.
+template <class Pick, class Body>
+__device__ __forceinline__ void for_each_neighbor(const GridDesc &G, const int2 *__restrict__ ranges,
+                                                  const int g[3], const float4 *__restrict__ xs, int jmin,
+                                                  Pick &&pick, Body &&body) {
+    for_each_run(G, ranges, g, [&](int s, int e) {
+        if (s < jmin) s = jmin;
+        for (int base = s; base < e; base += 32) {
+            const int cnt = (e - base) < 32 ? (e - base) : 32;
+            uint32_t mask = 0u;
+            for (int t = 0; t < cnt; ++t) {
+                const float4 b = xs[base + t];
+                if (pick(base + t, b)) mask |= 1u << t;
+            }
+            while (mask) {
+                const int t = __ffs(mask) - 1;
+                mask &= mask - 1u;
+                body(base + t, xs[base + t]);
+            }
+        }
+    });
 }
 
 inline int grid_blocks(long long n, int block = kBlock) { return (int)((n + block - 1) / block); }

@@ -210,7 +210,7 @@ void set_derived_params(bp_solver *s) {
     s->F.lambda_relaxation = P.lambda_relaxation;
     s->F.akinci = P.akinci;
     s->F.scorr_k = P.scorr_k;
-    s->F.scorr_n = (float)P.scorr_n;
+    s->F.scorr_n = P.scorr_n;
     s->F.delta_clamp = P.delta_clamp;
     s->F.vorticity_eps = P.vorticity_eps;
     s->F.tension_gamma = P.tension_gamma;
