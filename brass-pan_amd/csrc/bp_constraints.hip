@@ -44,9 +44,8 @@ __global__ __launch_bounds__(kBlock) void k_particle_particle(const float4 *__re
         G, ranges, g, xs, 0,
         [&](int j, float4 b) {
             const int pj = phase_of(b);
-            if (!(i != j && pi != pj && (pi > 0 || pj > 0))) return false;
             const float d2 = dist2(xip, xyz(b));
-            return d2 < r2x4 && d2 > 1e-5f;
+            return (i != j) & (pi != pj) & ((pi > 0) | (pj > 0)) & (d2 < r2x4) & (d2 > 1e-5f);
         },
         [&](int j, float4 b) {
             const f3 xjp = xyz(b);

@@ -74,26 +74,36 @@ __device__ __forceinline__ float pair_coef(float d2, float lam_sum, const SphCon
     return c;
 }
 
-// fluidPosition (fluid.cuh:161-228) + Δx apply (particlesolver.cuh:919-932)
-// in gather form.  Pair (i<j) exists in the reference iff particle i is not a
-// solid (fluid.cuh:176) and j lies in i's search cells; slot k therefore sums
+// Fixed-point scale of the pair corrections delivered by atomics: integer
+// addition makes their sum independent of arrival order.
+constexpr double kCorrScale = 1099511627776.0;  // 2^40
+__device__ __forceinline__ unsigned long long to_fixed(float v) {
+    return (unsigned long long)(long long)rint((double)v * kCorrScale);
+}
+
+// fluidPosition (fluid.cuh:161-228) in gather form, with the reference's pair
+// set.  The pair (i<j) exists iff particle i is not a solid (fluid.cuh:176)
+// and j's cell (from the grid build) lies in the 27 cells around i's current
+// cell.  Slot k sums, as m_j delta(k,j) with delta(k,j) = -delta(j,k):
 //   partners j > k found by its own search (k not solid), and
-//   partners j < k (j not solid) whose search around their current cell
-//   contains k's cell,
-// each as m_j δ(k,j), δ(k,j) = -δ(j,k) exactly.
+//   partners j < k (not solid) found by its own search whose search contains
+//   k's cell.
+// A pair k owns whose partner j's search does not reach k's cell is delivered
+// to j through corr[j] (fixed point).  k_position_apply adds both parts.
 __global__ __launch_bounds__(kBlock) void k_position(const float4 *__restrict__ xs, const float *__restrict__ m,
                                                      const float *__restrict__ lambda,
                                                      const uint32_t *__restrict__ ccell,
                                                      const int *__restrict__ build_keys,
-                                                     const int2 *__restrict__ ranges, float4 *__restrict__ xs_out,
-                                                     float4 *__restrict__ x_io, float4 *__restrict__ v_out, int n,
-                                                     GridDesc G, SphConst K, FluidParams F, Planes pl, Epilogue E) {
+                                                     const int2 *__restrict__ ranges, float4 *__restrict__ sum_out,
+                                                     unsigned long long *__restrict__ corr, int n, GridDesc G,
+                                                     SphConst K, FluidParams F) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const float4 a = xs[k];
     const f3 pk = xyz(a);
     const bool kf = phase_of(a) <= 0;
     const float lk = lambda[k];
+    const float mk = m[k];
     const uint32_t ck = ccell[k];
     const uint32_t bk = G.pack_addr(build_keys[k]);
     int g[3];
@@ -102,19 +112,52 @@ __global__ __launch_bounds__(kBlock) void k_position(const float4 *__restrict__ 
     for_each_neighbor(
         G, ranges, g, xs, 0,
         [&](int j, float4 b) {
-            if (j == k || (j > k && !kf) || (j < k && phase_of(b) > 0)) return false;
+            const bool own = (j != k) & (kf | (j < k)) & ((j > k) | (phase_of(b) <= 0));
             const float d2 = dist2(pk, xyz(b));
-            return d2 <= K.h2 && d2 > 0.0f;
+            return own & (d2 <= K.h2) & (d2 > 0.0f);
         },
         [&](int j, float4 b) {
-            if (j < k && !G.in_block(ccell[j], bk)) return;
+            const bool partner_sees_k = G.in_block(ccell[j], bk);
+            if (j < k && !partner_sees_k) return;
             const f3 diff = pk - xyz(b);
-            const float c = pair_coef(dist2(pk, xyz(b)), lk + lambda[j], K, F) * m[j];
-            sum.x = fmaf(c, diff.x, sum.x);
-            sum.y = fmaf(c, diff.y, sum.y);
-            sum.z = fmaf(c, diff.z, sum.z);
+            const float c = pair_coef(dist2(pk, xyz(b)), lk + lambda[j], K, F);
+            const float cj = c * m[j];
+            sum.x = fmaf(cj, diff.x, sum.x);
+            sum.y = fmaf(cj, diff.y, sum.y);
+            sum.z = fmaf(cj, diff.z, sum.z);
+            if (j > k && !partner_sees_k) {  // j will not find this pair: deliver -m_k delta(k,j)
+                const float ck2 = -c * mk;
+                atomicAdd(&corr[3 * j + 0], to_fixed(ck2 * diff.x));
+                atomicAdd(&corr[3 * j + 1], to_fixed(ck2 * diff.y));
+                atomicAdd(&corr[3 * j + 2], to_fixed(ck2 * diff.z));
+            }
         });
-    f3 out = pk + sum * F.inv_rest_density;
+    sum_out[k] = f4(sum, 0.0f);
+}
+
+// Δx = (own sum + delivered pairs) / rho0, x* += Δx (particlesolver.cuh:919-932)
+// and the fused epilogue; clears corr for the next pass.
+__global__ __launch_bounds__(kBlock) void k_position_apply(const float4 *__restrict__ xs,
+                                                           const float4 *__restrict__ sum_in,
+                                                           unsigned long long *__restrict__ corr,
+                                                           float4 *__restrict__ xs_out, float4 *__restrict__ x_io,
+                                                           float4 *__restrict__ v_out, int n, FluidParams F,
+                                                           Planes pl, Epilogue E) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float4 a = xs[k];
+    f3 sum = xyz(sum_in[k]);
+    const unsigned long long cx = corr[3 * k], cy = corr[3 * k + 1], cz = corr[3 * k + 2];
+    if ((cx | cy | cz) != 0ull) {
+        const float inv = (float)(1.0 / kCorrScale);
+        sum.x += (float)(long long)cx * inv;
+        sum.y += (float)(long long)cy * inv;
+        sum.z += (float)(long long)cz * inv;
+        corr[3 * k] = 0ull;
+        corr[3 * k + 1] = 0ull;
+        corr[3 * k + 2] = 0ull;
+    }
+    f3 out = xyz(a) + sum * F.inv_rest_density;
     if (E.planes_next || E.finish) {
         float4 x4 = x_io[k];
         const f3 xp = xyz(x4);
@@ -149,9 +192,8 @@ __global__ __launch_bounds__(kBlock) void k_post_a(const float4 *__restrict__ xs
     for_each_neighbor(
         G, ranges, g, xs, 0,
         [&](int j, float4 b) {
-            if (j == i || phase_of(b) != ph) return false;
             const float d2 = dist2(pi, xyz(b));
-            return d2 <= K.h2 && d2 > 0.0f;
+            return (j != i) & (phase_of(b) == ph) & (d2 <= K.h2) & (d2 > 0.0f);
         },
         [&](int j, float4 b) {
             const f3 diff = pi - xyz(b);
@@ -189,9 +231,8 @@ __global__ __launch_bounds__(kBlock) void k_post_b(const float4 *__restrict__ xs
     for_each_neighbor(
         G, ranges, g, xs, 0,
         [&](int j, float4 b) {
-            if (j == i || phase_of(b) != ph) return false;
             const float d2 = dist2(pi, xyz(b));
-            return d2 <= K.h2 && d2 > 0.0f;
+            return (j != i) & (phase_of(b) == ph) & (d2 <= K.h2) & (d2 > 0.0f);
         },
         [&](int j, float4 b) {
             const f3 diff = pi - xyz(b);
@@ -239,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_xsph(const float4 *__restrict__ xs, 
     f3 vn = mk3(0.f, 0.f, 0.f);
     for_each_neighbor(
         G, ranges, g, xs, 0,
-        [&](int j, float4 b) { return j != i && phase_of(b) == ph && dist2(pi, xyz(b)) <= K.h2; },
+        [&](int j, float4 b) { return (j != i) & (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
         [&](int j, float4 b) { vn += (xyz(v[j]) - vi) * K.poly6(dist2(pi, xyz(b))); });
     v_out[i] = f4(vi + c * vn, 0.0f);
 }
@@ -252,11 +293,13 @@ void launch_lambda(const float4 *xs, const float *m, const int2 *ranges, float *
                        n, G, K, F);
 }
 void launch_position(const float4 *xs, const float *m, const float *lambda, const uint32_t *ccell,
-                     const int *build_keys, const int2 *ranges, float4 *xs_out, float4 *x_io, float4 *v_out, int n,
-                     const GridDesc &G, const SphConst &K, const FluidParams &F, const Planes &pl,
-                     const Epilogue &E, hipStream_t st) {
+                     const int *build_keys, const int2 *ranges, float4 *sum_tmp, unsigned long long *corr,
+                     float4 *xs_out, float4 *x_io, float4 *v_out, int n, const GridDesc &G, const SphConst &K,
+                     const FluidParams &F, const Planes &pl, const Epilogue &E, hipStream_t st) {
     hipLaunchKernelGGL(k_position, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, m, lambda, ccell, build_keys,
-                       ranges, xs_out, x_io, v_out, n, G, K, F, pl, E);
+                       ranges, sum_tmp, corr, n, G, K, F);
+    hipLaunchKernelGGL(k_position_apply, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, sum_tmp, corr, xs_out,
+                       x_io, v_out, n, F, pl, E);
 }
 void launch_post_a(const float4 *xs, const float4 *v, const float *density, const int2 *ranges, float4 *omega,
                    float4 *normal, uint32_t *ccell, int n, const GridDesc &G, const SphConst &K, int do_omega,

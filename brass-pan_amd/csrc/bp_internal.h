@@ -216,9 +216,20 @@ __device__ __forceinline__ void for_each_neighbor(const GridDesc &G, const int2 
         for (int base = s; base < e; base += 32) {
             const int cnt = (e - base) < 32 ? (e - base) : 32;
             uint32_t mask = 0u;
-            for (int t = 0; t < cnt; ++t) {
-                const float4 b = xs[base + t];
-                if (pick(base + t, b)) mask |= 1u << t;
+            // batches of 8 independent loads in flight; indices past the run
+            // end are clamped to its last slot and masked out
+            for (int t0 = 0; t0 < cnt; t0 += 8) {
+                float4 bb[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int j = base + t0 + u;
+                    bb[u] = xs[j < e ? j : e - 1];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const bool ok = (t0 + u < cnt) & pick(base + t0 + u, bb[u]);
+                    mask |= (uint32_t)ok << (t0 + u);
+                }
             }
             while (mask) {
                 const int t = __ffs(mask) - 1;

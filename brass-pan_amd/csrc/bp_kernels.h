@@ -22,9 +22,9 @@ void launch_lambda(const float4 *xs, const float *m, const int2 *ranges, float *
                    uint32_t *ccell, int n, const GridDesc &G, const SphConst &K, const FluidParams &F,
                    hipStream_t st);
 void launch_position(const float4 *xs, const float *m, const float *lambda, const uint32_t *ccell,
-                     const int *build_keys, const int2 *ranges, float4 *xs_out, float4 *x_io, float4 *v_out, int n,
-                     const GridDesc &G, const SphConst &K, const FluidParams &F, const Planes &pl,
-                     const Epilogue &E, hipStream_t st);
+                     const int *build_keys, const int2 *ranges, float4 *sum_tmp, unsigned long long *corr,
+                     float4 *xs_out, float4 *x_io, float4 *v_out, int n, const GridDesc &G, const SphConst &K,
+                     const FluidParams &F, const Planes &pl, const Epilogue &E, hipStream_t st);
 void launch_post_a(const float4 *xs, const float4 *v, const float *density, const int2 *ranges, float4 *omega,
                    float4 *normal, uint32_t *ccell, int n, const GridDesc &G, const SphConst &K, int do_omega,
                    int do_normal, hipStream_t st);

@@ -97,6 +97,8 @@ struct bp_solver {
 
     float *lambda = nullptr, *density = nullptr;
     uint32_t *ccell = nullptr;
+    float4 *sum_tmp = nullptr;
+    unsigned long long *corr = nullptr;  // fixed-point pair corrections, kept zero between passes
     float4 *omega = nullptr, *normal = nullptr;
 
     int nd = 0;
@@ -309,8 +311,8 @@ bp_status substep(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick, 
             E.friction_static = fs;
             E.friction_dynamic = fd;
             TIMED(F_POSITION, launch_position(s->xs.cur(), s->m.cur(), s->lambda, s->ccell, sk, s->ranges,
-                                              s->xs.alt(), s->x.cur(), s->v.cur(), N, s->G, s->K, s->F, s->planes, E,
-                                              st));
+                                              s->sum_tmp, s->corr, s->xs.alt(), s->x.cur(), s->v.cur(), N, s->G,
+                                              s->K, s->F, s->planes, E, st));
             s->xs.flip();
             planes_done = E.planes_next;
             finished = E.finish;
@@ -530,6 +532,8 @@ bp_status bp_solver_create(const bp_scene_desc *sc, const bp_params *params, voi
     TRY(s->alloc(&s->lambda, N));
     TRY(s->alloc(&s->density, N));
     TRY(s->alloc(&s->ccell, N));
+    TRY(s->alloc(&s->sum_tmp, N));
+    TRY(s->alloc(&s->corr, 3 * (size_t)N));
     TRY(s->alloc(&s->omega, N));
     TRY(s->alloc(&s->normal, N));
     if (N > 0) {
@@ -543,6 +547,7 @@ bp_status bp_solver_create(const bp_scene_desc *sc, const bp_params *params, voi
         HIP_TRY(hipMemsetAsync(s->density, 0, N * sizeof(float), s->st));
         HIP_TRY(hipMemsetAsync(s->omega, 0, N * sizeof(float4), s->st));
         HIP_TRY(hipMemsetAsync(s->normal, 0, N * sizeof(float4), s->st));
+        HIP_TRY(hipMemsetAsync(s->corr, 0, 3 * (size_t)N * sizeof(unsigned long long), s->st));
     }
     HIP_TRY(hipMemsetAsync(s->ranges, 0xFF, (size_t)s->num_cells * sizeof(int2), s->st));
 

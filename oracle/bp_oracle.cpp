@@ -546,6 +546,11 @@ template <class R> struct Solver : SolverBase {
             }
         } else {
             // gather: each k sums m_j δ(k,j) over the pairs the reference forms
+            // that its own search finds; a pair its owner k finds but the
+            // partner j's search misses is delivered to j in 2^-40 fixed point
+            // (order-independent integer sums), as the GPU kernel does.
+            std::vector<long long> corr((size_t)3 * N, 0);
+            std::vector<V3<R>> sums(N);
             for (int k = 0; k < N; ++k) {
                 const V3<R> pk = xs[k];
                 const int bk = sorted_keys[k];
@@ -560,15 +565,30 @@ template <class R> struct Solver : SolverBase {
                     V3<R> diff = pk - pj;
                     R d2 = length2(diff);
                     if (!(d2 <= K.h2 && d2 > R(0))) return;
-                    if (j < k) {
-                        // owner j searched around its current cell
-                        int g[3];
-                        cur_cell_coords(pj, g);
-                        if (!in_block(g, bk)) return;
-                    }
+                    int gj[3];
+                    cur_cell_coords(pj, gj);
+                    const bool partner_sees_k = in_block(gj, bk);
+                    if (j < k && !partner_sees_k) return;  // owner j's search does not reach k
                     V3<R> delta = pair_delta(diff, d2, lk + lambda[j], wdq);
                     sum += mass[j] * delta;
+                    if (j > k && !partner_sees_k) {
+                        V3<R> give = -mass[k] * delta;
+                        corr[3 * (size_t)j + 0] += std::llrint((double)give.x * 1099511627776.0);
+                        corr[3 * (size_t)j + 1] += std::llrint((double)give.y * 1099511627776.0);
+                        corr[3 * (size_t)j + 2] += std::llrint((double)give.z * 1099511627776.0);
+                    }
                 });
+                sums[k] = sum;
+            }
+            const R inv = R(1.0 / 1099511627776.0);
+            for (int k = 0; k < N; ++k) {
+                V3<R> sum = sums[k];
+                const long long *c = &corr[3 * (size_t)k];
+                if (c[0] | c[1] | c[2]) {
+                    sum.x += R(c[0]) * inv;
+                    sum.y += R(c[1]) * inv;
+                    sum.z += R(c[2]) * inv;
+                }
                 dx[k] = sum * inv_rho;
             }
         }

@@ -17,8 +17,8 @@ import scenes
 pytestmark = pytest.mark.gpu
 
 # per-substep position tolerances (units: scene length; h = 0.1)
-POS_TOL_GATHER = 2e-5    # same summation order: only powf/sinf/cosf/acosf ulps differ
-POS_TOL_SCATTER = 2e-3   # reference order + scatter pair ownership; borderline pairs may flip
+POS_TOL_GATHER = 2e-5    # same summation order; rcp/sqrt/sin/cos ulps and FMA differ
+POS_TOL_SCATTER = 2e-3   # reference summation order; borderline branches may flip
 DT = 1.0 / 120.0
 
 
@@ -36,8 +36,9 @@ def set_same_state(gpu, ora):
 
 def per_substep_parity(sc, prm, substeps, label, tol_g=POS_TOL_GATHER, tol_s=POS_TOL_SCATTER, warm=3):
     """GPU vs oracle after one substep from the same state, `substeps` times.
-    Against the reference-order oracle the 99.9th percentile must stay below
-    tol_s / 2 and the maximum below tol_s: borderline pairs may flip."""
+    Against the reference-order oracle (same pair set, reference summation
+    order) the 99.9th percentile must stay below tol_s / 20 and the maximum
+    below tol_s."""
     arr = bp.SceneArrays(sc)
     og = orc.Oracle(arr, prm, fp64=False, mode=orc.GATHER)
     os_ = orc.Oracle(arr, prm, fp64=False, mode=orc.SCATTER)
@@ -71,7 +72,7 @@ def per_substep_parity(sc, prm, substeps, label, tol_g=POS_TOL_GATHER, tol_s=POS
         worst_g, worst_s = max(worst_g, float(err_g.max())), max(worst_s, float(err_s.max()))
         q_s = max(q_s, float(np.quantile(err_s, 0.999)))
         assert err_g.max() <= tol_g, f"substep {k}: |x_gpu - x_oracle(gather)| = {err_g.max()}"
-        assert np.quantile(err_s, 0.999) <= tol_s / 2 and err_s.max() <= tol_s, \
+        assert np.quantile(err_s, 0.999) <= tol_s / 20 and err_s.max() <= tol_s, \
             f"substep {k}: |x_gpu - x_oracle(scatter)| = {err_s.max()}"
         assert float(np.abs(gv - rv).max()) <= tol_g / DT * 2
         if og.num_rigid:
