@@ -106,6 +106,24 @@ def cpu_baseline(sc, prm, subdt, budget_s=20.0):
                                       f"FP32 oracle in reference order"}
 
 
+def copy_calibration(mib):
+    """A device-to-device copy of `mib` MiB through the HIP runtime: the known
+    traffic (mib read + mib written) that scales rocprof FETCH_SIZE and
+    WRITE_SIZE readings of the solver kernels."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    for _ in range(3):
+        hip.hipMemcpy(b, a, C.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,12 +132,16 @@ def main():
     ap.add_argument("--workload", default="mixed_phase_283k", choices=sorted(scenes.WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-large", action="store_true", help="skip the 4M large-block measurement")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device (rocprof counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU slab sharding is a design, not part of this build)", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     sc, prm, subdt = build_workload(args.workload)
     s, wall = time_plain(sc, prm, subdt, args.steps, args.warmup)
     n = s.n

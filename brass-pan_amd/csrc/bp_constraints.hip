@@ -27,7 +27,7 @@ __global__ __launch_bounds__(kBlock) void k_particle_particle(const float4 *__re
                                                               const int2 *__restrict__ ranges,
                                                               float4 *__restrict__ xs_out, int n, GridDesc G,
                                                               float r, float fs, float fd) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcd_index();
     if (i >= n) return;
     const float4 a = xs[i];
     const float4 xa = x[i];

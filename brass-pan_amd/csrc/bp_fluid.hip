@@ -21,7 +21,7 @@ __global__ __launch_bounds__(kBlock) void k_lambda(const float4 *__restrict__ xs
                                                    const int2 *__restrict__ ranges, float *__restrict__ lambda,
                                                    float *__restrict__ density, uint32_t *__restrict__ ccell, int n,
                                                    GridDesc G, SphConst K, FluidParams F) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcd_index();
     if (i >= n) return;
     const float4 a = xs[i];
     const f3 pi = xyz(a);
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(kBlock) void k_position(const float4 *__restrict__ 
                                                      const int2 *__restrict__ ranges, float4 *__restrict__ sum_out,
                                                      unsigned long long *__restrict__ corr, int n, GridDesc G,
                                                      SphConst K, FluidParams F) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = xcd_index();
     if (k >= n) return;
     const float4 a = xs[k];
     const f3 pk = xyz(a);
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void k_post_a(const float4 *__restrict__ xs
                                                    const int2 *__restrict__ ranges, float4 *__restrict__ omega,
                                                    float4 *__restrict__ normal, uint32_t *__restrict__ ccell, int n,
                                                    GridDesc G, SphConst K, int do_omega, int do_normal) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcd_index();
     if (i >= n) return;
     const float4 a = xs[i];
     const int ph = phase_of(a);
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void k_post_b(const float4 *__restrict__ xs
                                                    const int2 *__restrict__ ranges,
                                                    const uint32_t *__restrict__ ccell, int n, GridDesc G, SphConst K,
                                                    FluidParams F, int do_vort, int do_akinci, float dt) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcd_index();
     if (i >= n) return;
     const float4 a = xs[i];
     const int ph = phase_of(a);
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(kBlock) void k_xsph(const float4 *__restrict__ xs, 
                                                  const int2 *__restrict__ ranges,
                                                  const uint32_t *__restrict__ ccell, float4 *__restrict__ v_out,
                                                  int n, GridDesc G, SphConst K, float c, int use_ccell) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcd_index();
     if (i >= n) return;
     const float4 a = xs[i];
     const int ph = phase_of(a);

@@ -240,6 +240,19 @@ __device__ __forceinline__ void for_each_neighbor(const GridDesc &G, const int2 
     });
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
+// b % 8).  Neighbour passes renumber them so that each XCD walks one
+// contiguous range of sorted slots and its private L2 holds the neighbour
+// cells it needs.  Bijective for any grid size.
+constexpr int kNumXcd = 8;
+__device__ __forceinline__ int xcd_block() {
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    const int q = nb / kNumXcd, r = nb % kNumXcd;
+    const int xcd = b % kNumXcd, local = b / kNumXcd;
+    return (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+__device__ __forceinline__ int xcd_index() { return xcd_block() * (int)blockDim.x + (int)threadIdx.x; }
+
 inline int grid_blocks(long long n, int block = kBlock) { return (int)((n + block - 1) / block); }
 
 }  // namespace bp

@@ -74,7 +74,8 @@ def per_substep_parity(sc, prm, substeps, label, tol_g=POS_TOL_GATHER, tol_s=POS
         assert err_g.max() <= tol_g, f"substep {k}: |x_gpu - x_oracle(gather)| = {err_g.max()}"
         assert np.quantile(err_s, 0.999) <= tol_s / 20 and err_s.max() <= tol_s, \
             f"substep {k}: |x_gpu - x_oracle(scatter)| = {err_s.max()}"
-        assert float(np.abs(gv - rv).max()) <= tol_g / DT * 2
+        # v = (x* - x)/dt, then XSPH (gain c * sum W, about 80 at c = 0.01) amplifies rounding
+        assert float(np.abs(gv - rv).max()) <= tol_g / DT * 2 + 2e-4 * float(np.abs(rv).max())
         if og.num_rigid:
             gq, gcm = gpu.rigid_state()
             rq, rcm = og.rigid_state()
