@@ -64,7 +64,21 @@ def time_plain(sc, prm, subdt, steps, warmup):
     return s, wall
 
 
-def kernel_roofline(s, subdt, substeps=5):
+def profiled_traffic(family, workload):
+    """HBM bytes per launch of `family` on `workload` from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summaries (profiles/), or None."""
+    import glob
+    import re
+
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.txt"))):
+        for line in open(path):
+            m = re.match(rf"traffic_bytes_per_launch {family} {workload}: (\d+)", line)
+            if m:
+                return int(m.group(1))
+    return None
+
+
+def kernel_roofline(s, subdt, substeps=5, workload=None):
     """Per-family HIP-event timing on the solver's stream over `substeps`."""
     s.enable_kernel_timing(True)
     s.timed_substeps(substeps, subdt)
@@ -80,7 +94,8 @@ def kernel_roofline(s, subdt, substeps=5):
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     return {
         "kernel": fam, "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-        "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "avg_launch_ms": round(avg_ms, 4),
+        "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": profiled_traffic(fam, workload),
+        "alg_bytes_per_launch": int(alg_bytes), "avg_launch_ms": round(avg_ms, 4),
         "alg_bytes_per_particle": round(per, 2), "occupied_cells_per_particle": round(cpn, 4),
         "launches": launches,
     }, {k: [round(v[0], 3), v[1]] for k, v in sorted(times.items(), key=lambda kv: -kv[1][0])}
@@ -146,7 +161,7 @@ def main():
     s, wall = time_plain(sc, prm, subdt, args.steps, args.warmup)
     n = s.n
     value = n * args.steps / wall
-    roof, fam_times = kernel_roofline(s, subdt)
+    roof, fam_times = kernel_roofline(s, subdt, workload=args.workload)
     s.close()
     out = {
         "metric": "particle-substeps/s (4 iters)", "value": round(value, 1), "unit": "particle-substeps/s",
@@ -167,7 +182,7 @@ def main():
         lsc, lprm, lsub = build_workload("large_block_4m")
         lsteps, lwarm = 10, 2
         ls, lwall = time_plain(lsc, lprm, lsub, lsteps, lwarm)
-        lroof, ltimes = kernel_roofline(ls, lsub, substeps=2)
+        lroof, ltimes = kernel_roofline(ls, lsub, substeps=2, workload="large_block_4m")
         out["large_block_4m"] = {"value": round(ls.n * lsteps / lwall, 1),
                                  "ms_per_step": round(lwall / lsteps * 1e3, 4), "steps": lsteps,
                                  "warmup": lwarm, "roofline": lroof, "kernel_ms": ltimes}
