@@ -92,3 +92,14 @@ def test_create_without_device_fails_loudly():
     assert st == 5, msg  # BP_ERR_NO_DEVICE: no CPU fallback exists
     with pytest.raises(bp.BrassPanError):
         bp.Solver(small_scene(grid_dims=(64, 64, 64)))
+
+
+EXAMPLE = os.path.join(ROOT, "brass-pan_amd", "bin", "dam_break")
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK), reason="device present")
+def test_cpp_api_example_builds_and_fails_loudly_without_device():
+    import subprocess
+    assert os.path.exists(EXAMPLE), "brass-pan_amd/bin/dam_break not built"
+    r = subprocess.run([EXAMPLE, "1"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "no HIP device" in r.stderr
