@@ -89,6 +89,10 @@ SIGNATURES = {
     "bp_debug_fetch": (_S, [_P, C.c_int, _P]),
     "bp_neighbor_grid": (_S, [C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int),
                               C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "bp_batch_create": (_S, [C.POINTER(bp_scene_desc), C.POINTER(bp_params), C.c_int, _P, C.POINTER(_P)]),
+    "bp_batch_scene_range": (_S, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "bp_solver_set_graph": (_S, [_P, C.c_int]),
+    "bp_solver_autotune": (_S, [_P, C.c_float, C.c_int, C.c_char_p, C.c_int]),
     "bp_solver_timed_substeps": (_S, [_P, C.c_int, C.c_float, C.POINTER(C.c_float)]),
     "bp_solver_enable_kernel_timing": (_S, [_P, C.c_int]),
     "bp_solver_kernel_times": (_S, [_P, C.c_int, C.c_char_p, C.POINTER(C.c_float), C.POINTER(C.c_int),
@@ -216,6 +220,15 @@ class Solver:
         self.h = h
         self.num_rigid = len(self.arrays.rranges)
 
+    def set_graph(self, on: bool = True):
+        _check(self.lib.bp_solver_set_graph(self.h, 1 if on else 0))
+
+    def autotune(self, dt: float, substeps_per_trial: int = 2) -> dict:
+        import json
+        buf = C.create_string_buffer(512)
+        _check(self.lib.bp_solver_autotune(self.h, float(dt), int(substeps_per_trial), buf, 512))
+        return json.loads(buf.value.decode())
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.bp_solver_destroy(self.h)
@@ -341,3 +354,27 @@ def neighbor_grid(xs, cell_size, grid_dims):
     en = np.empty(cells, np.int32)
     _check(lib.bp_neighbor_grid(_fp(xs), n, _fp(cs), _ip(dims), _ip(keys), _ip(perm), _ip(st), _ip(en)))
     return keys, perm, st, en
+
+
+class BatchSolver(Solver):
+    """Many independent scenes advanced together (bp_batch_create): one pass
+    launch serves every scene.  Original ids are global; scene k owns
+    ranges[k] = (first, count)."""
+
+    def __init__(self, scene_list, params_list, stream: int | None = None):
+        self.lib = load_library()
+        self.scene_arrays = [SceneArrays(sc) for sc in scene_list]
+        descs = (bp_scene_desc * len(scene_list))(*[a.desc for a in self.scene_arrays])
+        prms = (bp_params * len(scene_list))(*params_list)
+        self.params = params_list[0]
+        h = C.c_void_p()
+        _check(self.lib.bp_batch_create(descs, prms, len(scene_list), C.c_void_p(stream) if stream else None,
+                                        C.byref(h)))
+        self.h = h
+        self.n = sum(a.n for a in self.scene_arrays)
+        self.num_rigid = sum(len(a.rranges) for a in self.scene_arrays)
+        self.ranges = []
+        for k in range(len(scene_list)):
+            f, c = C.c_int(), C.c_int()
+            _check(self.lib.bp_batch_scene_range(self.h, k, C.byref(f), C.byref(c)))
+            self.ranges.append((f.value, c.value))

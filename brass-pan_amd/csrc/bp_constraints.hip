@@ -12,23 +12,28 @@ namespace bp {
 
 // particlePlaneCollisionConstraint over all planes (particleplane.cuh:21-58,
 // particlesolver.cuh:872-880), in place.
+template <class SP>
 __global__ __launch_bounds__(kBlock) void k_plane_collide(float4 *__restrict__ xs, const float4 *__restrict__ x,
-                                                          int n, Planes pl, float r, float fs, float fd) {
+                                                          int n, SP sp) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const SceneConst &S = sp.at(i);
     const float4 a = xs[i];
-    const f3 out = plane_collide_one(xyz(a), xyz(x[i]), pl, r, fs, fd);
+    const f3 out = plane_collide_one(xyz(a), xyz(x[i]), S.pl, S.radius, S.friction_static, S.friction_dynamic);
     xs[i] = f4(out, a.w);
 }
 
 // particleParticleCollisionConstraint (particleparticle.cuh:5-101) + Δx apply.
-__global__ __launch_bounds__(kBlock) void k_particle_particle(const float4 *__restrict__ xs,
-                                                              const float4 *__restrict__ x,
-                                                              const int2 *__restrict__ ranges,
-                                                              float4 *__restrict__ xs_out, int n, GridDesc G,
-                                                              float r, float fs, float fd) {
+template <class SP>
+__global__ __launch_bounds__(kMaxBlock) void k_particle_particle(const float4 *__restrict__ xs,
+                                                                 const float4 *__restrict__ x,
+                                                                 const int2 *__restrict__ ranges,
+                                                                 float4 *__restrict__ xs_out, int n, SP sp) {
     const int i = xcd_index();
     if (i >= n) return;
+    const SceneConst &S = sp.at(i);
+    const GridDesc &G = S.G;
+    const float r = S.radius, fs = S.friction_static, fd = S.friction_dynamic;
     const float4 a = xs[i];
     const float4 xa = x[i];
     const f3 xi = xyz(xa), xip = xyz(a);
@@ -195,15 +200,18 @@ __global__ __launch_bounds__(kBlock) void k_gather_apply(float4 *__restrict__ xs
 
 // applyWindForce (wind.cuh:3-50) with a wind vector: relative air velocity is
 // the face-average particle velocity minus `wind` (0 reproduces the reference).
+template <class SP>
 __global__ __launch_bounds__(kBlock) void k_wind_compute(const float4 *__restrict__ xs, const float4 *__restrict__ v,
                                                          const int4 *__restrict__ faces,
                                                          const int *__restrict__ map_to_new,
-                                                         float4 *__restrict__ fbuf, int nf, f3 wind, float cd,
-                                                         float cl, float dt) {
+                                                         float4 *__restrict__ fbuf, int nf, SP sp, float dt) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nf) return;
     const int4 ids = faces[f];
     const int id1 = map_to_new[ids.x], id2 = map_to_new[ids.y], id3 = map_to_new[ids.z];
+    const SceneConst &S = sp.at(id1);
+    const f3 wind = S.wind;
+    const float cd = S.drag_coeff, cl = S.lift_coeff;
     fbuf[f] = make_float4(0.f, 0.f, 0.f, 0.f);
     const f3 p1 = xyz(xs[id1]), p2 = xyz(xs[id2]), p3 = xyz(xs[id3]);
     const f3 a = p2 - p1, b = p3 - p1;
@@ -344,14 +352,15 @@ __global__ __launch_bounds__(kBlock) void k_finish(const float4 *__restrict__ xs
 }
 
 // ------------------------------------------------------------ launchers
-void launch_plane_collide(float4 *xs, const float4 *x, int n, const Planes &pl, float r, float fs, float fd,
-                          hipStream_t st) {
-    hipLaunchKernelGGL(k_plane_collide, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, x, n, pl, r, fs, fd);
+template <class SP>
+void launch_plane_collide(float4 *xs, const float4 *x, int n, const SP &sp, hipStream_t st) {
+    hipLaunchKernelGGL(k_plane_collide<SP>, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, x, n, sp);
 }
+template <class SP>
 void launch_particle_particle(const float4 *xs, const float4 *x, const int2 *ranges, float4 *xs_out, int n,
-                              const GridDesc &G, float r, float fs, float fd, hipStream_t st) {
-    hipLaunchKernelGGL(k_particle_particle, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, x, ranges, xs_out, n, G,
-                       r, fs, fd);
+                              const SP &sp, int block, hipStream_t st) {
+    hipLaunchKernelGGL(k_particle_particle<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, x, ranges,
+                       xs_out, n, sp);
 }
 void launch_distance(float4 *xs, const float4 *x, const int2 *pairs, const float2 *params, const int *map_to_new,
                      float4 *kbuf, int nc, const int *verts, const int *offs, const int *ents, int nv,
@@ -368,11 +377,12 @@ void launch_bending(float4 *xs, const float4 *x, const int4 *bends, const int *m
     hipLaunchKernelGGL(k_gather_apply, dim3(grid_blocks(nv)), dim3(kBlock), 0, st, xs, verts, offs, ents, bbuf,
                        map_to_new, (const float *)nullptr, 0, nv);
 }
+template <class SP>
 void launch_wind(float4 *xs, const float4 *v, const float *m, const int4 *faces, const int *map_to_new,
-                 float4 *fbuf, int nf, f3 wind, float cd, float cl, float dt, const int *verts, const int *offs,
-                 const int *ents, int nv, hipStream_t st) {
-    hipLaunchKernelGGL(k_wind_compute, dim3(grid_blocks(nf)), dim3(kBlock), 0, st, xs, v, faces, map_to_new, fbuf,
-                       nf, wind, cd, cl, dt);
+                 float4 *fbuf, int nf, const SP &sp, float dt, const int *verts, const int *offs, const int *ents,
+                 int nv, hipStream_t st) {
+    hipLaunchKernelGGL(k_wind_compute<SP>, dim3(grid_blocks(nf)), dim3(kBlock), 0, st, xs, v, faces, map_to_new,
+                       fbuf, nf, sp, dt);
     hipLaunchKernelGGL(k_gather_apply, dim3(grid_blocks(nv)), dim3(kBlock), 0, st, xs, verts, offs, ents, fbuf,
                        map_to_new, m, 1, nv);
 }
@@ -390,5 +400,14 @@ void launch_pick(float4 *pos, float4 *v, const int *map_to_new, int orig, f3 p, 
 void launch_finish(const float4 *xs, float4 *x, float4 *v, int n, float inv_dt, float eps2, hipStream_t st) {
     hipLaunchKernelGGL(k_finish, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, x, v, n, inv_dt, eps2);
 }
+
+#define BP_INSTANTIATE_CONSTRAINTS(SP)                                                                          \
+    template void launch_plane_collide<SP>(float4 *, const float4 *, int, const SP &, hipStream_t);              \
+    template void launch_particle_particle<SP>(const float4 *, const float4 *, const int2 *, float4 *, int,      \
+                                               const SP &, int, hipStream_t);                                    \
+    template void launch_wind<SP>(float4 *, const float4 *, const float *, const int4 *, const int *, float4 *,  \
+                                  int, const SP &, float, const int *, const int *, const int *, int, hipStream_t);
+BP_INSTANTIATE_CONSTRAINTS(OneScene)
+BP_INSTANTIATE_CONSTRAINTS(SceneTable)
 
 }  // namespace bp

@@ -17,12 +17,17 @@ __device__ __forceinline__ void unpack_cell(uint32_t c, int g[3]) {
 
 // fluidLambda (fluid.cuh:78-159).  Also records the particle's current cell
 // (packed) for the position pass, which searches from the same x*.
-__global__ __launch_bounds__(kBlock) void k_lambda(const float4 *__restrict__ xs, const float *__restrict__ m,
+template <class SP>
+__global__ __launch_bounds__(kMaxBlock) void k_lambda(const float4 *__restrict__ xs, const float *__restrict__ m,
                                                    const int2 *__restrict__ ranges, float *__restrict__ lambda,
                                                    float *__restrict__ density, uint32_t *__restrict__ ccell, int n,
-                                                   GridDesc G, SphConst K, FluidParams F) {
+                                                   SP sp) {
     const int i = xcd_index();
     if (i >= n) return;
+    const SceneConst &S = sp.at(i);
+    const GridDesc &G = S.G;
+    const SphConst &K = S.K;
+    const FluidParams &F = S.F;
     const float4 a = xs[i];
     const f3 pi = xyz(a);
     int g[3];
@@ -90,15 +95,19 @@ __device__ __forceinline__ unsigned long long to_fixed(float v) {
 //   k's cell.
 // A pair k owns whose partner j's search does not reach k's cell is delivered
 // to j through corr[j] (fixed point).  k_position_apply adds both parts.
-__global__ __launch_bounds__(kBlock) void k_position(const float4 *__restrict__ xs, const float *__restrict__ m,
+template <class SP>
+__global__ __launch_bounds__(kMaxBlock) void k_position(const float4 *__restrict__ xs, const float *__restrict__ m,
                                                      const float *__restrict__ lambda,
                                                      const uint32_t *__restrict__ ccell,
                                                      const int *__restrict__ build_keys,
                                                      const int2 *__restrict__ ranges, float4 *__restrict__ sum_out,
-                                                     unsigned long long *__restrict__ corr, int n, GridDesc G,
-                                                     SphConst K, FluidParams F) {
+                                                     unsigned long long *__restrict__ corr, int n, SP sp) {
     const int k = xcd_index();
     if (k >= n) return;
+    const SceneConst &S = sp.at(k);
+    const GridDesc &G = S.G;
+    const SphConst &K = S.K;
+    const FluidParams &F = S.F;
     const float4 a = xs[k];
     const f3 pk = xyz(a);
     const bool kf = phase_of(a) <= 0;
@@ -137,14 +146,16 @@ __global__ __launch_bounds__(kBlock) void k_position(const float4 *__restrict__ 
 
 // Δx = (own sum + delivered pairs) / rho0, x* += Δx (particlesolver.cuh:919-932)
 // and the fused epilogue; clears corr for the next pass.
+template <class SP>
 __global__ __launch_bounds__(kBlock) void k_position_apply(const float4 *__restrict__ xs,
                                                            const float4 *__restrict__ sum_in,
                                                            unsigned long long *__restrict__ corr,
                                                            float4 *__restrict__ xs_out, float4 *__restrict__ x_io,
-                                                           float4 *__restrict__ v_out, int n, FluidParams F,
-                                                           Planes pl, Epilogue E) {
+                                                           float4 *__restrict__ v_out, int n, SP sp, Epilogue E) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
+    const SceneConst &S = sp.at(k);
+    const FluidParams &F = S.F;
     const float4 a = xs[k];
     f3 sum = xyz(sum_in[k]);
     const unsigned long long cx = corr[3 * k], cy = corr[3 * k + 1], cz = corr[3 * k + 2];
@@ -161,7 +172,7 @@ __global__ __launch_bounds__(kBlock) void k_position_apply(const float4 *__restr
     if (E.planes_next || E.finish) {
         float4 x4 = x_io[k];
         const f3 xp = xyz(x4);
-        if (E.planes_next) out = plane_collide_one(out, xp, pl, E.radius, E.friction_static, E.friction_dynamic);
+        if (E.planes_next) out = plane_collide_one(out, xp, S.pl, S.radius, S.friction_static, S.friction_dynamic);
         if (E.finish) {  // updateVelocity + updatePositions (particlesolver.cuh:71-81,121-137)
             v_out[k] = f4((out - xp) * E.inv_dt, 0.0f);
             const float d2 = length2(out - xp);
@@ -173,13 +184,17 @@ __global__ __launch_bounds__(kBlock) void k_position_apply(const float4 *__restr
 
 // fluidOmega (fluid.cuh:231-279) and fluidNormal (fluid.cuh:387-434) in one
 // sweep (independent sums over the same neighbours).  omega.w = |omega|.
-__global__ __launch_bounds__(kBlock) void k_post_a(const float4 *__restrict__ xs, const float4 *__restrict__ v,
+template <class SP>
+__global__ __launch_bounds__(kMaxBlock) void k_post_a(const float4 *__restrict__ xs, const float4 *__restrict__ v,
                                                    const float *__restrict__ density,
                                                    const int2 *__restrict__ ranges, float4 *__restrict__ omega,
                                                    float4 *__restrict__ normal, uint32_t *__restrict__ ccell, int n,
-                                                   GridDesc G, SphConst K, int do_omega, int do_normal) {
+                                                   SP sp, int do_omega, int do_normal) {
     const int i = xcd_index();
     if (i >= n) return;
+    const SceneConst &S = sp.at(i);
+    const GridDesc &G = S.G;
+    const SphConst &K = S.K;
     const float4 a = xs[i];
     const int ph = phase_of(a);
     if (ph >= 0) return;
@@ -209,15 +224,20 @@ __global__ __launch_bounds__(kBlock) void k_post_a(const float4 *__restrict__ xs
 // fluidVorticity (fluid.cuh:281-336, in place) followed by fluidAkinciTension
 // (fluid.cuh:436-489) for the same particle: both only change the particle's
 // own velocity, so the fused pass writes v in place.
-__global__ __launch_bounds__(kBlock) void k_post_b(const float4 *__restrict__ xs, float4 *__restrict__ v,
+template <class SP>
+__global__ __launch_bounds__(kMaxBlock) void k_post_b(const float4 *__restrict__ xs, float4 *__restrict__ v,
                                                    const float4 *__restrict__ omega,
                                                    const float4 *__restrict__ normal,
                                                    const float *__restrict__ density,
                                                    const int2 *__restrict__ ranges,
-                                                   const uint32_t *__restrict__ ccell, int n, GridDesc G, SphConst K,
-                                                   FluidParams F, int do_vort, int do_akinci, float dt) {
+                                                   const uint32_t *__restrict__ ccell, int n, SP sp, int do_vort,
+                                                   int do_akinci, float dt) {
     const int i = xcd_index();
     if (i >= n) return;
+    const SceneConst &S = sp.at(i);
+    const GridDesc &G = S.G;
+    const SphConst &K = S.K;
+    const FluidParams &F = S.F;
     const float4 a = xs[i];
     const int ph = phase_of(a);
     if (ph >= 0) return;
@@ -257,12 +277,17 @@ __global__ __launch_bounds__(kBlock) void k_post_b(const float4 *__restrict__ xs
 }
 
 // fluidXSph (fluid.cuh:338-385).  Non-fluid particles keep their velocity.
-__global__ __launch_bounds__(kBlock) void k_xsph(const float4 *__restrict__ xs, const float4 *__restrict__ v,
+template <class SP>
+__global__ __launch_bounds__(kMaxBlock) void k_xsph(const float4 *__restrict__ xs, const float4 *__restrict__ v,
                                                  const int2 *__restrict__ ranges,
                                                  const uint32_t *__restrict__ ccell, float4 *__restrict__ v_out,
-                                                 int n, GridDesc G, SphConst K, float c, int use_ccell) {
+                                                 int n, SP sp, int use_ccell) {
     const int i = xcd_index();
     if (i >= n) return;
+    const SceneConst &S = sp.at(i);
+    const GridDesc &G = S.G;
+    const SphConst &K = S.K;
+    const float c = S.F.xsph_c;
     const float4 a = xs[i];
     const int ph = phase_of(a);
     const float4 v4 = v[i];
@@ -286,37 +311,57 @@ __global__ __launch_bounds__(kBlock) void k_xsph(const float4 *__restrict__ xs, 
 }
 
 // ------------------------------------------------------------ launchers
+template <class SP>
 void launch_lambda(const float4 *xs, const float *m, const int2 *ranges, float *lambda, float *density,
-                   uint32_t *ccell, int n, const GridDesc &G, const SphConst &K, const FluidParams &F,
-                   hipStream_t st) {
-    hipLaunchKernelGGL(k_lambda, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, m, ranges, lambda, density, ccell,
-                       n, G, K, F);
+                   uint32_t *ccell, int n, const SP &sp, int block, hipStream_t st) {
+    hipLaunchKernelGGL(k_lambda<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, m, ranges, lambda, density,
+                       ccell, n, sp);
 }
+template <class SP>
 void launch_position(const float4 *xs, const float *m, const float *lambda, const uint32_t *ccell,
                      const int *build_keys, const int2 *ranges, float4 *sum_tmp, unsigned long long *corr,
-                     float4 *xs_out, float4 *x_io, float4 *v_out, int n, const GridDesc &G, const SphConst &K,
-                     const FluidParams &F, const Planes &pl, const Epilogue &E, hipStream_t st) {
-    hipLaunchKernelGGL(k_position, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, m, lambda, ccell, build_keys,
-                       ranges, sum_tmp, corr, n, G, K, F);
-    hipLaunchKernelGGL(k_position_apply, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, sum_tmp, corr, xs_out,
-                       x_io, v_out, n, F, pl, E);
+                     float4 *xs_out, float4 *x_io, float4 *v_out, int n, const SP &sp, const Epilogue &E, int block,
+                     hipStream_t st) {
+    hipLaunchKernelGGL(k_position<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, m, lambda, ccell,
+                       build_keys, ranges, sum_tmp, corr, n, sp);
+    hipLaunchKernelGGL(k_position_apply<SP>, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, sum_tmp, corr, xs_out,
+                       x_io, v_out, n, sp, E);
 }
+template <class SP>
 void launch_post_a(const float4 *xs, const float4 *v, const float *density, const int2 *ranges, float4 *omega,
-                   float4 *normal, uint32_t *ccell, int n, const GridDesc &G, const SphConst &K, int do_omega,
-                   int do_normal, hipStream_t st) {
-    hipLaunchKernelGGL(k_post_a, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, v, density, ranges, omega, normal,
-                       ccell, n, G, K, do_omega, do_normal);
+                   float4 *normal, uint32_t *ccell, int n, const SP &sp, int do_omega, int do_normal, int block,
+                   hipStream_t st) {
+    hipLaunchKernelGGL(k_post_a<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, v, density, ranges, omega,
+                       normal, ccell, n, sp, do_omega, do_normal);
 }
+template <class SP>
 void launch_post_b(const float4 *xs, float4 *v, const float4 *omega, const float4 *normal, const float *density,
-                   const int2 *ranges, const uint32_t *ccell, int n, const GridDesc &G, const SphConst &K,
-                   const FluidParams &F, int do_vort, int do_akinci, float dt, hipStream_t st) {
-    hipLaunchKernelGGL(k_post_b, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, v, omega, normal, density, ranges,
-                       ccell, n, G, K, F, do_vort, do_akinci, dt);
+                   const int2 *ranges, const uint32_t *ccell, int n, const SP &sp, int do_vort, int do_akinci,
+                   float dt, int block, hipStream_t st) {
+    hipLaunchKernelGGL(k_post_b<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, v, omega, normal, density,
+                       ranges, ccell, n, sp, do_vort, do_akinci, dt);
 }
+template <class SP>
 void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const uint32_t *ccell, float4 *v_out,
-                 int n, const GridDesc &G, const SphConst &K, float c, int use_ccell, hipStream_t st) {
-    hipLaunchKernelGGL(k_xsph, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, v, ranges, ccell, v_out, n, G, K, c,
-                       use_ccell);
+                 int n, const SP &sp, int use_ccell, int block, hipStream_t st) {
+    hipLaunchKernelGGL(k_xsph<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, v, ranges, ccell, v_out, n,
+                       sp, use_ccell);
 }
+
+#define BP_INSTANTIATE_FLUID(SP)                                                                                \
+    template void launch_lambda<SP>(const float4 *, const float *, const int2 *, float *, float *, uint32_t *,  \
+                                    int, const SP &, int, hipStream_t);                                         \
+    template void launch_position<SP>(const float4 *, const float *, const float *, const uint32_t *,           \
+                                      const int *, const int2 *, float4 *, unsigned long long *, float4 *,      \
+                                      float4 *, float4 *, int, const SP &, const Epilogue &, int, hipStream_t); \
+    template void launch_post_a<SP>(const float4 *, const float4 *, const float *, const int2 *, float4 *,      \
+                                    float4 *, uint32_t *, int, const SP &, int, int, int, hipStream_t);         \
+    template void launch_post_b<SP>(const float4 *, float4 *, const float4 *, const float4 *, const float *,    \
+                                    const int2 *, const uint32_t *, int, const SP &, int, int, float, int,      \
+                                    hipStream_t);                                                               \
+    template void launch_xsph<SP>(const float4 *, const float4 *, const int2 *, const uint32_t *, float4 *,     \
+                                  int, const SP &, int, int, hipStream_t);
+BP_INSTANTIATE_FLUID(OneScene)
+BP_INSTANTIATE_FLUID(SceneTable)
 
 }  // namespace bp

@@ -17,9 +17,10 @@ __device__ __forceinline__ f3 predict(float4 x4, float4 v4, float dt, f3 gstep) 
 // Clears the cells set by the previous substep (resetStartEndId, grid.cuh:101-132)
 // and hashes the predicted position (updateGridId, grid.cuh:30-47).  The value
 // of each key is its slot, supplied implicitly to the first sort pass.
+template <class SP>
 __global__ __launch_bounds__(kBlock) void k_predict_hash(const float4 *__restrict__ x, const float4 *__restrict__ v,
-                                                         int *__restrict__ keys, int n, float dt, f3 gstep,
-                                                         GridDesc G, const int *__restrict__ old_sorted_keys,
+                                                         int *__restrict__ keys, int n, float dt, f3 gstep, SP sp,
+                                                         const int *__restrict__ old_sorted_keys,
                                                          int *__restrict__ ranges_flat) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -36,6 +37,7 @@ __global__ __launch_bounds__(kBlock) void k_predict_hash(const float4 *__restric
         }
         if (i == n - 1) ranges_flat[2 * u + 1] = -1;
     }
+    const GridDesc &G = sp.at(i).G;
     const f3 xs = predict(x[i], v[i], dt, gstep);
     int g[3];
     G.pos(xs, g);
@@ -205,18 +207,21 @@ __global__ __launch_bounds__(kBlock) void k_find_ranges(const int *__restrict__ 
 // particlesolver.cuh:822-833) + optionally the first iteration's plane
 // collisions (particleplane.cuh:21-58).  x* is recomputed from the gathered
 // x, v with the same expression k_predict_hash hashed, so it is bit-identical.
+template <class SP>
 __global__ __launch_bounds__(kBlock) void k_reorder(const int *__restrict__ perm, const float4 *__restrict__ x_in,
                                                     const float4 *__restrict__ v_in, const float *__restrict__ m_in,
                                                     const int *__restrict__ ph_in, const int *__restrict__ o_in,
-                                                    float4 *__restrict__ xs_out, float4 *__restrict__ x_out,
-                                                    float4 *__restrict__ v_out, float *__restrict__ m_out,
-                                                    int *__restrict__ ph_out, int *__restrict__ o_out,
+                                                    const int *__restrict__ sid_in, float4 *__restrict__ xs_out,
+                                                    float4 *__restrict__ x_out, float4 *__restrict__ v_out,
+                                                    float *__restrict__ m_out, int *__restrict__ ph_out,
+                                                    int *__restrict__ o_out, int *__restrict__ sid_out,
                                                     int *__restrict__ map_to_new, int n, float dt, f3 gstep,
-                                                    float mass_k, int need_w, Planes pl, int stab_passes,
-                                                    float radius, int collide, float fs, float fd) {
+                                                    float mass_k, int need_w, int stab_passes, int collide, SP sp) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int p = perm[i];
+    const SceneConst &S = sp.at(p);  // the source slot's scene (scenes never mix)
+    const Planes &pl = S.pl;
     const float4 x4 = x_in[p];
     const float4 v4 = v_in[p];
     const float m = m_in[p];
@@ -234,26 +239,28 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const int *__restrict__ perm
             const f3 on = mk3(pl.o[q][0], pl.o[q][1], pl.o[q][2]);
             const f3 nn = mk3(pl.nrm[q][0], pl.nrm[q][1], pl.nrm[q][2]);
             const f3 o2p = on - xp;
-            const float d = dot(o2p, nn) + radius;
+            const float d = dot(o2p, nn) + S.radius;
             if (d <= 0.0f) continue;
             xp += d * nn;
             xs += d * nn;
         }
     }
-    if (collide) xs = plane_collide_one(xs, xp, pl, radius, fs, fd);
+    if (collide) xs = plane_collide_one(xs, xp, pl, S.radius, S.friction_static, S.friction_dynamic);
     xs_out[i] = f4(xs, __int_as_float(ph));
     x_out[i] = f4(xp, w);
     v_out[i] = v4;
     m_out[i] = m;
     ph_out[i] = ph;
     o_out[i] = o;
+    if (sid_out) sid_out[i] = sid_in[p];
     map_to_new[o] = i;
 }
 
 // ------------------------------------------------------------ launchers
-void launch_predict_hash(const float4 *x, const float4 *v, int *keys, int n, float dt, f3 gstep, const GridDesc &G,
+template <class SP>
+void launch_predict_hash(const float4 *x, const float4 *v, int *keys, int n, float dt, f3 gstep, const SP &sp,
                          const int *old_sorted_keys, int2 *ranges, hipStream_t st) {
-    hipLaunchKernelGGL(k_predict_hash, dim3(grid_blocks(n)), dim3(kBlock), 0, st, x, v, keys, n, dt, gstep, G,
+    hipLaunchKernelGGL(k_predict_hash<SP>, dim3(grid_blocks(n)), dim3(kBlock), 0, st, x, v, keys, n, dt, gstep, sp,
                        old_sorted_keys, (int *)ranges);
 }
 
@@ -292,14 +299,24 @@ void launch_find_ranges(const int *sorted_keys, int n, int2 *ranges, hipStream_t
     hipLaunchKernelGGL(k_find_ranges, dim3(grid_blocks(n)), dim3(kBlock), 0, st, sorted_keys, n, (int *)ranges);
 }
 
+template <class SP>
 void launch_reorder(const int *perm, const float4 *x_in, const float4 *v_in, const float *m_in, const int *ph_in,
-                    const int *o_in, float4 *xs_out, float4 *x_out, float4 *v_out, float *m_out, int *ph_out,
-                    int *o_out, int *map_to_new, int n, float dt, f3 gstep, float mass_k, int need_w,
-                    const Planes &pl, int stab_passes, float radius, int collide, float fs, float fd,
-                    hipStream_t st) {
-    hipLaunchKernelGGL(k_reorder, dim3(grid_blocks(n)), dim3(kBlock), 0, st, perm, x_in, v_in, m_in, ph_in, o_in,
-                       xs_out, x_out, v_out, m_out, ph_out, o_out, map_to_new, n, dt, gstep, mass_k, need_w, pl,
-                       stab_passes, radius, collide, fs, fd);
+                    const int *o_in, const int *sid_in, float4 *xs_out, float4 *x_out, float4 *v_out, float *m_out,
+                    int *ph_out, int *o_out, int *sid_out, int *map_to_new, int n, float dt, f3 gstep, float mass_k,
+                    int need_w, int stab_passes, int collide, const SP &sp, hipStream_t st) {
+    hipLaunchKernelGGL(k_reorder<SP>, dim3(grid_blocks(n)), dim3(kBlock), 0, st, perm, x_in, v_in, m_in, ph_in, o_in,
+                       sid_in, xs_out, x_out, v_out, m_out, ph_out, o_out, sid_out, map_to_new, n, dt, gstep, mass_k,
+                       need_w, stab_passes, collide, sp);
 }
+
+#define BP_INSTANTIATE_GRID(SP)                                                                                 \
+    template void launch_predict_hash<SP>(const float4 *, const float4 *, int *, int, float, f3, const SP &,    \
+                                          const int *, int2 *, hipStream_t);                                    \
+    template void launch_reorder<SP>(const int *, const float4 *, const float4 *, const float *, const int *,    \
+                                     const int *, const int *, float4 *, float4 *, float4 *, float *, int *,     \
+                                     int *, int *, int *, int, float, f3, float, int, int, int, const SP &,      \
+                                     hipStream_t);
+BP_INSTANTIATE_GRID(OneScene)
+BP_INSTANTIATE_GRID(SceneTable)
 
 }  // namespace bp

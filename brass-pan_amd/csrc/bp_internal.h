@@ -9,7 +9,8 @@ namespace bp {
 
 constexpr int kMaxPlanes = 16;
 constexpr int kMaxRigidParticles = 64;  // NUM_MAX_PARTICLE_PER_RIGID_BODY (cubrasstower.cuh:22)
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;      // streaming kernels
+constexpr int kMaxBlock = 512;   // neighbour passes: block size chosen at run time (autotuner)
 
 // ------------------------------------------------------------ small vectors
 // Own 3-vector type: component-wise operators with the left-to-right order of
@@ -71,9 +72,10 @@ struct GridDesc {
     uint32_t stride_x;  // dims[1]*dims[2]
     uint32_t shift_y;   // log2(dims[2])
     uint32_t shift_x;   // log2(dims[1]*dims[2])
-    // cell address -> packed wrapped coordinates
+    uint32_t base;      // first cell id of this scene (scenes of a batch own disjoint cell ranges)
+    // cell id -> packed wrapped coordinates
     __device__ __forceinline__ uint32_t pack_addr(int a) const {
-        const uint32_t u = (uint32_t)a;
+        const uint32_t u = (uint32_t)a - base;
         return ((u >> shift_x) << 20) | (((u >> shift_y) & mask[1]) << 10) | (u & mask[2]);
     }
     __device__ __forceinline__ void pos(f3 p, int g[3]) const {
@@ -82,7 +84,7 @@ struct GridDesc {
         g[2] = (int)((p.z + 50000.0f) / cs[2]);
     }
     __device__ __forceinline__ int addr(int x, int y, int z) const {
-        return (int)(((uint32_t)x & mask[0]) * stride_x + ((uint32_t)y & mask[1]) * (uint32_t)dims[2] +
+        return (int)(base + ((uint32_t)x & mask[0]) * stride_x + ((uint32_t)y & mask[1]) * (uint32_t)dims[2] +
                      ((uint32_t)z & mask[2]));
     }
     // wrapped coordinates packed 10:10:10 (dims <= 1024 per axis)
@@ -140,7 +142,29 @@ struct Epilogue {
     int finish;           // last iteration: updateVelocity + updatePositions
     float inv_dt;
     float sleep_eps2;
+};
+
+// Everything a kernel needs to know about the scene a particle belongs to.
+struct SceneConst {
+    GridDesc G;
+    SphConst K;
+    FluidParams F;
     float radius, friction_static, friction_dynamic;
+    f3 wind;
+    float drag_coeff, lift_coeff;
+    Planes pl;
+};
+
+// Parameter sources: one scene by value (plain solver), or a per-particle
+// lookup in a scene table (batched solver: scenes concatenated slot-wise).
+struct OneScene {
+    SceneConst c;
+    __device__ __forceinline__ const SceneConst &at(int) const { return c; }
+};
+struct SceneTable {
+    const SceneConst *__restrict__ table;
+    const int *__restrict__ sid;  // slot -> scene
+    __device__ __forceinline__ const SceneConst &at(int slot) const { return table[sid[slot]]; }
 };
 
 // ------------------------------------------------------------ device helpers
@@ -180,7 +204,7 @@ __device__ __forceinline__ void for_each_run(const GridDesc &G, const int2 *__re
     const bool wrap = z0 > z2;
     for (int dy = -1; dy <= 1; ++dy) {
         for (int dx = -1; dx <= 1; ++dx) {
-            const uint32_t col = ((uint32_t)(g[0] + dx) & G.mask[0]) * G.stride_x +
+            const uint32_t col = G.base + ((uint32_t)(g[0] + dx) & G.mask[0]) * G.stride_x +
                                  ((uint32_t)(g[1] + dy) & G.mask[1]) * (uint32_t)G.dims[2];
             uint32_t a = z0, b = wrap ? G.mask[2] : z2;
 #pragma unroll

@@ -4,49 +4,59 @@
 
 namespace bp {
 
+// Kernels that depend on the scene take a parameter source SP: OneScene
+// (plain solver, constants by value) or SceneTable (batched solver).
+
 // bp_grid.hip
-void launch_predict_hash(const float4 *x, const float4 *v, int *keys, int n, float dt, f3 gstep, const GridDesc &G,
+template <class SP>
+void launch_predict_hash(const float4 *x, const float4 *v, int *keys, int n, float dt, f3 gstep, const SP &sp,
                          const int *old_sorted_keys, int2 *ranges, hipStream_t st);
 size_t radix_counts_len(int n);
 void launch_radix_sort(int *keys_a, int *keys_b, int *vals_a, int *vals_b, int *counts, int *totals, int n, int bits,
                        int **out_keys, int **out_vals, hipStream_t st);
 void launch_find_ranges(const int *sorted_keys, int n, int2 *ranges, hipStream_t st);
+template <class SP>
 void launch_reorder(const int *perm, const float4 *x_in, const float4 *v_in, const float *m_in, const int *ph_in,
-                    const int *o_in, float4 *xs_out, float4 *x_out, float4 *v_out, float *m_out, int *ph_out,
-                    int *o_out, int *map_to_new, int n, float dt, f3 gstep, float mass_k, int need_w,
-                    const Planes &pl, int stab_passes, float radius, int collide, float fs, float fd,
-                    hipStream_t st);
+                    const int *o_in, const int *sid_in, float4 *xs_out, float4 *x_out, float4 *v_out, float *m_out,
+                    int *ph_out, int *o_out, int *sid_out, int *map_to_new, int n, float dt, f3 gstep, float mass_k,
+                    int need_w, int stab_passes, int collide, const SP &sp, hipStream_t st);
 
-// bp_fluid.hip
+// bp_fluid.hip (`block`: threads per workgroup of the neighbour pass, <= kMaxBlock)
+template <class SP>
 void launch_lambda(const float4 *xs, const float *m, const int2 *ranges, float *lambda, float *density,
-                   uint32_t *ccell, int n, const GridDesc &G, const SphConst &K, const FluidParams &F,
-                   hipStream_t st);
+                   uint32_t *ccell, int n, const SP &sp, int block, hipStream_t st);
+template <class SP>
 void launch_position(const float4 *xs, const float *m, const float *lambda, const uint32_t *ccell,
                      const int *build_keys, const int2 *ranges, float4 *sum_tmp, unsigned long long *corr,
-                     float4 *xs_out, float4 *x_io, float4 *v_out, int n, const GridDesc &G, const SphConst &K,
-                     const FluidParams &F, const Planes &pl, const Epilogue &E, hipStream_t st);
+                     float4 *xs_out, float4 *x_io, float4 *v_out, int n, const SP &sp, const Epilogue &E, int block,
+                     hipStream_t st);
+template <class SP>
 void launch_post_a(const float4 *xs, const float4 *v, const float *density, const int2 *ranges, float4 *omega,
-                   float4 *normal, uint32_t *ccell, int n, const GridDesc &G, const SphConst &K, int do_omega,
-                   int do_normal, hipStream_t st);
+                   float4 *normal, uint32_t *ccell, int n, const SP &sp, int do_omega, int do_normal, int block,
+                   hipStream_t st);
+template <class SP>
 void launch_post_b(const float4 *xs, float4 *v, const float4 *omega, const float4 *normal, const float *density,
-                   const int2 *ranges, const uint32_t *ccell, int n, const GridDesc &G, const SphConst &K,
-                   const FluidParams &F, int do_vort, int do_akinci, float dt, hipStream_t st);
+                   const int2 *ranges, const uint32_t *ccell, int n, const SP &sp, int do_vort, int do_akinci,
+                   float dt, int block, hipStream_t st);
+template <class SP>
 void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const uint32_t *ccell, float4 *v_out,
-                 int n, const GridDesc &G, const SphConst &K, float c, int use_ccell, hipStream_t st);
+                 int n, const SP &sp, int use_ccell, int block, hipStream_t st);
 
 // bp_constraints.hip
-void launch_plane_collide(float4 *xs, const float4 *x, int n, const Planes &pl, float r, float fs, float fd,
-                          hipStream_t st);
+template <class SP>
+void launch_plane_collide(float4 *xs, const float4 *x, int n, const SP &sp, hipStream_t st);
+template <class SP>
 void launch_particle_particle(const float4 *xs, const float4 *x, const int2 *ranges, float4 *xs_out, int n,
-                              const GridDesc &G, float r, float fs, float fd, hipStream_t st);
+                              const SP &sp, int block, hipStream_t st);
 void launch_distance(float4 *xs, const float4 *x, const int2 *pairs, const float2 *params, const int *map_to_new,
                      float4 *kbuf, int nc, const int *verts, const int *offs, const int *ents, int nv,
                      hipStream_t st);
 void launch_bending(float4 *xs, const float4 *x, const int4 *bends, const int *map_to_new, float4 *bbuf, int nb,
                     float kb, const int *verts, const int *offs, const int *ents, int nv, hipStream_t st);
+template <class SP>
 void launch_wind(float4 *xs, const float4 *v, const float *m, const int4 *faces, const int *map_to_new,
-                 float4 *fbuf, int nf, f3 wind, float cd, float cl, float dt, const int *verts, const int *offs,
-                 const int *ents, int nv, hipStream_t st);
+                 float4 *fbuf, int nf, const SP &sp, float dt, const int *verts, const int *offs, const int *ents,
+                 int nv, hipStream_t st);
 void launch_shape_matching(float4 *xs, const int *map_to_new, const float4 *rest, const int2 *ranges, float4 *rot,
                            float4 *cms, int nr, int iters, hipStream_t st);
 void launch_immovable(float4 *xs, const float4 *x, const int *ids, const int *map_to_new, int ni, hipStream_t st);

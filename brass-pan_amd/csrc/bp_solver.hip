@@ -73,19 +73,20 @@ int ilog2(unsigned v) {
 struct bp_solver {
     int device = 0;
     hipStream_t st = nullptr;
+    bool own_stream = false;
     bp_params P{};
     int N = 0;
-    GridDesc G{};
+    int nscenes = 1;
     int num_cells = 0, sort_bits = 0;
-    SphConst K{};
-    FluidParams F{};
-    Planes planes{};
-    float radius = 0.f;
-    bool has_solid = false, has_fluid = false;
+    std::vector<SceneConst> hsc;       // per-scene kernel constants
+    SceneConst *dsc = nullptr;         // device copy (batched solver)
+    std::vector<int> poff;             // scene s owns original ids [poff[s], poff[s+1])
+    std::vector<int> scene_of_orig;
+    bool has_solid = false, has_fluid = false, any_planes = false;
 
     DBuf<float4> xs, x, v;
     DBuf<float> m;
-    DBuf<int> ph, orig;
+    DBuf<int> ph, orig, sid;
     int *map_to_new = nullptr;
 
     int *keys[2] = {nullptr, nullptr};
@@ -122,8 +123,17 @@ struct bp_solver {
 
     std::vector<float> mass_by_orig;
     std::vector<int> phase_by_orig;
-    std::vector<int> group_by_orig;
     std::vector<void *> allocs;
+
+    // neighbour-pass workgroup sizes (bp_solver_autotune)
+    int blk_lambda = 256, blk_position = 256, blk_post_a = 256, blk_post_b = 256, blk_xsph = 256, blk_pp = 256;
+
+    // hipGraph of two substeps (the ping-pong buffers return to their start)
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    float graph_dt = 0.f;
+    int graph_sig = -1;  // buffer parities the graph was captured with (buffer_sig)
 
     // per-family kernel timing (bp_solver_enable_kernel_timing)
     bool timing = false;
@@ -167,6 +177,15 @@ struct bp_solver {
         (void)hipEventRecord(b, st);
         recs.push_back(Rec{fam, a, b});
     }
+    // which half of every ping-pong pair is current: a captured graph holds
+    // raw pointers, so it replays only from the parities it was captured at
+    int buffer_sig() const { return x.c | xs.c << 1 | v.c << 2 | (sk_idx + 1) << 3; }
+    void drop_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+    }
 };
 
 #define TIMED(fam, stmt)      \
@@ -202,23 +221,6 @@ bp_status build_csr(bp_solver *s, Csr &c, int nent_per, const int *ids, int coun
     return BP_OK;
 }
 
-void set_derived_params(bp_solver *s) {
-    const bp_params &P = s->P;
-    s->F.rest_density = P.fluid_rest_density;
-    s->F.inv_rest_density = 1.0f / P.fluid_rest_density;
-    s->F.solid_density_scale = P.solid_density_scale;
-    s->F.lambda_mode = P.lambda_mode;
-    s->F.lambda_denominator = P.lambda_denominator;
-    s->F.lambda_relaxation = P.lambda_relaxation;
-    s->F.akinci = P.akinci;
-    s->F.scorr_k = P.scorr_k;
-    s->F.scorr_n = P.scorr_n;
-    s->F.delta_clamp = P.delta_clamp;
-    s->F.vorticity_eps = P.vorticity_eps;
-    s->F.tension_gamma = P.tension_gamma;
-    s->F.xsph_c = P.xsph_c;
-}
-
 void set_kernel_const(SphConst &K, float h, float scorr_dq) {
     // SetKernelRadius (fluid.cuh:11-27), host powf
     K.c1 = 315.f / 64.f / 3.141592f / powf(h, 9.0f);
@@ -234,6 +236,29 @@ void set_kernel_const(SphConst &K, float h, float scorr_dq) {
     K.wdq = K.c1 * t * t * t;
 }
 
+// per-scene constants from the scene description and its parameters
+void set_scene_const(SceneConst &c, const bp_params &P, float h) {
+    set_kernel_const(c.K, h, P.scorr_dq);
+    c.F.rest_density = P.fluid_rest_density;
+    c.F.inv_rest_density = 1.0f / P.fluid_rest_density;
+    c.F.solid_density_scale = P.solid_density_scale;
+    c.F.lambda_mode = P.lambda_mode;
+    c.F.lambda_denominator = P.lambda_denominator;
+    c.F.lambda_relaxation = P.lambda_relaxation;
+    c.F.akinci = P.akinci;
+    c.F.scorr_k = P.scorr_k;
+    c.F.scorr_n = P.scorr_n;
+    c.F.delta_clamp = P.delta_clamp;
+    c.F.vorticity_eps = P.vorticity_eps;
+    c.F.tension_gamma = P.tension_gamma;
+    c.F.xsph_c = P.xsph_c;
+    c.friction_static = P.friction_static;
+    c.friction_dynamic = P.friction_dynamic;
+    c.wind = mk3(P.wind[0], P.wind[1], P.wind[2]);
+    c.drag_coeff = P.drag_coeff;
+    c.lift_coeff = P.lift_coeff;
+}
+
 bp_status check_params(const bp_params *p) {
     if (!p) return fail(BP_ERR_INVALID_ARG, "params is NULL");
     if (p->iterations < 0 || p->stabilization_passes < 0)
@@ -241,11 +266,25 @@ bp_status check_params(const bp_params *p) {
     if (!(p->fluid_rest_density > 0.0f)) return fail(BP_ERR_INVALID_ARG, "fluid_rest_density must be > 0");
     if (p->lambda_mode != 0 && p->lambda_mode != 1) return fail(BP_ERR_INVALID_ARG, "lambda_mode must be 0 or 1");
     if (p->shape_matching_iterations < 0) return fail(BP_ERR_INVALID_ARG, "shape_matching_iterations < 0");
+    if (p->scorr_n < 0 || p->scorr_n > 64) return fail(BP_ERR_INVALID_ARG, "scorr_n must be in [0, 64]");
     return BP_OK;
 }
 
-// one substep of ParticleSolver::update (particlesolver.cuh:787-1063)
-bp_status substep(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick, int picked, f3 ppos) {
+// Parameters that shape the pass sequence must agree across the scenes of a batch.
+bool same_schedule(const bp_params &a, const bp_params &b) {
+    return a.iterations == b.iterations && a.stabilization_passes == b.stabilization_passes &&
+           a.gravity[0] == b.gravity[0] && a.gravity[1] == b.gravity[1] && a.gravity[2] == b.gravity[2] &&
+           a.akinci == b.akinci && (a.vorticity_eps != 0.0f) == (b.vorticity_eps != 0.0f) &&
+           (a.xsph_c != 0.0f) == (b.xsph_c != 0.0f) && a.mass_scaling_k == b.mass_scaling_k &&
+           a.sleep_epsilon == b.sleep_epsilon && a.wind_enabled == b.wind_enabled &&
+           a.bend_stiffness == b.bend_stiffness && a.shape_matching_iterations == b.shape_matching_iterations &&
+           a.nan_guard == b.nan_guard;
+}
+
+// one substep of ParticleSolver::update (particlesolver.cuh:787-1063); `sp()`
+// returns the scene parameter source for the current slot order
+template <class SPF>
+bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick, int picked, f3 ppos, SPF sp) {
     const int N = s->N;
     const bp_params &P = s->P;
     hipStream_t st = s->st;
@@ -254,14 +293,13 @@ bp_status substep(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick, 
     const bool tail_groups = s->nd > 0 || s->nb > 0 || s->nr > 0 || s->ni > 0;
     const bool need_w = s->has_solid || s->nd > 0 || s->nb > 0;
     const int iters = P.iterations;
-    const float fs = P.friction_static, fd = P.friction_dynamic;
 
     if (pick) TIMED(F_PICK, launch_pick(s->x.cur(), s->v.cur(), s->map_to_new, picked, ppos, 0, zero, st));
 
     // predict + hash, clearing the ranges of the previous substep
     int kin_idx = (s->sk_idx == 0) ? 1 : 0;
     int *kin = s->keys[kin_idx], *kother = s->keys[1 - kin_idx];
-    TIMED(F_PREDICT, launch_predict_hash(s->x.cur(), s->v.cur(), kin, N, subdt, gstep, s->G,
+    TIMED(F_PREDICT, launch_predict_hash(s->x.cur(), s->v.cur(), kin, N, subdt, gstep, sp(),
                                          s->sk_idx >= 0 ? s->sorted_keys : nullptr, s->ranges, st));
     int *sk = nullptr, *pm = nullptr;
     TIMED(F_SORT, launch_radix_sort(kin, kother, s->vals[0], s->vals[1], s->counts, s->totals, N, s->sort_bits, &sk,
@@ -271,48 +309,44 @@ bp_status substep(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick, 
     s->sk_idx = (sk == s->keys[0]) ? 0 : 1;
     TIMED(F_RANGES, launch_find_ranges(sk, N, s->ranges, st));
 
-    const bool fuse_first = s->planes.n > 0 && !wind_on && !pick && iters > 0;
-    TIMED(F_REORDER, launch_reorder(pm, s->x.cur(), s->v.cur(), s->m.cur(), s->ph.cur(), s->orig.cur(), s->xs.cur(),
-                                    s->x.alt(), s->v.alt(), s->m.alt(), s->ph.alt(), s->orig.alt(), s->map_to_new,
-                                    N, subdt, gstep, P.mass_scaling_k, need_w ? 1 : 0, s->planes,
-                                    P.stabilization_passes, s->radius, fuse_first ? 1 : 0, fs, fd, st));
+    const bool fuse_first = s->any_planes && !wind_on && !pick && iters > 0;
+    TIMED(F_REORDER, launch_reorder(pm, s->x.cur(), s->v.cur(), s->m.cur(), s->ph.cur(), s->orig.cur(), s->sid.cur(),
+                                    s->xs.cur(), s->x.alt(), s->v.alt(), s->m.alt(), s->ph.alt(), s->orig.alt(),
+                                    s->sid.alt(), s->map_to_new, N, subdt, gstep, P.mass_scaling_k, need_w ? 1 : 0,
+                                    P.stabilization_passes, fuse_first ? 1 : 0, sp(), st));
     s->x.flip();
     s->v.flip();
     s->m.flip();
     s->ph.flip();
     s->orig.flip();
+    if (s->sid.b[0]) s->sid.flip();
 
     if (wind_on)
-        TIMED(F_WIND, launch_wind(s->xs.cur(), s->v.cur(), s->m.cur(), s->faces, s->map_to_new, s->fbuf, s->nf,
-                                  mk3(P.wind[0], P.wind[1], P.wind[2]), P.drag_coeff, P.lift_coeff, subdt,
-                                  s->fcsr.verts, s->fcsr.offs, s->fcsr.ents, s->fcsr.nv, st));
+        TIMED(F_WIND, launch_wind(s->xs.cur(), s->v.cur(), s->m.cur(), s->faces, s->map_to_new, s->fbuf, s->nf, sp(),
+                                  subdt, s->fcsr.verts, s->fcsr.offs, s->fcsr.ents, s->fcsr.nv, st));
 
     bool planes_done = fuse_first;
     bool finished = false;
     for (int it = 0; it < iters; ++it) {
         if (pick) TIMED(F_PICK, launch_pick(s->xs.cur(), s->v.cur(), s->map_to_new, picked, ppos, 0, zero, st));
-        if (s->planes.n > 0 && !planes_done)
-            TIMED(F_PLANES, launch_plane_collide(s->xs.cur(), s->x.cur(), N, s->planes, s->radius, fs, fd, st));
+        if (s->any_planes && !planes_done) TIMED(F_PLANES, launch_plane_collide(s->xs.cur(), s->x.cur(), N, sp(), st));
         planes_done = false;
         if (s->has_solid) {
-            TIMED(F_PP, launch_particle_particle(s->xs.cur(), s->x.cur(), s->ranges, s->xs.alt(), N, s->G, s->radius,
-                                                 fs, fd, st));
+            TIMED(F_PP, launch_particle_particle(s->xs.cur(), s->x.cur(), s->ranges, s->xs.alt(), N, sp(), s->blk_pp,
+                                                 st));
             s->xs.flip();
         }
         if (s->has_fluid) {
-            TIMED(F_LAMBDA, launch_lambda(s->xs.cur(), s->m.cur(), s->ranges, s->lambda, s->density, s->ccell, N,
-                                          s->G, s->K, s->F, st));
+            TIMED(F_LAMBDA, launch_lambda(s->xs.cur(), s->m.cur(), s->ranges, s->lambda, s->density, s->ccell, N, sp(),
+                                          s->blk_lambda, st));
             Epilogue E{};
-            E.planes_next = (s->planes.n > 0 && it + 1 < iters && !tail_groups && !pick) ? 1 : 0;
+            E.planes_next = (s->any_planes && it + 1 < iters && !tail_groups && !pick) ? 1 : 0;
             E.finish = (it + 1 == iters && !tail_groups) ? 1 : 0;
             E.inv_dt = inv_dt;
             E.sleep_eps2 = P.sleep_epsilon * P.sleep_epsilon;
-            E.radius = s->radius;
-            E.friction_static = fs;
-            E.friction_dynamic = fd;
             TIMED(F_POSITION, launch_position(s->xs.cur(), s->m.cur(), s->lambda, s->ccell, sk, s->ranges,
-                                              s->sum_tmp, s->corr, s->xs.alt(), s->x.cur(), s->v.cur(), N, s->G,
-                                              s->K, s->F, s->planes, E, st));
+                                              s->sum_tmp, s->corr, s->xs.alt(), s->x.cur(), s->v.cur(), N, sp(), E,
+                                              s->blk_position, st));
             s->xs.flip();
             planes_done = E.planes_next;
             finished = E.finish;
@@ -338,17 +372,72 @@ bp_status substep(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick, 
         const int do_ak = P.akinci ? 1 : 0;
         if (do_vort || do_ak) {
             TIMED(F_POST_A, launch_post_a(s->xs.cur(), s->v.cur(), s->density, s->ranges, s->omega, s->normal,
-                                          s->ccell, N, s->G, s->K, do_vort, do_ak, st));
+                                          s->ccell, N, sp(), do_vort, do_ak, s->blk_post_a, st));
             TIMED(F_POST_B, launch_post_b(s->xs.cur(), s->v.cur(), s->omega, s->normal, s->density, s->ranges,
-                                          s->ccell, N, s->G, s->K, s->F, do_vort, do_ak, subdt, st));
+                                          s->ccell, N, sp(), do_vort, do_ak, subdt, s->blk_post_b, st));
         }
         if (P.xsph_c != 0.0f) {
-            TIMED(F_XSPH, launch_xsph(s->xs.cur(), s->v.cur(), s->ranges, s->ccell, s->v.alt(), N, s->G, s->K,
-                                      P.xsph_c, (do_vort || do_ak) ? 1 : 0, st));
+            TIMED(F_XSPH, launch_xsph(s->xs.cur(), s->v.cur(), s->ranges, s->ccell, s->v.alt(), N, sp(),
+                                      (do_vort || do_ak) ? 1 : 0, s->blk_xsph, st));
             s->v.flip();
         }
     }
     HIP_TRY(hipGetLastError());
+    return BP_OK;
+}
+
+bp_status substep(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick, int picked, f3 ppos) {
+    if (s->nscenes == 1) {
+        const OneScene one{s->hsc[0]};
+        return substep_t(s, subdt, inv_dt, gstep, pick, picked, ppos, [&]() { return one; });
+    }
+    return substep_t(s, subdt, inv_dt, gstep, pick, picked, ppos,
+                     [&]() { return SceneTable{s->dsc, s->sid.cur()}; });
+}
+
+f3 gravity_step(const bp_params &P, float subdt) {
+    // make_float3(0,-9.8,0) * dt * dt (particlesolver.cuh:56), per component
+    return mk3(P.gravity[0] * subdt * subdt, P.gravity[1] * subdt * subdt, P.gravity[2] * subdt * subdt);
+}
+
+// Runs `count` plain substeps; with a graph enabled, pairs of substeps replay a
+// captured graph (the ping-pong buffers return to their start after two).
+bp_status run_substeps(bp_solver *s, int count, float subdt) {
+    const float inv_dt = 1.0f / subdt;  // particlesolver.cuh:1000
+    const f3 gstep = gravity_step(s->P, subdt);
+    int k = 0;
+    if (s->use_graph && !s->timing) {
+        while (k < count && s->sk_idx < 0) {  // the first substep has no ranges to clear
+            bp_status r = substep(s, subdt, inv_dt, gstep, false, -1, mk3(0.f, 0.f, 0.f));
+            if (r != BP_OK) return r;
+            ++k;
+        }
+        while (count - k >= 2) {
+            if (!s->gexec || s->graph_dt != subdt || s->graph_sig != s->buffer_sig()) {
+                s->drop_graph();
+                const int sig0 = s->buffer_sig();
+                HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+                bp_status r1 = substep(s, subdt, inv_dt, gstep, false, -1, mk3(0.f, 0.f, 0.f));
+                bp_status r2 = (r1 == BP_OK) ? substep(s, subdt, inv_dt, gstep, false, -1, mk3(0.f, 0.f, 0.f)) : r1;
+                hipGraph_t g = nullptr;
+                hipError_t e = hipStreamEndCapture(s->st, &g);
+                if (r2 != BP_OK) return r2;
+                if (e != hipSuccess) return fail(BP_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+                s->graph = g;
+                HIP_TRY(hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0));
+                s->graph_dt = subdt;
+                s->graph_sig = sig0;
+                if (s->buffer_sig() != sig0) return fail(BP_ERR_HIP, "graph capture: buffers not periodic over two substeps");
+            }
+            // capture recorded the launches without running them
+            HIP_TRY(hipGraphLaunch(s->gexec, s->st));
+            k += 2;
+        }
+    }
+    for (; k < count; ++k) {
+        bp_status r = substep(s, subdt, inv_dt, gstep, false, -1, mk3(0.f, 0.f, 0.f));
+        if (r != BP_OK) return r;
+    }
     return BP_OK;
 }
 
@@ -363,6 +452,249 @@ bp_status collect_timing(bp_solver *s) {
     }
     s->recs.clear();
     s->ev_next = 0;
+    return BP_OK;
+}
+
+bp_status validate_scene(const bp_scene_desc *sc) {
+    const int N = sc->num_particles;
+    if (N < 0) return fail(BP_ERR_INVALID_ARG, "num_particles < 0");
+    if (N > 0 && (!sc->positions || !sc->masses || !sc->phases))
+        return fail(BP_ERR_INVALID_ARG, "positions, masses and phases are required");
+    for (int a = 0; a < 3; ++a) {
+        if (!is_pow2(sc->grid_dims[a]) || sc->grid_dims[a] < 4 || sc->grid_dims[a] > 1024)
+            return fail(BP_ERR_INVALID_ARG, "grid_dims must be powers of two in [4, 1024]");
+        if (!(sc->cell_size[a] > 0.0f)) return fail(BP_ERR_INVALID_ARG, "cell_size must be > 0");
+    }
+    if (!(sc->kernel_radius > 0.0f)) return fail(BP_ERR_INVALID_ARG, "kernel_radius must be > 0");
+    if (sc->num_planes < 0 || sc->num_planes > kMaxPlanes)
+        return fail(BP_ERR_INVALID_ARG, "num_planes must be in [0, 16]");
+    if (sc->num_planes > 0 && (!sc->plane_origins || !sc->plane_normals))
+        return fail(BP_ERR_INVALID_ARG, "plane arrays are NULL");
+    auto ids_ok = [&](const int *ids, long long count) {
+        if (count > 0 && !ids) return false;
+        for (long long i = 0; i < count; ++i)
+            if (ids[i] < 0 || ids[i] >= N) return false;
+        return true;
+    };
+    if (sc->num_distance < 0 || !ids_ok(sc->distance_pairs, 2LL * sc->num_distance) ||
+        (sc->num_distance > 0 && !sc->distance_params))
+        return fail(BP_ERR_INVALID_ARG, "invalid distance constraints");
+    if (sc->num_bendings < 0 || !ids_ok(sc->bendings, 4LL * sc->num_bendings))
+        return fail(BP_ERR_INVALID_ARG, "invalid bending constraints");
+    if (sc->num_faces < 0 || !ids_ok(sc->faces, 3LL * sc->num_faces))
+        return fail(BP_ERR_INVALID_ARG, "invalid wind faces");
+    if (sc->num_immovables < 0 || !ids_ok(sc->immovables, sc->num_immovables))
+        return fail(BP_ERR_INVALID_ARG, "invalid immovable ids");
+    if (sc->num_rigid < 0) return fail(BP_ERR_INVALID_ARG, "num_rigid < 0");
+    for (int b = 0; b < sc->num_rigid; ++b) {
+        const int lo = sc->rigid_ranges[2 * b], hi = sc->rigid_ranges[2 * b + 1];
+        if (lo < 0 || hi <= lo || hi > N || hi > sc->rigid_rest_len || !sc->rigid_rest_positions)
+            return fail(BP_ERR_INVALID_ARG, "invalid rigid range or rest positions");
+        if (hi - lo > kMaxRigidParticles)
+            return fail(BP_ERR_CAPACITY, "rigid body has more than 64 particles "
+                                         "(NUM_MAX_PARTICLE_PER_RIGID_BODY)");
+    }
+    return BP_OK;
+}
+
+// Creates a solver over `ns` scenes laid end to end (ns == 1: the plain solver).
+bp_status create_impl(const bp_scene_desc *scs, const bp_params *params, int ns, void *hip_stream,
+                      bp_solver **out) {
+    if (!out) return fail(BP_ERR_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    if (!scs || !params || ns <= 0) return fail(BP_ERR_INVALID_ARG, "scenes / params missing");
+    long long total = 0, cells_total = 0;
+    for (int k = 0; k < ns; ++k) {
+        bp_status st = check_params(&params[k]);
+        if (st != BP_OK) return st;
+        if (!same_schedule(params[0], params[k]))
+            return fail(BP_ERR_INVALID_ARG, "scenes of a batch must share iterations, pass enables, gravity and "
+                                            "solid/cloth constants");
+        st = validate_scene(&scs[k]);
+        if (st != BP_OK) return st;
+        total += scs[k].num_particles;
+        cells_total += (long long)scs[k].grid_dims[0] * scs[k].grid_dims[1] * scs[k].grid_dims[2];
+    }
+    if (total > (1LL << 30) || cells_total > (1LL << 30))
+        return fail(BP_ERR_CAPACITY, "at most 2^30 particles and 2^30 cells per solver");
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(BP_ERR_NO_DEVICE, "no HIP device visible");
+    bp_solver *s = new bp_solver();
+#define TRY(x)                    \
+    do {                          \
+        bp_status r_ = (x);       \
+        if (r_ != BP_OK) {        \
+            bp_solver_destroy(s); \
+            return r_;            \
+        }                         \
+    } while (0)
+#define HTRY(x) TRY(([&]() -> bp_status { HIP_TRY(x); return BP_OK; })())
+    HTRY(hipGetDevice(&s->device));
+    {
+        hipDeviceProp_t prop;
+        HTRY(hipGetDeviceProperties(&prop, s->device));
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+            bp_solver_destroy(s);
+            return fail(BP_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", this build targets gfx950");
+        }
+    }
+    if (hip_stream) {
+        s->st = (hipStream_t)hip_stream;
+    } else {  // own stream (graph capture needs one); default priority
+        HTRY(hipStreamCreate(&s->st));
+        s->own_stream = true;
+    }
+    s->P = params[0];
+    s->N = (int)total;
+    s->nscenes = ns;
+    s->num_cells = (int)cells_total;
+    s->sort_bits = ilog2((unsigned)s->num_cells);
+    const int N = s->N;
+
+    // concatenate: particles, constraints (ids offset), per-scene constants
+    std::vector<float4> hx(N), hv(N);
+    std::vector<int> iota(N), hsid(N);
+    std::vector<float> hm(N);
+    std::vector<int> hph(N);
+    std::vector<int> dp, bd, fc, rr, im;
+    std::vector<float> dpar;
+    std::vector<float4> rest;
+    s->poff.assign(ns + 1, 0);
+    uint32_t cell_base = 0;
+    s->hsc.resize(ns);
+    for (int k = 0; k < ns; ++k) {
+        const bp_scene_desc &sc = scs[k];
+        const int off = s->poff[k];
+        s->poff[k + 1] = off + sc.num_particles;
+        for (int i = 0; i < sc.num_particles; ++i) {
+            const int g = off + i;
+            hx[g] = make_float4(sc.positions[3 * i], sc.positions[3 * i + 1], sc.positions[3 * i + 2], 0.f);
+            hv[g] = sc.velocities ? make_float4(sc.velocities[3 * i], sc.velocities[3 * i + 1],
+                                                sc.velocities[3 * i + 2], 0.f)
+                                  : make_float4(0.f, 0.f, 0.f, 0.f);
+            hm[g] = sc.masses[i];
+            hph[g] = sc.phases[i];
+            iota[g] = g;
+            hsid[g] = k;
+            if (sc.phases[i] > 0) s->has_solid = true; else s->has_fluid = true;
+        }
+        for (int i = 0; i < 2 * sc.num_distance; ++i) dp.push_back(sc.distance_pairs[i] + off);
+        for (int i = 0; i < 2 * sc.num_distance; ++i) dpar.push_back(sc.distance_params[i]);
+        for (int i = 0; i < 4 * sc.num_bendings; ++i) bd.push_back(sc.bendings[i] + off);
+        for (int i = 0; i < 3 * sc.num_faces; ++i) fc.push_back(sc.faces[i] + off);
+        for (int i = 0; i < sc.num_immovables; ++i) im.push_back(sc.immovables[i] + off);
+        for (int b = 0; b < sc.num_rigid; ++b) {
+            rr.push_back(sc.rigid_ranges[2 * b] + off);
+            rr.push_back(sc.rigid_ranges[2 * b + 1] + off);
+        }
+        if (sc.num_rigid > 0) {
+            if (rest.size() < (size_t)(off + sc.rigid_rest_len)) rest.resize(off + sc.rigid_rest_len);
+            for (int i = 0; i < sc.rigid_rest_len; ++i)
+                rest[off + i] = make_float4(sc.rigid_rest_positions[3 * i], sc.rigid_rest_positions[3 * i + 1],
+                                            sc.rigid_rest_positions[3 * i + 2], 0.f);
+        }
+        SceneConst &c = s->hsc[k];
+        c = SceneConst{};
+        for (int a = 0; a < 3; ++a) {
+            c.G.cs[a] = sc.cell_size[a];
+            c.G.dims[a] = sc.grid_dims[a];
+            c.G.mask[a] = (uint32_t)sc.grid_dims[a] - 1u;
+        }
+        c.G.stride_x = (uint32_t)(sc.grid_dims[1] * sc.grid_dims[2]);
+        c.G.shift_y = (uint32_t)ilog2((unsigned)sc.grid_dims[2]);
+        c.G.shift_x = (uint32_t)ilog2((unsigned)(sc.grid_dims[1] * sc.grid_dims[2]));
+        c.G.base = cell_base;
+        cell_base += (uint32_t)(sc.grid_dims[0] * sc.grid_dims[1] * sc.grid_dims[2]);
+        set_scene_const(c, params[k], sc.kernel_radius);
+        c.radius = sc.particle_radius;
+        c.pl.n = sc.num_planes;
+        for (int p = 0; p < sc.num_planes; ++p)
+            for (int a = 0; a < 3; ++a) {
+                c.pl.o[p][a] = sc.plane_origins[3 * p + a];
+                c.pl.nrm[p][a] = sc.plane_normals[3 * p + a];
+            }
+        if (sc.num_planes > 0) s->any_planes = true;
+    }
+    s->mass_by_orig = hm;
+    s->phase_by_orig = hph;
+    s->scene_of_orig = hsid;
+
+    for (int k = 0; k < 2; ++k) {
+        TRY(s->alloc(&s->xs.b[k], N));
+        TRY(s->alloc(&s->x.b[k], N));
+        TRY(s->alloc(&s->v.b[k], N));
+        TRY(s->alloc(&s->m.b[k], N));
+        TRY(s->alloc(&s->ph.b[k], N));
+        TRY(s->alloc(&s->orig.b[k], N));
+        if (ns > 1) TRY(s->alloc(&s->sid.b[k], N));
+        TRY(s->alloc(&s->keys[k], N));
+        TRY(s->alloc(&s->vals[k], N));
+    }
+    TRY(s->alloc(&s->map_to_new, N));
+    TRY(s->alloc(&s->counts, radix_counts_len(N)));
+    TRY(s->alloc(&s->totals, 256));
+    TRY(s->alloc(&s->ranges, s->num_cells));
+    TRY(s->alloc(&s->lambda, N));
+    TRY(s->alloc(&s->density, N));
+    TRY(s->alloc(&s->ccell, N));
+    TRY(s->alloc(&s->sum_tmp, N));
+    TRY(s->alloc(&s->corr, 3 * (size_t)N));
+    TRY(s->alloc(&s->omega, N));
+    TRY(s->alloc(&s->normal, N));
+    if (ns > 1) TRY(s->upload(&s->dsc, s->hsc.data(), s->hsc.size()));
+    if (N > 0) {
+        HTRY(hipMemcpyAsync(s->x.cur(), hx.data(), N * sizeof(float4), hipMemcpyHostToDevice, s->st));
+        HTRY(hipMemcpyAsync(s->v.cur(), hv.data(), N * sizeof(float4), hipMemcpyHostToDevice, s->st));
+        HTRY(hipMemcpyAsync(s->m.cur(), hm.data(), N * sizeof(float), hipMemcpyHostToDevice, s->st));
+        HTRY(hipMemcpyAsync(s->ph.cur(), hph.data(), N * sizeof(int), hipMemcpyHostToDevice, s->st));
+        HTRY(hipMemcpyAsync(s->orig.cur(), iota.data(), N * sizeof(int), hipMemcpyHostToDevice, s->st));
+        HTRY(hipMemcpyAsync(s->map_to_new, iota.data(), N * sizeof(int), hipMemcpyHostToDevice, s->st));
+        if (ns > 1) HTRY(hipMemcpyAsync(s->sid.cur(), hsid.data(), N * sizeof(int), hipMemcpyHostToDevice, s->st));
+        HTRY(hipMemsetAsync(s->lambda, 0, N * sizeof(float), s->st));
+        HTRY(hipMemsetAsync(s->density, 0, N * sizeof(float), s->st));
+        HTRY(hipMemsetAsync(s->omega, 0, N * sizeof(float4), s->st));
+        HTRY(hipMemsetAsync(s->normal, 0, N * sizeof(float4), s->st));
+        HTRY(hipMemsetAsync(s->corr, 0, 3 * (size_t)N * sizeof(unsigned long long), s->st));
+    }
+    HTRY(hipMemsetAsync(s->ranges, 0xFF, (size_t)s->num_cells * sizeof(int2), s->st));
+
+    s->nd = (int)dp.size() / 2;
+    if (s->nd > 0) {
+        TRY(s->upload(&s->dpairs, (const int2 *)dp.data(), s->nd));
+        TRY(s->upload(&s->dparams, (const float2 *)dpar.data(), s->nd));
+        TRY(s->alloc(&s->kbuf, s->nd));
+        TRY(build_csr(s, s->dcsr, 2, dp.data(), s->nd, N, 1));
+    }
+    s->nb = (int)bd.size() / 4;
+    if (s->nb > 0) {
+        TRY(s->upload(&s->bends, (const int4 *)bd.data(), s->nb));
+        TRY(s->alloc(&s->bbuf, 4 * (size_t)s->nb));
+        TRY(build_csr(s, s->bcsr, 4, bd.data(), s->nb, N, 1));
+    }
+    s->nf = (int)fc.size() / 3;
+    if (s->nf > 0) {
+        std::vector<int4> f4v(s->nf);
+        for (int f = 0; f < s->nf; ++f) f4v[f] = make_int4(fc[3 * f], fc[3 * f + 1], fc[3 * f + 2], 0);
+        TRY(s->upload(&s->faces, f4v.data(), s->nf));
+        TRY(s->alloc(&s->fbuf, s->nf));
+        TRY(build_csr(s, s->fcsr, 3, fc.data(), s->nf, N, 3));
+    }
+    s->nr = (int)rr.size() / 2;
+    if (s->nr > 0) {
+        TRY(s->upload(&s->rranges, (const int2 *)rr.data(), s->nr));
+        TRY(s->upload(&s->rrest, rest.data(), rest.size()));
+        std::vector<float4> q(s->nr, make_float4(0.f, 0.f, 0.f, 1.f));
+        TRY(s->upload(&s->rrot, q.data(), q.size()));
+        TRY(s->alloc(&s->rcm, s->nr));
+        HTRY(hipMemsetAsync(s->rcm, 0, s->nr * sizeof(float4), s->st));
+    }
+    s->ni = (int)im.size();
+    if (s->ni > 0) TRY(s->upload(&s->immov, im.data(), s->ni));
+    HTRY(hipStreamSynchronize(s->st));
+#undef HTRY
+#undef TRY
+    *out = s;
     return BP_OK;
 }
 
@@ -409,197 +741,30 @@ bp_status bp_default_params(bp_params *p) {
 }
 
 bp_status bp_solver_create(const bp_scene_desc *sc, const bp_params *params, void *hip_stream, bp_solver **out) {
-    if (!out) return fail(BP_ERR_INVALID_ARG, "out is NULL");
-    *out = nullptr;
     if (!sc) return fail(BP_ERR_INVALID_ARG, "scene is NULL");
-    bp_status st = check_params(params);
-    if (st != BP_OK) return st;
-    const int N = sc->num_particles;
-    if (N < 0) return fail(BP_ERR_INVALID_ARG, "num_particles < 0");
-    if (N > 0 && (!sc->positions || !sc->masses || !sc->phases))
-        return fail(BP_ERR_INVALID_ARG, "positions, masses and phases are required");
-    for (int a = 0; a < 3; ++a) {
-        if (!is_pow2(sc->grid_dims[a]) || sc->grid_dims[a] < 4 || sc->grid_dims[a] > 1024)
-            return fail(BP_ERR_INVALID_ARG, "grid_dims must be powers of two in [4, 1024]");
-        if (!(sc->cell_size[a] > 0.0f)) return fail(BP_ERR_INVALID_ARG, "cell_size must be > 0");
-    }
-    if (!(sc->kernel_radius > 0.0f)) return fail(BP_ERR_INVALID_ARG, "kernel_radius must be > 0");
-    if (sc->num_planes < 0 || sc->num_planes > kMaxPlanes)
-        return fail(BP_ERR_INVALID_ARG, "num_planes must be in [0, 16]");
-    if (sc->num_planes > 0 && (!sc->plane_origins || !sc->plane_normals))
-        return fail(BP_ERR_INVALID_ARG, "plane arrays are NULL");
-    auto ids_ok = [&](const int *ids, long long count) {
-        if (count > 0 && !ids) return false;
-        for (long long i = 0; i < count; ++i)
-            if (ids[i] < 0 || ids[i] >= N) return false;
-        return true;
-    };
-    if (sc->num_distance < 0 || !ids_ok(sc->distance_pairs, 2LL * sc->num_distance) ||
-        (sc->num_distance > 0 && !sc->distance_params))
-        return fail(BP_ERR_INVALID_ARG, "invalid distance constraints");
-    if (sc->num_bendings < 0 || !ids_ok(sc->bendings, 4LL * sc->num_bendings))
-        return fail(BP_ERR_INVALID_ARG, "invalid bending constraints");
-    if (sc->num_faces < 0 || !ids_ok(sc->faces, 3LL * sc->num_faces))
-        return fail(BP_ERR_INVALID_ARG, "invalid wind faces");
-    if (sc->num_immovables < 0 || !ids_ok(sc->immovables, sc->num_immovables))
-        return fail(BP_ERR_INVALID_ARG, "invalid immovable ids");
-    if (sc->num_rigid < 0) return fail(BP_ERR_INVALID_ARG, "num_rigid < 0");
-    for (int b = 0; b < sc->num_rigid; ++b) {
-        const int lo = sc->rigid_ranges[2 * b], hi = sc->rigid_ranges[2 * b + 1];
-        if (lo < 0 || hi <= lo || hi > N || hi > sc->rigid_rest_len || !sc->rigid_rest_positions)
-            return fail(BP_ERR_INVALID_ARG, "invalid rigid range or rest positions");
-        if (hi - lo > kMaxRigidParticles)
-            return fail(BP_ERR_CAPACITY, "rigid body has more than 64 particles "
-                                         "(NUM_MAX_PARTICLE_PER_RIGID_BODY)");
-    }
+    return create_impl(sc, params, 1, hip_stream, out);
+}
 
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(BP_ERR_NO_DEVICE, "no HIP device visible");
-    bp_solver *s = new bp_solver();
-    HIP_TRY(hipGetDevice(&s->device));
-    {
-        hipDeviceProp_t prop;
-        HIP_TRY(hipGetDeviceProperties(&prop, s->device));
-        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
-            delete s;
-            return fail(BP_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", this build targets gfx950");
-        }
-    }
-    s->st = (hipStream_t)hip_stream;
-    s->P = *params;
-    s->N = N;
-    for (int a = 0; a < 3; ++a) {
-        s->G.cs[a] = sc->cell_size[a];
-        s->G.dims[a] = sc->grid_dims[a];
-        s->G.mask[a] = (uint32_t)sc->grid_dims[a] - 1u;
-    }
-    s->G.stride_x = (uint32_t)(sc->grid_dims[1] * sc->grid_dims[2]);
-    s->G.shift_y = (uint32_t)ilog2((unsigned)sc->grid_dims[2]);
-    s->G.shift_x = (uint32_t)ilog2((unsigned)(sc->grid_dims[1] * sc->grid_dims[2]));
-    s->num_cells = sc->grid_dims[0] * sc->grid_dims[1] * sc->grid_dims[2];
-    s->sort_bits = ilog2((unsigned)s->num_cells);
-    s->radius = sc->particle_radius;
-    set_kernel_const(s->K, sc->kernel_radius, params->scorr_dq);
-    set_derived_params(s);
-    s->planes.n = sc->num_planes;
-    for (int p = 0; p < sc->num_planes; ++p)
-        for (int a = 0; a < 3; ++a) {
-            s->planes.o[p][a] = sc->plane_origins[3 * p + a];
-            s->planes.nrm[p][a] = sc->plane_normals[3 * p + a];
-        }
+bp_status bp_batch_create(const bp_scene_desc *scenes, const bp_params *params, int num_scenes, void *hip_stream,
+                          bp_solver **out) {
+    return create_impl(scenes, params, num_scenes, hip_stream, out);
+}
 
-    s->mass_by_orig.assign(sc->masses, sc->masses + N);
-    s->phase_by_orig.assign(sc->phases, sc->phases + N);
-    s->group_by_orig.assign(N, 0);
-    if (sc->group_ids) s->group_by_orig.assign(sc->group_ids, sc->group_ids + N);
-    for (int i = 0; i < N; ++i) {
-        if (sc->phases[i] > 0) s->has_solid = true; else s->has_fluid = true;
-    }
-
-    // particle state, original order = initial slot order (initOrder_int, util.cuh:312-328)
-    std::vector<float4> hx(N), hv(N);
-    std::vector<int> iota(N);
-    for (int i = 0; i < N; ++i) {
-        hx[i] = make_float4(sc->positions[3 * i], sc->positions[3 * i + 1], sc->positions[3 * i + 2], 0.f);
-        if (sc->velocities)
-            hv[i] = make_float4(sc->velocities[3 * i], sc->velocities[3 * i + 1], sc->velocities[3 * i + 2], 0.f);
-        else
-            hv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        iota[i] = i;
-    }
-#define TRY(x)                 \
-    do {                       \
-        bp_status r_ = (x);    \
-        if (r_ != BP_OK) {     \
-            bp_solver_destroy(s); \
-            return r_;         \
-        }                      \
-    } while (0)
-    for (int k = 0; k < 2; ++k) {
-        TRY(s->alloc(&s->xs.b[k], N));
-        TRY(s->alloc(&s->x.b[k], N));
-        TRY(s->alloc(&s->v.b[k], N));
-        TRY(s->alloc(&s->m.b[k], N));
-        TRY(s->alloc(&s->ph.b[k], N));
-        TRY(s->alloc(&s->orig.b[k], N));
-        TRY(s->alloc(&s->keys[k], N));
-        TRY(s->alloc(&s->vals[k], N));
-    }
-    TRY(s->alloc(&s->map_to_new, N));
-    TRY(s->alloc(&s->counts, radix_counts_len(N)));
-    TRY(s->alloc(&s->totals, 256));
-    TRY(s->alloc(&s->ranges, s->num_cells));
-    TRY(s->alloc(&s->lambda, N));
-    TRY(s->alloc(&s->density, N));
-    TRY(s->alloc(&s->ccell, N));
-    TRY(s->alloc(&s->sum_tmp, N));
-    TRY(s->alloc(&s->corr, 3 * (size_t)N));
-    TRY(s->alloc(&s->omega, N));
-    TRY(s->alloc(&s->normal, N));
-    if (N > 0) {
-        HIP_TRY(hipMemcpyAsync(s->x.cur(), hx.data(), N * sizeof(float4), hipMemcpyHostToDevice, s->st));
-        HIP_TRY(hipMemcpyAsync(s->v.cur(), hv.data(), N * sizeof(float4), hipMemcpyHostToDevice, s->st));
-        HIP_TRY(hipMemcpyAsync(s->m.cur(), sc->masses, N * sizeof(float), hipMemcpyHostToDevice, s->st));
-        HIP_TRY(hipMemcpyAsync(s->ph.cur(), sc->phases, N * sizeof(int), hipMemcpyHostToDevice, s->st));
-        HIP_TRY(hipMemcpyAsync(s->orig.cur(), iota.data(), N * sizeof(int), hipMemcpyHostToDevice, s->st));
-        HIP_TRY(hipMemcpyAsync(s->map_to_new, iota.data(), N * sizeof(int), hipMemcpyHostToDevice, s->st));
-        HIP_TRY(hipMemsetAsync(s->lambda, 0, N * sizeof(float), s->st));
-        HIP_TRY(hipMemsetAsync(s->density, 0, N * sizeof(float), s->st));
-        HIP_TRY(hipMemsetAsync(s->omega, 0, N * sizeof(float4), s->st));
-        HIP_TRY(hipMemsetAsync(s->normal, 0, N * sizeof(float4), s->st));
-        HIP_TRY(hipMemsetAsync(s->corr, 0, 3 * (size_t)N * sizeof(unsigned long long), s->st));
-    }
-    HIP_TRY(hipMemsetAsync(s->ranges, 0xFF, (size_t)s->num_cells * sizeof(int2), s->st));
-
-    // constraints (original ids)
-    s->nd = sc->num_distance;
-    if (s->nd > 0) {
-        TRY(s->upload(&s->dpairs, (const int2 *)sc->distance_pairs, s->nd));
-        TRY(s->upload(&s->dparams, (const float2 *)sc->distance_params, s->nd));
-        TRY(s->alloc(&s->kbuf, s->nd));
-        TRY(build_csr(s, s->dcsr, 2, sc->distance_pairs, s->nd, N, 1));
-    }
-    s->nb = sc->num_bendings;
-    if (s->nb > 0) {
-        TRY(s->upload(&s->bends, (const int4 *)sc->bendings, s->nb));
-        TRY(s->alloc(&s->bbuf, 4 * (size_t)s->nb));
-        TRY(build_csr(s, s->bcsr, 4, sc->bendings, s->nb, N, 1));
-    }
-    s->nf = sc->num_faces;
-    if (s->nf > 0) {
-        std::vector<int4> f4v(s->nf);
-        for (int f = 0; f < s->nf; ++f)
-            f4v[f] = make_int4(sc->faces[3 * f], sc->faces[3 * f + 1], sc->faces[3 * f + 2], 0);
-        TRY(s->upload(&s->faces, f4v.data(), s->nf));
-        TRY(s->alloc(&s->fbuf, s->nf));
-        TRY(build_csr(s, s->fcsr, 3, sc->faces, s->nf, N, 3));
-    }
-    s->nr = sc->num_rigid;
-    if (s->nr > 0) {
-        TRY(s->upload(&s->rranges, (const int2 *)sc->rigid_ranges, s->nr));
-        std::vector<float4> rest(sc->rigid_rest_len);
-        for (int i = 0; i < sc->rigid_rest_len; ++i)
-            rest[i] = make_float4(sc->rigid_rest_positions[3 * i], sc->rigid_rest_positions[3 * i + 1],
-                                  sc->rigid_rest_positions[3 * i + 2], 0.f);
-        TRY(s->upload(&s->rrest, rest.data(), rest.size()));
-        std::vector<float4> q(s->nr, make_float4(0.f, 0.f, 0.f, 1.f));
-        TRY(s->upload(&s->rrot, q.data(), q.size()));
-        TRY(s->alloc(&s->rcm, s->nr));
-        HIP_TRY(hipMemsetAsync(s->rcm, 0, s->nr * sizeof(float4), s->st));
-    }
-    s->ni = sc->num_immovables;
-    if (s->ni > 0) TRY(s->upload(&s->immov, sc->immovables, s->ni));
-#undef TRY
-    HIP_TRY(hipStreamSynchronize(s->st));
-    *out = s;
+bp_status bp_batch_scene_range(bp_solver *s, int scene, int *first_original_id, int *count) {
+    if (!s || !first_original_id || !count) return fail(BP_ERR_INVALID_ARG, "NULL argument");
+    if (scene < 0 || scene >= s->nscenes) return fail(BP_ERR_INVALID_ARG, "scene out of range");
+    *first_original_id = s->poff[scene];
+    *count = s->poff[scene + 1] - s->poff[scene];
     return BP_OK;
 }
 
 bp_status bp_solver_destroy(bp_solver *s) {
     if (!s) return BP_OK;
     if (s->st) (void)hipStreamSynchronize(s->st); else (void)hipDeviceSynchronize();
+    s->drop_graph();
     for (void *p : s->allocs) (void)hipFree(p);
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
+    if (s->own_stream) (void)hipStreamDestroy(s->st);
     delete s;
     return BP_OK;
 }
@@ -608,9 +773,10 @@ bp_status bp_solver_set_params(bp_solver *s, const bp_params *p) {
     if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
     bp_status st = check_params(p);
     if (st != BP_OK) return st;
+    if (s->nscenes != 1) return fail(BP_ERR_UNSUPPORTED, "per-scene parameters of a batch are fixed at creation");
     s->P = *p;
-    set_kernel_const(s->K, s->K.h, p->scorr_dq);
-    set_derived_params(s);
+    set_scene_const(s->hsc[0], *p, s->hsc[0].K.h);
+    s->drop_graph();
     return BP_OK;
 }
 
@@ -620,18 +786,20 @@ bp_status bp_solver_step(bp_solver *s, int num_substeps, float dt, int picked, c
     if (num_substeps <= 0) return fail(BP_ERR_INVALID_ARG, "num_substeps must be > 0");
     if (s->N <= 0) return BP_OK;  // particlesolver.cuh:778
     const float subdt = dt / (float)num_substeps;  // :780
-    const float inv_dt = 1.0f / subdt;             // :1000
-    // make_float3(0,-9.8,0) * dt * dt (particlesolver.cuh:56), per component
-    const f3 gstep = mk3(s->P.gravity[0] * subdt * subdt, s->P.gravity[1] * subdt * subdt,
-                         s->P.gravity[2] * subdt * subdt);
     const bool pick = picked >= 0 && picked < s->N;  // :790
     if (pick && !ppos) return fail(BP_ERR_INVALID_ARG, "picked position is NULL");
-    const f3 pp = pick ? mk3(ppos[0], ppos[1], ppos[2]) : mk3(0.f, 0.f, 0.f);
-    for (int k = 0; k < num_substeps; ++k) {
-        bp_status st = substep(s, subdt, inv_dt, gstep, pick, picked, pp);
+    if (!pick) {
+        bp_status st = run_substeps(s, num_substeps, subdt);
         if (st != BP_OK) return st;
-    }
-    if (pick) {  // :1066-1071: keep the solved position, set the picked velocity
+    } else {
+        const float inv_dt = 1.0f / subdt;
+        const f3 gstep = gravity_step(s->P, subdt);
+        const f3 pp = mk3(ppos[0], ppos[1], ppos[2]);
+        for (int k = 0; k < num_substeps; ++k) {
+            bp_status st = substep(s, subdt, inv_dt, gstep, true, picked, pp);
+            if (st != BP_OK) return st;
+        }
+        // :1066-1071: keep the solved position, set the picked velocity
         const f3 pv = pvel ? mk3(pvel[0], pvel[1], pvel[2]) : mk3(0.f, 0.f, 0.f);
         TIMED(F_PICK, launch_pick(s->x.cur(), s->v.cur(), s->map_to_new, picked, pp, 1, pv, s->st));
     }
@@ -644,6 +812,66 @@ bp_status bp_solver_step(bp_solver *s, int num_substeps, float dt, int picked, c
             if (!std::isfinite(hx[i].x) || !std::isfinite(hx[i].y) || !std::isfinite(hx[i].z))
                 return fail(BP_ERR_NOT_FINITE, "non-finite position at slot " + std::to_string(i));
     }
+    return BP_OK;
+}
+
+bp_status bp_solver_set_graph(bp_solver *s, int enable) {
+    if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
+    s->use_graph = enable != 0;
+    if (!s->use_graph) s->drop_graph();
+    return BP_OK;
+}
+
+bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, char *report, int report_len) {
+    if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
+    if (substeps_per_trial <= 0) return fail(BP_ERR_INVALID_ARG, "substeps_per_trial must be > 0");
+    const int cand[4] = {64, 128, 256, 512};
+    const int fams[6] = {F_LAMBDA, F_POSITION, F_POST_A, F_POST_B, F_XSPH, F_PP};
+    float best_ms[6];
+    int best_blk[6];
+    for (int f = 0; f < 6; ++f) { best_ms[f] = 1e30f; best_blk[f] = 256; }
+    // the trials advance a copy of the state; the caller's state is restored after
+    std::vector<float> keep_x(3 * (size_t)s->N), keep_v(3 * (size_t)s->N), keep_q(4 * (size_t)s->nr + 4),
+        keep_cm(3 * (size_t)s->nr + 3);
+    std::vector<int> keep_map(s->N);
+    bp_status st = bp_get_state(s, keep_x.data(), keep_v.data(), keep_map.data());
+    if (st == BP_OK && s->nr > 0) st = bp_get_rigid_state(s, keep_q.data(), keep_cm.data());
+    if (st != BP_OK) return st;
+    const bool g = s->use_graph;
+    s->use_graph = false;
+    st = bp_solver_enable_kernel_timing(s, 1);
+    if (st != BP_OK) return st;
+    for (int c = 0; c < 4; ++c) {
+        s->blk_lambda = s->blk_position = s->blk_post_a = s->blk_post_b = s->blk_xsph = s->blk_pp = cand[c];
+        for (int f = 0; f < F_COUNT; ++f) { s->fam_ms[f] = 0.f; s->fam_n[f] = 0; }
+        st = run_substeps(s, substeps_per_trial, dt);
+        if (st == BP_OK) st = collect_timing(s);
+        if (st != BP_OK) break;
+        for (int f = 0; f < 6; ++f) {
+            const int fam = fams[f];
+            if (s->fam_n[fam] == 0) continue;
+            const float avg = s->fam_ms[fam] / (float)s->fam_n[fam];
+            if (avg < best_ms[f]) { best_ms[f] = avg; best_blk[f] = cand[c]; }
+        }
+    }
+    s->blk_lambda = best_blk[0];
+    s->blk_position = best_blk[1];
+    s->blk_post_a = best_blk[2];
+    s->blk_post_b = best_blk[3];
+    s->blk_xsph = best_blk[4];
+    s->blk_pp = best_blk[5];
+    (void)bp_solver_enable_kernel_timing(s, 0);
+    s->use_graph = g;
+    s->drop_graph();
+    if (st != BP_OK) return st;
+    st = bp_set_state(s, keep_x.data(), keep_v.data(), keep_map.data());
+    if (st == BP_OK && s->nr > 0) st = bp_set_rigid_state(s, keep_q.data());
+    if (st != BP_OK) return st;
+    if (report && report_len > 0)
+        std::snprintf(report, (size_t)report_len,
+                      "{\"fluid_lambda\": %d, \"fluid_position\": %d, \"omega_normal\": %d, "
+                      "\"vorticity_tension\": %d, \"xsph\": %d, \"particle_particle\": %d}",
+                      best_blk[0], best_blk[1], best_blk[2], best_blk[3], best_blk[4], best_blk[5]);
     return BP_OK;
 }
 
@@ -781,6 +1009,12 @@ bp_status bp_set_state(bp_solver *s, const float *pos, const float *vel, const i
     HIP_TRY(hipMemcpyAsync(s->map_to_new, inv.data(), N * sizeof(int), hipMemcpyHostToDevice, s->st));
     HIP_TRY(hipMemcpyAsync(s->m.cur(), hm.data(), N * sizeof(float), hipMemcpyHostToDevice, s->st));
     HIP_TRY(hipMemcpyAsync(s->ph.cur(), hp.data(), N * sizeof(int), hipMemcpyHostToDevice, s->st));
+    std::vector<int> hs;
+    if (s->nscenes > 1) {
+        hs.resize(N);
+        for (int i = 0; i < N; ++i) hs[i] = s->scene_of_orig[o[i]];
+        HIP_TRY(hipMemcpyAsync(s->sid.cur(), hs.data(), N * sizeof(int), hipMemcpyHostToDevice, s->st));
+    }
     HIP_TRY(hipStreamSynchronize(s->st));
     return BP_OK;
 }
@@ -842,7 +1076,8 @@ bp_status bp_debug_fetch(bp_solver *s, int which, void *out) {
 bp_status bp_neighbor_grid(const float *xs, int n, const float cell_size[3], const int grid_dims[3], int *keys,
                            int *perm, int *cell_start, int *cell_end) {
     if (n < 0 || (n > 0 && !xs) || !cell_size || !grid_dims) return fail(BP_ERR_INVALID_ARG, "invalid arguments");
-    GridDesc G{};
+    OneScene one{};
+    GridDesc &G = one.c.G;
     for (int a = 0; a < 3; ++a) {
         if (!is_pow2(grid_dims[a]) || grid_dims[a] < 4 || grid_dims[a] > 1024)
             return fail(BP_ERR_INVALID_ARG, "grid_dims must be powers of two in [4, 1024]");
@@ -885,7 +1120,7 @@ bp_status bp_neighbor_grid(const float *xs, int n, const float cell_size[3], con
         chk(hipMemsetAsync(ranges, 0xFF, (size_t)cells * sizeof(int2), st));
         if (n > 0) {
             // x* = x + 0*0 + 0 is x exactly: the hash sees the given positions
-            launch_predict_hash(dx, dv, k[0], n, 0.0f, mk3(0.f, 0.f, 0.f), G, nullptr, ranges, st);
+            launch_predict_hash(dx, dv, k[0], n, 0.0f, mk3(0.f, 0.f, 0.f), one, nullptr, ranges, st);
             launch_radix_sort(k[0], k[1], vv[0], vv[1], counts, totals, n, ilog2((unsigned)cells), &sk, &pm, st);
             launch_find_ranges(sk, n, ranges, st);
             if (keys) chk(hipMemcpyAsync(keys, sk, n * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -916,11 +1151,9 @@ bp_status bp_solver_timed_substeps(bp_solver *s, int substeps, float dt, float *
     hipEvent_t a, b;
     HIP_TRY(hipEventCreate(&a));
     HIP_TRY(hipEventCreate(&b));
-    const f3 gstep = mk3(s->P.gravity[0] * dt * dt, s->P.gravity[1] * dt * dt, s->P.gravity[2] * dt * dt);
-    const float inv_dt = 1.0f / dt;
     HIP_TRY(hipEventRecord(a, s->st));
-    for (int k = 0; k < substeps && s->N > 0; ++k) {
-        bp_status st = substep(s, dt, inv_dt, gstep, false, -1, mk3(0.f, 0.f, 0.f));
+    if (s->N > 0) {
+        bp_status st = run_substeps(s, substeps, dt);
         if (st != BP_OK) return st;
     }
     HIP_TRY(hipEventRecord(b, s->st));

@@ -155,6 +155,7 @@ class Scene:
         """Six inward-facing planes of the box [lo, hi]."""
         lo = np.asarray(lo, np.float32)
         hi = np.asarray(hi, np.float32)
+        self.box = (lo, hi)
         for a in range(3):
             n = np.zeros(3, np.float32)
             n[a] = 1.0
@@ -382,6 +383,65 @@ def mixed_phase_params(**kw):
              fluid_rest_density=lattice_rest_density(SPACING, H, 1.0), akinci=0)
     p.update(kw)
     return bp.default_params(**p)
+
+
+def next_pow2(v):
+    p = 4
+    while p < v:
+        p *= 2
+    return p
+
+
+def batch_member(rng: Rng, n_target: int):
+    """One scene of the batched workload: a fluid block, 1-8 rigid 4x4x4 boxes
+    and one square cloth (32..96 per side, at most half the particles) under a
+    random horizontal wind of 0..8 m/s, with its own kernel radius h and a box
+    and grid sized to it."""
+    u = rng.uniform(12)
+    h = (0.08, 0.1, 0.12)[min(2, int(u[0] * 3))]
+    sp, rad = 0.5 * h, 0.25 * h
+    cloth_n = 32 + int(u[1] * 65)
+    while cloth_n > 32 and cloth_n * cloth_n > n_target // 2:
+        cloth_n -= 1
+    nboxes = 1 + min(7, int(u[2] * 8))
+    fluid_n = max(512, n_target - cloth_n * cloth_n - 64 * nboxes)
+    a = max(4, int(round(fluid_n ** (1.0 / 3.0))))
+    b = max(1, fluid_n // (a * a))
+    # box: fluid at the floor, boxes beside it, cloth above
+    fluid_w = a * sp
+    cloth_w = cloth_n * sp
+    width = max(2.0 * fluid_w + 1.0, cloth_w + 2.0)
+    height = b * sp + cloth_w + 2.0
+    dims = (next_pow2(width / h), next_pow2(height / h), next_pow2(width / h))
+    sc = Scene(particle_radius=rad, fluid_kernel_radius=h, cell_size=h, grid_dims=dims)
+    for k in range(nboxes):
+        v = rng.uniform(7)
+        start = (fluid_w + 0.5 + v[0] * (width - fluid_w - 1.0), 0.2 + v[1] * 1.0, 0.3 + v[2] * (width - 0.8))
+        axis = v[3:6] * 2.0 - 1.0
+        axis = axis / max(np.linalg.norm(axis), 1e-6)
+        ang = v[6] * 2.0 * math.pi
+        sc.add_rigid_box((4, 4, 4), start, sp, 1.0, rotation=(*(axis * math.sin(ang / 2)), math.cos(ang / 2)))
+    sc.add_fluid_block((a, b, a), (sp / 2, sp / 2, sp / 2), sp, 1.0, jitter=0.01 * h, rng=rng)
+    cs, _ = sc.add_cloth((0.5, b * sp + 1.0, width / 2), (sp, 0, 0), (0, sp, 0), cloth_n, cloth_n, 0.5, 0.2,
+                         0.1, True)
+    for y in range(cloth_n):
+        sc.make_immovable(cs + y)
+    sc.add_box_planes((0, 0, 0), (width, height, width))
+    wmag = 8.0 * u[3]
+    wang = 2.0 * math.pi * u[4]
+    import brasspan as bp
+    prm = bp.default_params(iterations=4, fluid_rest_density=lattice_rest_density(sp, h, 1.0), akinci=0,
+                            friction_static=0.3, friction_dynamic=0.3,
+                            wind=(wmag * math.cos(wang), 0.0, wmag * math.sin(wang)))
+    return sc, prm
+
+
+def batched_scenes(num=512, seed=4, lo=2048, hi=65536):
+    """`num` scenes with particle counts drawn log-uniform in [lo, hi]."""
+    rng = Rng(seed)
+    targets = np.exp(np.log(lo) + rng.uniform(num) * (np.log(hi) - np.log(lo))).astype(int)
+    out = [batch_member(rng, int(t)) for t in targets]
+    return [o[0] for o in out], [o[1] for o in out]
 
 
 WORKLOADS = {

@@ -111,7 +111,7 @@ typedef struct bp_scene_desc {
 /* Device view of the solver state, sorted order (bp_get_device_arrays). */
 typedef struct bp_device_view {
     int    num_particles;
-    float *new_positions;  /* float4[N]: x* (xyz, w = mass)                    */
+    float *new_positions;  /* float4[N]: x* (xyz, w = phase as int bits)       */
     float *positions;      /* float4[N]: x  (xyz, w = inverse scaled mass)     */
     float *velocities;     /* float4[N]: v  (xyz, w = 0)                       */
     int   *phases;         /* int[N]                                           */
@@ -185,6 +185,30 @@ bp_status bp_debug_fetch(bp_solver *s, int which, void *host_out);
 bp_status bp_neighbor_grid(const float *xs, int n, const float cell_size[3],
                            const int grid_dims[3], int *keys, int *perm,
                            int *cell_start, int *cell_end);
+
+/* Batched solver (no counterpart in the reference, which runs one scene):
+ * `num_scenes` independent scenes laid end to end in one solver, each with its
+ * own grid, kernel radius, planes and per-scene parameters (rest density, wind,
+ * XSPH, friction, ...); every pass runs once over all scenes.  Parameters that
+ * shape the pass sequence (iterations, pass enables, gravity, solid and cloth
+ * constants) must agree.  Original ids are global: scene s owns
+ * [first, first + count) from bp_batch_scene_range.  All other entry points
+ * (step, state, queries) act on the whole batch. */
+bp_status bp_batch_create(const bp_scene_desc *scenes, const bp_params *params, int num_scenes,
+                          void *hip_stream, bp_solver **out);
+bp_status bp_batch_scene_range(bp_solver *s, int scene, int *first_original_id, int *count);
+
+/* Replays substeps from a captured hipGraph (two substeps per graph, captured
+ * on first use and whenever dt changes).  No counterpart in the reference
+ * (which launches every kernel each frame, particlesolver.cuh:796-1062). */
+bp_status bp_solver_set_graph(bp_solver *s, int enable);
+
+/* Chooses the workgroup size of each neighbour pass among 64..512 by timing
+ * `substeps_per_trial` substeps of size dt per candidate, then restores the
+ * particle and rigid state it started from (results do not depend on the
+ * workgroup sizes).  Writes a JSON object {pass name: size} into `report`
+ * (may be NULL).  No counterpart in the reference. */
+bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, char *report, int report_len);
 
 /* Timing helper for benchmarks: runs `substeps` substeps of size dt and
  * returns the elapsed GPU time in ms measured with HIP events on the
