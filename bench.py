@@ -4,7 +4,9 @@ A step is one substep (grid build + sort + 4 solver iterations + post passes)
 of the plain solver over a synthetic scene resident in HBM.  Prints one JSON
 line.  The default workload is the 282,624-particle mixed-phase scene (fluid +
 64 shape-matched boxes + 128x128 cloth under wind); the same line also carries
-the 4,194,304-particle large block, the HBM-roofline case.
+the 4,194,304-particle large block, the HBM-roofline case, and the 512-scene
+batch (2k..64k particles per scene) against the plain solver looped over the
+same scenes.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
@@ -121,6 +123,75 @@ def cpu_baseline(sc, prm, subdt, budget_s=20.0):
                                       f"FP32 oracle in reference order"}
 
 
+def hip_stream():
+    """A new HIP stream at the default priority (a raw handle for the C ABI)."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    h = C.c_void_p()
+    assert hip.hipStreamCreate(C.byref(h)) == 0
+    return h.value
+
+
+def time_batched(frames=3, warmup=1, num=512):
+    """The 512-scene batched workload (2 substeps per frame of 1/60 s) in one
+    bp_batch_create handle with graphs and tuned block sizes, against the
+    plain solver looped over the same scenes on one stream, eager and with
+    graphs.  The batch must equal the eager loop bit for bit."""
+    scs, prms = scenes.batched_scenes(num)
+    dt, sub = 1.0 / 60.0, 2
+    n = sum(sc.num_particles() for sc in scs)
+    bs = bp.BatchSolver(scs, prms)
+    tune = bs.autotune(dt / sub, 2)
+    bs.set_graph(True)
+    for _ in range(warmup):
+        bs.step(sub, dt)
+    bs.sync()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        bs.step(sub, dt)
+    bs.sync()
+    wall_b = time.perf_counter() - t0
+    bpos, _ = bs.state_original_order()
+    ranges = bs.ranges
+    bs.close()
+
+    stream = hip_stream()
+    loop = [bp.Solver(sc, prm, stream=stream) for sc, prm in zip(scs, prms)]
+
+    def run_loop(count):
+        for _ in range(count):
+            for s in loop:
+                s.step(sub, dt)
+        loop[-1].sync()
+
+    run_loop(warmup)
+    t0 = time.perf_counter()
+    run_loop(frames)
+    wall_e = time.perf_counter() - t0
+    same = all(np.array_equal(bpos[f:f + c], s.state_original_order()[0]) for (f, c), s in zip(ranges, loop))
+    for s in loop:
+        s.set_graph(True)
+    run_loop(warmup)
+    t0 = time.perf_counter()
+    run_loop(frames)
+    wall_g = time.perf_counter() - t0
+    for s in loop:
+        s.close()
+    ps = n * sub * frames
+    best_loop = min(wall_e, wall_g)
+    return {
+        "value": round(ps / wall_b, 1), "unit": "particle-substeps/s", "ms_per_substep": round(wall_b / (frames * sub) * 1e3, 3),
+        "plain_loop_eager": round(ps / wall_e, 1), "plain_loop_graph": round(ps / wall_g, 1),
+        "speedup_vs_plain_loop": round(best_loop / wall_b, 2),
+        "bitexact_vs_plain_loop": bool(same),
+        "config": {"workload": f"batched_{num}_scenes", "scenes": num, "particles": n,
+                   "particles_min": min(c for _, c in ranges), "particles_max": max(c for _, c in ranges),
+                   "iterations": prms[0].iterations, "substeps_per_frame": sub, "frames": frames,
+                   "warmup_frames": warmup, "block_sizes": tune},
+    }
+
+
 def copy_calibration(mib):
     """A device-to-device copy of `mib` MiB through the HIP runtime: the known
     traffic (mib read + mib written) that scales rocprof FETCH_SIZE and
@@ -147,6 +218,7 @@ def main():
     ap.add_argument("--workload", default="mixed_phase_283k", choices=sorted(scenes.WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-large", action="store_true", help="skip the 4M large-block measurement")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 512-scene batched measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device (rocprof counter calibration)")
     args = ap.parse_args()
@@ -164,7 +236,7 @@ def main():
     roof, fam_times = kernel_roofline(s, subdt, workload=args.workload)
     s.close()
     out = {
-        "metric": "particle-substeps/s (4 iters)", "value": round(value, 1), "unit": "particle-substeps/s",
+        "metric": "particle-substeps/s (4 iters) plain vs 512-scene batched", "value": round(value, 1), "unit": "particle-substeps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded splitmix64 scene, no checkpoint)",
@@ -187,6 +259,8 @@ def main():
                                  "ms_per_step": round(lwall / lsteps * 1e3, 4), "steps": lsteps,
                                  "warmup": lwarm, "roofline": lroof, "kernel_ms": ltimes}
         ls.close()
+    if not args.no_batched:
+        out["batched_512"] = time_batched()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, prm, subdt)
     print(json.dumps(out), flush=True)
