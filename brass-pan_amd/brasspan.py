@@ -93,6 +93,9 @@ SIGNATURES = {
     "bp_batch_scene_range": (_S, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "bp_solver_set_graph": (_S, [_P, C.c_int]),
     "bp_solver_autotune": (_S, [_P, C.c_float, C.c_int, C.c_char_p, C.c_int]),
+    "bp_forward_checkpointed": (_S, [_P, C.c_int, C.c_float, C.c_int]),
+    "bp_backward": (_S, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                         C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "bp_solver_timed_substeps": (_S, [_P, C.c_int, C.c_float, C.POINTER(C.c_float)]),
     "bp_solver_enable_kernel_timing": (_S, [_P, C.c_int]),
     "bp_solver_kernel_times": (_S, [_P, C.c_int, C.c_char_p, C.POINTER(C.c_float), C.POINTER(C.c_int),
@@ -228,6 +231,24 @@ class Solver:
         buf = C.create_string_buffer(512)
         _check(self.lib.bp_solver_autotune(self.h, float(dt), int(substeps_per_trial), buf, 512))
         return json.loads(buf.value.decode())
+
+    # ---------------------------------------------------------- adjoint
+    def forward_checkpointed(self, substeps: int, subdt: float, every: int = 8):
+        """Differentiable rollout (bp_forward_checkpointed)."""
+        _check(self.lib.bp_forward_checkpointed(self.h, int(substeps), float(subdt), int(every)))
+
+    def backward(self, grad_x, grad_v=None):
+        """Adjoint of the last rollout (bp_backward).  Returns dL/dx0, dL/dv0
+        (N x 3, original order) and dL/dtheta (scenes x 6: wind xyz, drag,
+        lift, xsph_c)."""
+        gx = None if grad_x is None else np.ascontiguousarray(grad_x, np.float32).reshape(self.n, 3)
+        gv = None if grad_v is None else np.ascontiguousarray(grad_v, np.float32).reshape(self.n, 3)
+        gx0 = np.zeros((self.n, 3), np.float32)
+        gv0 = np.zeros((self.n, 3), np.float32)
+        nsc = len(getattr(self, "ranges", [None]))
+        gth = np.zeros((nsc, 6), np.float32)
+        _check(self.lib.bp_backward(self.h, _fp(gx), _fp(gv), _fp(gx0), _fp(gv0), _fp(gth)))
+        return gx0, gv0, gth
 
     def close(self):
         if getattr(self, "h", None):

@@ -28,7 +28,8 @@ template <class SP>
 __global__ __launch_bounds__(kMaxBlock) void k_particle_particle(const float4 *__restrict__ xs,
                                                                  const float4 *__restrict__ x,
                                                                  const int2 *__restrict__ ranges,
-                                                                 float4 *__restrict__ xs_out, int n, SP sp) {
+                                                                 float4 *__restrict__ xs_out, int n, SP sp,
+                                                                 int *__restrict__ contacts) {
     const int i = xcd_index();
     if (i >= n) return;
     const SceneConst &S = sp.at(i);
@@ -83,6 +84,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_particle_particle(const float4 *_
         });
     const f3 d = (cnt == 0) ? sumD : sumD + sumF / (float)cnt;
     xs_out[i] = f4(xip + d, a.w);
+    // the adjoint treats this pass as the identity: it counts the particles
+    // whose contact set is not empty
+    if (contacts && (sumD.x != 0.0f || sumD.y != 0.0f || sumD.z != 0.0f || cnt > 0)) atomicAdd(contacts, 1);
 }
 
 // distanceConstraints (distance.cuh:3-41): k of each pair
@@ -358,9 +362,9 @@ void launch_plane_collide(float4 *xs, const float4 *x, int n, const SP &sp, hipS
 }
 template <class SP>
 void launch_particle_particle(const float4 *xs, const float4 *x, const int2 *ranges, float4 *xs_out, int n,
-                              const SP &sp, int block, hipStream_t st) {
+                              const SP &sp, int block, int *contacts, hipStream_t st) {
     hipLaunchKernelGGL(k_particle_particle<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, x, ranges,
-                       xs_out, n, sp);
+                       xs_out, n, sp, contacts);
 }
 void launch_distance(float4 *xs, const float4 *x, const int2 *pairs, const float2 *params, const int *map_to_new,
                      float4 *kbuf, int nc, const int *verts, const int *offs, const int *ents, int nv,
@@ -404,7 +408,7 @@ void launch_finish(const float4 *xs, float4 *x, float4 *v, int n, float inv_dt, 
 #define BP_INSTANTIATE_CONSTRAINTS(SP)                                                                          \
     template void launch_plane_collide<SP>(float4 *, const float4 *, int, const SP &, hipStream_t);              \
     template void launch_particle_particle<SP>(const float4 *, const float4 *, const int2 *, float4 *, int,      \
-                                               const SP &, int, hipStream_t);                                    \
+                                               const SP &, int, int *, hipStream_t);                             \
     template void launch_wind<SP>(float4 *, const float4 *, const float *, const int4 *, const int *, float4 *,  \
                                   int, const SP &, float, const int *, const int *, const int *, int, hipStream_t);
 BP_INSTANTIATE_CONSTRAINTS(OneScene)

@@ -9,12 +9,6 @@
 
 namespace bp {
 
-__device__ __forceinline__ void unpack_cell(uint32_t c, int g[3]) {
-    g[0] = (int)(c >> 20);
-    g[1] = (int)((c >> 10) & 1023u);
-    g[2] = (int)(c & 1023u);
-}
-
 // fluidLambda (fluid.cuh:78-159).  Also records the particle's current cell
 // (packed) for the position pass, which searches from the same x*.
 template <class SP>

@@ -169,6 +169,12 @@ struct SceneTable {
 
 // ------------------------------------------------------------ device helpers
 __device__ __forceinline__ int phase_of(float4 a) { return __float_as_int(a.w); }
+// packed 10:10:10 cell (GridDesc::pack) -> grid position
+__device__ __forceinline__ void unpack_cell(uint32_t c, int g[3]) {
+    g[0] = (int)(c >> 20);
+    g[1] = (int)((c >> 10) & 1023u);
+    g[2] = (int)(c & 1023u);
+}
 
 // particlePlaneCollisionConstraint (particleplane.cuh:21-58) for one particle
 __device__ __forceinline__ f3 plane_collide_one(f3 xs, f3 pos, const Planes &pl, float r, float fs, float fd) {

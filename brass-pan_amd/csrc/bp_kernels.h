@@ -47,7 +47,7 @@ template <class SP>
 void launch_plane_collide(float4 *xs, const float4 *x, int n, const SP &sp, hipStream_t st);
 template <class SP>
 void launch_particle_particle(const float4 *xs, const float4 *x, const int2 *ranges, float4 *xs_out, int n,
-                              const SP &sp, int block, hipStream_t st);
+                              const SP &sp, int block, int *contacts, hipStream_t st);
 void launch_distance(float4 *xs, const float4 *x, const int2 *pairs, const float2 *params, const int *map_to_new,
                      float4 *kbuf, int nc, const int *verts, const int *offs, const int *ents, int nv,
                      hipStream_t st);
@@ -63,5 +63,38 @@ void launch_immovable(float4 *xs, const float4 *x, const int *ids, const int *ma
 void launch_pick(float4 *pos, float4 *v, const int *map_to_new, int orig, f3 p, int keep_pos, f3 vel,
                  hipStream_t st);
 void launch_finish(const float4 *xs, float4 *x, float4 *v, int n, float inv_dt, float eps2, hipStream_t st);
+
+// bp_adjoint.hip (adjoint passes; g* buffers hold adjoints in slot order)
+void launch_adj_inverse(const int *orig, int *map_to_new, int n, hipStream_t st);
+void launch_adj_seed(const float *seed, const int *orig, float4 *g, float scale, int n, hipStream_t st);
+void launch_adj_unseed(const float4 *g, const int *orig, float *out, float inv_scale, int n, hipStream_t st);
+template <class SP>
+void launch_adj_cells(const float4 *xs, uint32_t *ccell, int n, const SP &sp, hipStream_t st);
+template <class SP>
+void launch_adj_xsph(const float4 *xf, const float4 *x1, const float4 *gx2, const float4 *gv3, const uint32_t *ccell,
+                     const int *keys, const int2 *ranges, float4 *gv2_own, float4 *gx_own, float *cacc,
+                     unsigned long long *corr, float4 *gxs, float4 *gx1, float *gw, float inv_dt, float eps2, int n,
+                     const SP &sp, int block, hipStream_t st);
+template <class SP>
+void launch_adj_projection(const float4 *xs, const float *m, const float *lambda, const uint32_t *ccell,
+                           const int *keys, const int2 *ranges, float4 *gxs, float4 *gx_own, float *glam_own,
+                           float *grho, unsigned long long *corr, int n, const SP &sp, int block, hipStream_t st);
+template <class SP>
+void launch_adj_planes(const float4 *xp, const float4 *x1, float4 *gxs, float4 *gx1, int n, const SP &sp,
+                       hipStream_t st);
+void launch_adj_immovable(float4 *gxs, float4 *gx1, const int *ids, const int *map_to_new, int ni, hipStream_t st);
+void launch_adj_distance(const float4 *xd, const float4 *x1, const int2 *pairs, const float2 *params,
+                         const int *map_to_new, float4 *gxs, float *gw, float4 *cb, float2 *wb, int nc,
+                         const int *verts, const int *offs, const int *ents, int nv, hipStream_t st);
+template <class SP>
+void launch_adj_wind(const float4 *xsw, const float4 *v_in, const int *perm, const float *m, const int4 *faces,
+                     const int *map_to_new, float4 *gxs, float4 *gv1, float4 *fbar, float4 *facc, int nf, const SP &sp,
+                     float dt, const int *verts, const int *offs, const int *ents, int nv, hipStream_t st);
+template <class SP>
+void launch_adj_reorder(const int *perm, const float4 *x_in, const float4 *x1, const float4 *gxs, const float4 *gx1,
+                        const float4 *gv1, const float *gw, float4 *gx_old, float4 *gv_old, int n, float dt,
+                        float mass_k, int stab_passes, const SP &sp, hipStream_t st);
+void launch_adj_scene_reduce(const float4 *facc, const float *cacc, const int *poff, const int *foff, float *out,
+                             int nscenes, hipStream_t st);
 
 }  // namespace bp

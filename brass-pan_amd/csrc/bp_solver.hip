@@ -18,12 +18,12 @@
 #include <string>
 #include <vector>
 
-#include "../../include/brasspan.h"
-#include "bp_kernels.h"
+#include "bp_solver_impl.h"
 
 using namespace bp;
+using namespace bp_host;
 
-namespace {
+namespace bp_host {
 
 thread_local std::string g_last_error;
 
@@ -32,169 +32,16 @@ bp_status fail(bp_status st, const std::string &msg) {
     return st;
 }
 
-#define HIP_TRY(expr)                                                                                   \
-    do {                                                                                                \
-        hipError_t e_ = (expr);                                                                         \
-        if (e_ != hipSuccess)                                                                           \
-            return fail(BP_ERR_HIP, std::string(#expr) + " failed: " + hipGetErrorString(e_));          \
-    } while (0)
+}  // namespace bp_host
 
-template <class T> struct DBuf {
-    T *b[2] = {nullptr, nullptr};
-    int c = 0;
-    T *cur() const { return b[c]; }
-    T *alt() const { return b[1 - c]; }
-    void flip() { c = 1 - c; }
-};
+namespace {
 
-struct Csr {
-    int nv = 0;
-    int *verts = nullptr, *offs = nullptr, *ents = nullptr;
-};
-
-enum Family {
-    F_PREDICT, F_SORT, F_RANGES, F_REORDER, F_WIND, F_PLANES, F_PP, F_LAMBDA, F_POSITION, F_DISTANCE,
-    F_BENDING, F_SHAPE, F_IMMOVABLE, F_FINISH, F_POST_A, F_POST_B, F_XSPH, F_PICK, F_COUNT
-};
 const char *kFamilyNames[F_COUNT] = {"predict_hash", "radix_sort", "cell_ranges", "reorder", "wind",
                                      "plane_collide", "particle_particle", "fluid_lambda", "fluid_position",
                                      "distance", "bending", "shape_matching", "immovable", "finish",
                                      "omega_normal", "vorticity_tension", "xsph", "pick"};
 
-bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
-int ilog2(unsigned v) {
-    int r = 0;
-    while ((1u << r) < v) ++r;
-    return r;
-}
-
 }  // namespace
-
-struct bp_solver {
-    int device = 0;
-    hipStream_t st = nullptr;
-    bool own_stream = false;
-    bp_params P{};
-    int N = 0;
-    int nscenes = 1;
-    int num_cells = 0, sort_bits = 0;
-    std::vector<SceneConst> hsc;       // per-scene kernel constants
-    SceneConst *dsc = nullptr;         // device copy (batched solver)
-    std::vector<int> poff;             // scene s owns original ids [poff[s], poff[s+1])
-    std::vector<int> scene_of_orig;
-    bool has_solid = false, has_fluid = false, any_planes = false;
-
-    DBuf<float4> xs, x, v;
-    DBuf<float> m;
-    DBuf<int> ph, orig, sid;
-    int *map_to_new = nullptr;
-
-    int *keys[2] = {nullptr, nullptr};
-    int *vals[2] = {nullptr, nullptr};
-    int *counts = nullptr, *totals = nullptr;
-    int sk_idx = -1;  // keys buffer holding the last sorted keys
-    int *sorted_keys = nullptr, *perm = nullptr;
-    int2 *ranges = nullptr;
-
-    float *lambda = nullptr, *density = nullptr;
-    uint32_t *ccell = nullptr;
-    float4 *sum_tmp = nullptr;
-    unsigned long long *corr = nullptr;  // fixed-point pair corrections, kept zero between passes
-    float4 *omega = nullptr, *normal = nullptr;
-
-    int nd = 0;
-    int2 *dpairs = nullptr;
-    float2 *dparams = nullptr;
-    float4 *kbuf = nullptr;
-    Csr dcsr;
-    int nb = 0;
-    int4 *bends = nullptr;
-    float4 *bbuf = nullptr;
-    Csr bcsr;
-    int nf = 0;
-    int4 *faces = nullptr;
-    float4 *fbuf = nullptr;
-    Csr fcsr;
-    int nr = 0;
-    int2 *rranges = nullptr;
-    float4 *rrest = nullptr, *rrot = nullptr, *rcm = nullptr;
-    int ni = 0;
-    int *immov = nullptr;
-
-    std::vector<float> mass_by_orig;
-    std::vector<int> phase_by_orig;
-    std::vector<void *> allocs;
-
-    // neighbour-pass workgroup sizes (bp_solver_autotune)
-    int blk_lambda = 256, blk_position = 256, blk_post_a = 256, blk_post_b = 256, blk_xsph = 256, blk_pp = 256;
-
-    // hipGraph of two substeps (the ping-pong buffers return to their start)
-    bool use_graph = false;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    float graph_dt = 0.f;
-    int graph_sig = -1;  // buffer parities the graph was captured with (buffer_sig)
-
-    // per-family kernel timing (bp_solver_enable_kernel_timing)
-    bool timing = false;
-    std::vector<hipEvent_t> ev_pool;
-    size_t ev_next = 0;
-    struct Rec {
-        int fam;
-        hipEvent_t a, b;
-    };
-    std::vector<Rec> recs;
-    float fam_ms[F_COUNT] = {};
-    int fam_n[F_COUNT] = {};
-
-    template <class T> bp_status alloc(T **p, size_t count) {
-        *p = nullptr;
-        if (count == 0) count = 1;
-        hipError_t e = hipMalloc((void **)p, count * sizeof(T));
-        if (e != hipSuccess) return fail(BP_ERR_HIP, std::string("hipMalloc failed: ") + hipGetErrorString(e));
-        allocs.push_back((void *)*p);
-        return BP_OK;
-    }
-    template <class T> bp_status upload(T **p, const T *host, size_t count) {
-        bp_status s = alloc(p, count);
-        if (s != BP_OK) return s;
-        if (count && host) HIP_TRY(hipMemcpyAsync(*p, host, count * sizeof(T), hipMemcpyHostToDevice, st));
-        return BP_OK;
-    }
-
-    void tbeg(int fam, hipEvent_t *out) {
-        *out = nullptr;
-        if (!timing) return;
-        if (ev_next + 2 > ev_pool.size()) return;
-        *out = ev_pool[ev_next];
-        (void)hipEventRecord(*out, st);
-        ev_next += 1;
-        (void)fam;
-    }
-    void tend(int fam, hipEvent_t a) {
-        if (!timing || !a) return;
-        hipEvent_t b = ev_pool[ev_next++];
-        (void)hipEventRecord(b, st);
-        recs.push_back(Rec{fam, a, b});
-    }
-    // which half of every ping-pong pair is current: a captured graph holds
-    // raw pointers, so it replays only from the parities it was captured at
-    int buffer_sig() const { return x.c | xs.c << 1 | v.c << 2 | (sk_idx + 1) << 3; }
-    void drop_graph() {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (graph) (void)hipGraphDestroy(graph);
-        gexec = nullptr;
-        graph = nullptr;
-    }
-};
-
-#define TIMED(fam, stmt)      \
-    do {                      \
-        hipEvent_t ev_;       \
-        s->tbeg(fam, &ev_);   \
-        stmt;                 \
-        s->tend(fam, ev_);    \
-    } while (0)
 
 namespace {
 
@@ -281,10 +128,23 @@ bool same_schedule(const bp_params &a, const bp_params &b) {
            a.nan_guard == b.nan_guard;
 }
 
+}  // namespace
+
+namespace bp_host {
+
+namespace {
+// device-to-device snapshot of `count` elements on the solver's stream
+template <class T> hipError_t snap(T *dst, const T *src, size_t count, hipStream_t st) {
+    if (!dst) return hipSuccess;
+    return hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToDevice, st);
+}
+}  // namespace
+
 // one substep of ParticleSolver::update (particlesolver.cuh:787-1063); `sp()`
 // returns the scene parameter source for the current slot order
 template <class SPF>
-bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick, int picked, f3 ppos, SPF sp) {
+bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick, int picked, f3 ppos, SPF sp,
+                    bool no_fuse, const SubstepHooks *hk) {
     const int N = s->N;
     const bp_params &P = s->P;
     hipStream_t st = s->st;
@@ -293,8 +153,11 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
     const bool tail_groups = s->nd > 0 || s->nb > 0 || s->nr > 0 || s->ni > 0;
     const bool need_w = s->has_solid || s->nd > 0 || s->nb > 0;
     const int iters = P.iterations;
+    const SubstepHooks H = hk ? *hk : SubstepHooks{};
 
     if (pick) TIMED(F_PICK, launch_pick(s->x.cur(), s->v.cur(), s->map_to_new, picked, ppos, 0, zero, st));
+    HIP_TRY(snap(H.x_in, s->x.cur(), N, st));
+    HIP_TRY(snap(H.v_in, s->v.cur(), N, st));
 
     // predict + hash, clearing the ranges of the previous substep
     int kin_idx = (s->sk_idx == 0) ? 1 : 0;
@@ -308,8 +171,10 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
     s->perm = pm;
     s->sk_idx = (sk == s->keys[0]) ? 0 : 1;
     TIMED(F_RANGES, launch_find_ranges(sk, N, s->ranges, st));
+    HIP_TRY(snap(H.keys, sk, N, st));
+    HIP_TRY(snap(H.perm, pm, N, st));
 
-    const bool fuse_first = s->any_planes && !wind_on && !pick && iters > 0;
+    const bool fuse_first = !no_fuse && s->any_planes && !wind_on && !pick && iters > 0;
     TIMED(F_REORDER, launch_reorder(pm, s->x.cur(), s->v.cur(), s->m.cur(), s->ph.cur(), s->orig.cur(), s->sid.cur(),
                                     s->xs.cur(), s->x.alt(), s->v.alt(), s->m.alt(), s->ph.alt(), s->orig.alt(),
                                     s->sid.alt(), s->map_to_new, N, subdt, gstep, P.mass_scaling_k, need_w ? 1 : 0,
@@ -320,6 +185,11 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
     s->ph.flip();
     s->orig.flip();
     if (s->sid.b[0]) s->sid.flip();
+    HIP_TRY(snap(H.x1, s->x.cur(), N, st));
+    HIP_TRY(snap(H.xsw, s->xs.cur(), N, st));
+    HIP_TRY(snap(H.m, s->m.cur(), N, st));
+    HIP_TRY(snap(H.orig, s->orig.cur(), N, st));
+    if (s->sid.b[0]) HIP_TRY(snap(H.sid, s->sid.cur(), N, st));
 
     if (wind_on)
         TIMED(F_WIND, launch_wind(s->xs.cur(), s->v.cur(), s->m.cur(), s->faces, s->map_to_new, s->fbuf, s->nf, sp(),
@@ -329,19 +199,22 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
     bool finished = false;
     for (int it = 0; it < iters; ++it) {
         if (pick) TIMED(F_PICK, launch_pick(s->xs.cur(), s->v.cur(), s->map_to_new, picked, ppos, 0, zero, st));
+        if (H.xp) HIP_TRY(snap(H.xp[it], s->xs.cur(), N, st));
         if (s->any_planes && !planes_done) TIMED(F_PLANES, launch_plane_collide(s->xs.cur(), s->x.cur(), N, sp(), st));
         planes_done = false;
         if (s->has_solid) {
             TIMED(F_PP, launch_particle_particle(s->xs.cur(), s->x.cur(), s->ranges, s->xs.alt(), N, sp(), s->blk_pp,
-                                                 st));
+                                                 H.contacts, st));
             s->xs.flip();
         }
+        if (H.xl) HIP_TRY(snap(H.xl[it], s->xs.cur(), N, st));
         if (s->has_fluid) {
             TIMED(F_LAMBDA, launch_lambda(s->xs.cur(), s->m.cur(), s->ranges, s->lambda, s->density, s->ccell, N, sp(),
                                           s->blk_lambda, st));
+            if (H.lam) HIP_TRY(snap(H.lam[it], s->lambda, N, st));
             Epilogue E{};
-            E.planes_next = (s->any_planes && it + 1 < iters && !tail_groups && !pick) ? 1 : 0;
-            E.finish = (it + 1 == iters && !tail_groups) ? 1 : 0;
+            E.planes_next = (!no_fuse && s->any_planes && it + 1 < iters && !tail_groups && !pick) ? 1 : 0;
+            E.finish = (!no_fuse && it + 1 == iters && !tail_groups) ? 1 : 0;
             E.inv_dt = inv_dt;
             E.sleep_eps2 = P.sleep_epsilon * P.sleep_epsilon;
             TIMED(F_POSITION, launch_position(s->xs.cur(), s->m.cur(), s->lambda, s->ccell, sk, s->ranges,
@@ -351,6 +224,7 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
             planes_done = E.planes_next;
             finished = E.finish;
         }
+        if (H.xd) HIP_TRY(snap(H.xd[it], s->xs.cur(), N, st));
         if (s->nd > 0)
             TIMED(F_DISTANCE, launch_distance(s->xs.cur(), s->x.cur(), s->dpairs, s->dparams, s->map_to_new, s->kbuf,
                                               s->nd, s->dcsr.verts, s->dcsr.offs, s->dcsr.ents, s->dcsr.nv, st));
@@ -363,6 +237,7 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
                                                  s->nr, P.shape_matching_iterations, st));
         if (s->ni > 0) TIMED(F_IMMOVABLE, launch_immovable(s->xs.cur(), s->x.cur(), s->immov, s->map_to_new, s->ni, st));
     }
+    HIP_TRY(snap(H.xf, s->xs.cur(), N, st));
     if (!finished)
         TIMED(F_FINISH, launch_finish(s->xs.cur(), s->x.cur(), s->v.cur(), N, inv_dt,
                                       P.sleep_epsilon * P.sleep_epsilon, st));
@@ -386,19 +261,24 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
     return BP_OK;
 }
 
-bp_status substep(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick, int picked, f3 ppos) {
+bp_status substep(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick, int picked, f3 ppos, bool no_fuse,
+                  const SubstepHooks *hk) {
     if (s->nscenes == 1) {
         const OneScene one{s->hsc[0]};
-        return substep_t(s, subdt, inv_dt, gstep, pick, picked, ppos, [&]() { return one; });
+        return substep_t(s, subdt, inv_dt, gstep, pick, picked, ppos, [&]() { return one; }, no_fuse, hk);
     }
-    return substep_t(s, subdt, inv_dt, gstep, pick, picked, ppos,
-                     [&]() { return SceneTable{s->dsc, s->sid.cur()}; });
+    return substep_t(s, subdt, inv_dt, gstep, pick, picked, ppos, [&]() { return SceneTable{s->dsc, s->sid.cur()}; },
+                     no_fuse, hk);
 }
 
 f3 gravity_step(const bp_params &P, float subdt) {
     // make_float3(0,-9.8,0) * dt * dt (particlesolver.cuh:56), per component
     return mk3(P.gravity[0] * subdt * subdt, P.gravity[1] * subdt * subdt, P.gravity[2] * subdt * subdt);
 }
+
+}  // namespace bp_host
+
+namespace {
 
 // Runs `count` plain substeps; with a graph enabled, pairs of substeps replay a
 // captured graph (the ping-pong buffers return to their start after two).
@@ -762,6 +642,7 @@ bp_status bp_solver_destroy(bp_solver *s) {
     if (!s) return BP_OK;
     if (s->st) (void)hipStreamSynchronize(s->st); else (void)hipDeviceSynchronize();
     s->drop_graph();
+    adjoint_release(s);
     for (void *p : s->allocs) (void)hipFree(p);
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
     if (s->own_stream) (void)hipStreamDestroy(s->st);
@@ -771,6 +652,7 @@ bp_status bp_solver_destroy(bp_solver *s) {
 
 bp_status bp_solver_set_params(bp_solver *s, const bp_params *p) {
     if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
+    adjoint_invalidate(s);
     bp_status st = check_params(p);
     if (st != BP_OK) return st;
     if (s->nscenes != 1) return fail(BP_ERR_UNSUPPORTED, "per-scene parameters of a batch are fixed at creation");
@@ -783,6 +665,7 @@ bp_status bp_solver_set_params(bp_solver *s, const bp_params *p) {
 bp_status bp_solver_step(bp_solver *s, int num_substeps, float dt, int picked, const float *ppos,
                          const float *pvel) {
     if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
+    adjoint_invalidate(s);
     if (num_substeps <= 0) return fail(BP_ERR_INVALID_ARG, "num_substeps must be > 0");
     if (s->N <= 0) return BP_OK;  // particlesolver.cuh:778
     const float subdt = dt / (float)num_substeps;  // :780
@@ -824,6 +707,7 @@ bp_status bp_solver_set_graph(bp_solver *s, int enable) {
 
 bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, char *report, int report_len) {
     if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
+    adjoint_invalidate(s);
     if (substeps_per_trial <= 0) return fail(BP_ERR_INVALID_ARG, "substeps_per_trial must be > 0");
     const int cand[4] = {64, 128, 256, 512};
     const int fams[6] = {F_LAMBDA, F_POSITION, F_POST_A, F_POST_B, F_XSPH, F_PP};
@@ -916,6 +800,7 @@ bp_status bp_get_particle(bp_solver *s, int slot, float out_pos[3]) {
 
 bp_status bp_set_particle(bp_solver *s, int slot, const float pos[3], const float vel[3]) {
     if (!s || !pos || !vel) return fail(BP_ERR_INVALID_ARG, "NULL argument");
+    adjoint_invalidate(s);
     if (slot < 0 || slot >= s->N) return BP_OK;  // particlesolver.cuh:390
     float4 a, b;
     HIP_TRY(hipMemcpyAsync(&a, s->x.cur() + slot, sizeof(float4), hipMemcpyDeviceToHost, s->st));
@@ -978,6 +863,7 @@ bp_status bp_copy_state_original_order(bp_solver *s, float *pos, float *vel) {
 
 bp_status bp_set_state(bp_solver *s, const float *pos, const float *vel, const int *map) {
     if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
+    adjoint_invalidate(s);
     const int N = s->N;
     if (N == 0) return BP_OK;
     std::vector<float4> hx(N), hv(N);
@@ -1035,6 +921,7 @@ bp_status bp_get_rigid_state(bp_solver *s, float *quats, float *cms) {
 
 bp_status bp_set_rigid_state(bp_solver *s, const float *quats) {
     if (!s || !quats) return fail(BP_ERR_INVALID_ARG, "NULL argument");
+    adjoint_invalidate(s);
     if (s->nr == 0) return BP_OK;
     std::vector<float4> q(s->nr);
     for (int b = 0; b < s->nr; ++b) q[b] = make_float4(quats[4 * b], quats[4 * b + 1], quats[4 * b + 2], quats[4 * b + 3]);
@@ -1147,6 +1034,7 @@ bp_status bp_neighbor_grid(const float *xs, int n, const float cell_size[3], con
 
 bp_status bp_solver_timed_substeps(bp_solver *s, int substeps, float dt, float *out_ms) {
     if (!s || !out_ms) return fail(BP_ERR_INVALID_ARG, "NULL argument");
+    adjoint_invalidate(s);
     if (substeps <= 0) return fail(BP_ERR_INVALID_ARG, "substeps must be > 0");
     hipEvent_t a, b;
     HIP_TRY(hipEventCreate(&a));

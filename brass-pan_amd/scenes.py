@@ -444,6 +444,47 @@ def batched_scenes(num=512, seed=4, lo=2048, hi=65536):
     return [o[0] for o in out], [o[1] for o in out]
 
 
+def adjoint_member(rng: Rng, fluid_dims=(32, 16, 32), cloth_n=64, box=6.4, grid=(64, 64, 64)):
+    """One scene of the differentiable workload: a jittered PBF fluid block
+    (32x16x32 = 16,384 by default) on the floor of a frictionless box and a
+    cloth flag of cloth_n^2 particles (one solid phase, no self-contact)
+    pinned along its left edge, high enough above the fluid that the two never
+    touch, under a seeded wind of 2..6 m/s."""
+    u = rng.uniform(3)
+    sc = Scene(particle_radius=RADIUS, fluid_kernel_radius=H, cell_size=H, grid_dims=grid)
+    # 1 cm clear of the floor and walls: a particle started exactly at contact
+    # distance sits on the kink of the plane projection, where the rollout
+    # has no derivative
+    sc.add_fluid_block(fluid_dims, (SPACING / 2 + 0.01,) * 3, SPACING, 1.0, jitter=0.01 * H, rng=rng)
+    top = fluid_dims[1] * SPACING + 0.01
+    x0 = fluid_dims[0] * SPACING * 0.25
+    y0 = top + 1.2
+    cs, _ = sc.add_cloth((x0, y0, box * 0.6), (SPACING, 0, 0), (0, SPACING, 0), cloth_n, cloth_n, 0.5, 0.2, 0.1,
+                         False)
+    for y in range(cloth_n):
+        sc.make_immovable(cs + y)
+    sc.add_box_planes((0, 0, 0), (box, box, box))
+    wmag = 2.0 + 4.0 * u[0]
+    wang = 0.5 * math.pi * (u[1] - 0.5)
+    import brasspan as bp
+    prm = bp.default_params(iterations=4, fluid_rest_density=lattice_rest_density(SPACING, H, 1.0), akinci=0,
+                            vorticity_eps=0.0, xsph_c=2e-5, friction_static=0.0, friction_dynamic=0.0,
+                            wind=(wmag * math.cos(wang), 0.0, wmag * math.sin(wang)))
+    return sc, prm
+
+
+def adjoint_scenes(num=64, seed=5, **kw):
+    """The differentiable workload: `num` scenes of adjoint_member."""
+    rng = Rng(seed)
+    out = [adjoint_member(rng, **kw) for _ in range(num)]
+    return [o[0] for o in out], [o[1] for o in out]
+
+
+ADJOINT_SUBSTEPS = 32
+ADJOINT_SUBDT = 1.0 / 120.0
+ADJOINT_EVERY = 8
+
+
 WORKLOADS = {
     # name: (builder, params builder, dt, substeps per frame)
     "dam_break_16k": (dam_break, dam_break_params, 1.0 / 60.0, 2),

@@ -203,6 +203,32 @@ bp_status bp_batch_scene_range(bp_solver *s, int scene, int *first_original_id, 
  * (which launches every kernel each frame, particlesolver.cuh:796-1062). */
 bp_status bp_solver_set_graph(bp_solver *s, int enable);
 
+/* ---------------------------------------------------------------- adjoint
+ * Differentiable rollout (no counterpart in the reference).  Differentiated:
+ * prediction, wind (drag/lift with the wind vector), plane collisions, the
+ * PBF projection (lambda_mode 0, sCorr, clamp), distance constraints, pins,
+ * the velocity/position update and XSPH.  Scenes with rigid bodies, bending
+ * constraints, vorticity or Akinci tension are refused (BP_ERR_UNSUPPORTED),
+ * and so is a rollout in which particle-particle contact occurs.  Neighbour
+ * sets are those of the forward passes, held fixed.
+ *
+ * bp_forward_checkpointed advances `substeps` substeps of size `subdt`
+ * (every pass a separate kernel, no picking) and keeps the full particle
+ * state entering every `checkpoint_every`-th substep. */
+bp_status bp_forward_checkpointed(bp_solver *s, int substeps, float subdt, int checkpoint_every);
+
+/* Reverse sweep over the last bp_forward_checkpointed rollout: each segment
+ * is replayed from its checkpoint (recording the state before every pass),
+ * then differentiated in reverse.  Inputs: dL/dx_T, dL/dv_T (float3 per
+ * particle, original order; either may be NULL = 0).  Outputs (each may be
+ * NULL): dL/dx_0, dL/dv_0 (float3, original order) and, per scene s,
+ * grad_theta[6 s .. 6 s + 5] = dL/d(wind x, wind y, wind z, drag_coeff,
+ * lift_coeff, xsph_c).  The solver is left at the end of the rollout; the
+ * rollout can be differentiated again with other seeds.  Any state change
+ * (step, set_state, set_params, ...) discards it. */
+bp_status bp_backward(bp_solver *s, const float *grad_x, const float *grad_v, float *grad_x0, float *grad_v0,
+                      float *grad_theta);
+
 /* Chooses the workgroup size of each neighbour pass among 64..512 by timing
  * `substeps_per_trial` substeps of size dt per candidate, then restores the
  * particle and rigid state it started from (results do not depend on the

@@ -15,9 +15,98 @@
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 namespace {
+
+// ---------------------------------------------------------------- dual numbers
+// Forward-mode derivative type for the FP64 tangent (or_rollout_loss): value
+// v and kDirs directional derivatives.  Comparisons and branches use the
+// value, so a Dual rollout takes exactly the branches of the double rollout
+// and its derivative is the derivative of that rollout with every branch,
+// clamp and neighbour set held where it is.
+constexpr int kDirs = 8;
+struct Dual {
+    double v = 0.0;
+    double d[kDirs] = {};
+    Dual() = default;
+    Dual(double x) : v(x) {}
+    explicit operator double() const { return v; }
+    explicit operator float() const { return (float)v; }
+};
+inline Dual operator+(const Dual &a, const Dual &b) {
+    Dual r(a.v + b.v);
+    for (int k = 0; k < kDirs; ++k) r.d[k] = a.d[k] + b.d[k];
+    return r;
+}
+inline Dual operator-(const Dual &a, const Dual &b) {
+    Dual r(a.v - b.v);
+    for (int k = 0; k < kDirs; ++k) r.d[k] = a.d[k] - b.d[k];
+    return r;
+}
+inline Dual operator-(const Dual &a) {
+    Dual r(-a.v);
+    for (int k = 0; k < kDirs; ++k) r.d[k] = -a.d[k];
+    return r;
+}
+inline Dual operator*(const Dual &a, const Dual &b) {
+    Dual r(a.v * b.v);
+    for (int k = 0; k < kDirs; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+    return r;
+}
+inline Dual operator/(const Dual &a, const Dual &b) {
+    Dual r(a.v / b.v);
+    for (int k = 0; k < kDirs; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) / b.v;
+    return r;
+}
+inline Dual &operator+=(Dual &a, const Dual &b) { return a = a + b; }
+inline Dual &operator-=(Dual &a, const Dual &b) { return a = a - b; }
+inline bool operator<(const Dual &a, const Dual &b) { return a.v < b.v; }
+inline bool operator<=(const Dual &a, const Dual &b) { return a.v <= b.v; }
+inline bool operator>(const Dual &a, const Dual &b) { return a.v > b.v; }
+inline bool operator>=(const Dual &a, const Dual &b) { return a.v >= b.v; }
+inline bool operator==(const Dual &a, const Dual &b) { return a.v == b.v; }
+inline bool operator!=(const Dual &a, const Dual &b) { return a.v != b.v; }
+inline Dual chain(const Dual &a, double val, double dfda) {
+    Dual r(val);
+    for (int k = 0; k < kDirs; ++k) r.d[k] = (a.d[k] == 0.0) ? 0.0 : dfda * a.d[k];
+    return r;
+}
+// sqrt'(0) is taken as 0 (lengths of zero vectors)
+inline Dual sqrt(const Dual &a) {
+    const double r = std::sqrt(a.v);
+    return chain(a, r, r > 0.0 ? 0.5 / r : 0.0);
+}
+inline Dual pow(const Dual &a, const Dual &b) {
+    const double val = std::pow(a.v, b.v);
+    Dual r(val);
+    for (int k = 0; k < kDirs; ++k) {
+        double t = (a.d[k] == 0.0) ? 0.0 : b.v * std::pow(a.v, b.v - 1.0) * a.d[k];
+        if (b.d[k] != 0.0) t += val * std::log(a.v) * b.d[k];
+        r.d[k] = t;
+    }
+    return r;
+}
+inline Dual fabs(const Dual &a) { return a.v < 0.0 ? -a : a; }
+inline Dual sin(const Dual &a) { return chain(a, std::sin(a.v), std::cos(a.v)); }
+inline Dual cos(const Dual &a) { return chain(a, std::cos(a.v), -std::sin(a.v)); }
+inline Dual acos(const Dual &a) { return chain(a, std::acos(a.v), -1.0 / std::sqrt(1.0 - a.v * a.v)); }
+// fminf / fmaxf: the operand selected by value (NaN loses)
+inline Dual fmin(const Dual &a, const Dual &b) { return (std::isnan(a.v) || b.v < a.v) ? b : a; }
+inline Dual fmax(const Dual &a, const Dual &b) { return (std::isnan(a.v) || b.v > a.v) ? b : a; }
+
+// math on the real type R (float, double or Dual)
+// value without derivative (identity for float / double)
+template <class R> inline R nograd(R a) { return a; }
+template <> inline Dual nograd<Dual>(Dual a) { return Dual(a.v); }
+
+template <class R> inline R sqrt_(R a) { using std::sqrt; return sqrt(a); }
+template <class R> inline R pow_(R a, R b) { using std::pow; return pow(a, b); }
+template <class R> inline R fabs_(R a) { using std::fabs; return fabs(a); }
+template <class R> inline R sin_(R a) { using std::sin; return sin(a); }
+template <class R> inline R cos_(R a) { using std::cos; return cos(a); }
+template <class R> inline R acos_(R a) { using std::acos; return acos(a); }
 
 template <class R> struct V3 {
     R x, y, z;
@@ -34,20 +123,20 @@ template <class R> inline V3<R> &operator-=(V3<R> &a, V3<R> b) { a = a - b; retu
 // helper_math.h dot/length2/length/cross semantics (:1253,1308,1296,1437)
 template <class R> inline R dot(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 template <class R> inline R length2(V3<R> a) { return dot(a, a); }
-template <class R> inline R length(V3<R> a) { return std::sqrt(dot(a, a)); }
+template <class R> inline R length(V3<R> a) { return sqrt_(dot(a, a)); }
 template <class R> inline V3<R> cross(V3<R> a, V3<R> b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 // device normalize = v * rsqrt(dot(v,v)) (helper_math.h:1326-1330); host uses 1/sqrt.
-template <class R> inline V3<R> normalize(V3<R> v) { R inv = R(1) / std::sqrt(dot(v, v)); return v * inv; }
+template <class R> inline V3<R> normalize(V3<R> v) { R inv = R(1) / sqrt_(dot(v, v)); return v * inv; }
 
 template <class R> struct Q4 {
     R x, y, z, w;
 };
 
 // fminf / fmaxf semantics for the reference's float min/max (Q11).
-template <class R> inline R fmin_(R a, R b) { return std::fmin(a, b); }
-template <class R> inline R fmax_(R a, R b) { return std::fmax(a, b); }
+template <class R> inline R fmin_(R a, R b) { using std::fmin; return fmin(a, b); }
+template <class R> inline R fmax_(R a, R b) { using std::fmax; return fmax(a, b); }
 
 // ---------------------------------------------------------------- grid hash
 // calcGridPos (grid.cuh:9-13): int3((p + 50000) / cellSize), FP32, truncation.
@@ -81,10 +170,10 @@ template <class R> struct KConst {
     R c1, c2, c3, c4, h, h2, halfh;
     void set(R hh) {
         const R pi = R(3.141592f);
-        c1 = R(315) / R(64) / pi / std::pow(hh, R(9));
-        c2 = R(-45) / pi / std::pow(hh, R(6));
-        c3 = R(32) / pi / std::pow(hh, R(9));
-        c4 = std::pow(hh, R(6)) / R(64);
+        c1 = R(315) / R(64) / pi / pow_(hh, R(9));
+        c2 = R(-45) / pi / pow_(hh, R(6));
+        c3 = R(32) / pi / pow_(hh, R(9));
+        c4 = pow_(hh, R(6)) / R(64);
         h2 = hh * hh;
         halfh = hh * R(0.5f);
         h = hh;
@@ -93,7 +182,7 @@ template <class R> struct KConst {
     inline R poly6(R r2) const { R t = h2 - r2; return c1 * t * t * t; }
     // gradientSpikyKernel (fluid.cuh:45-56)
     inline V3<R> grad(V3<R> v, R r2) const {
-        R r = std::sqrt(r2);
+        R r = sqrt_(r2);
         R t = h - r;
         R s = c2 * t * t;
         return (s * v) / r;
@@ -118,11 +207,11 @@ template <class R> inline Q4<R> qmul(Q4<R> a, Q4<R> b) {
     return r;
 }
 template <class R> inline Q4<R> angle_axis(V3<R> d, R angle) {
-    R s = std::sin(angle * R(0.5f)), c = std::cos(angle * R(0.5f));
+    R s = sin_(angle * R(0.5f)), c = cos_(angle * R(0.5f));
     return Q4<R>{d.x * s, d.y * s, d.z * s, c};
 }
 template <class R> inline Q4<R> qnormalize(Q4<R> q) {
-    R inv = R(1) / std::sqrt(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    R inv = R(1) / sqrt_(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
     return Q4<R>{q.x * inv, q.y * inv, q.z * inv, q.w * inv};
 }
 template <class R> struct M3 {
@@ -154,11 +243,11 @@ template <class R> Q4<R> extract_rotation(const V3<R> A[3], Q4<R> q, int iters) 
     for (int it = 0; it < iters; ++it) {
         M3<R> Rm = quat_to_mat(q);
         V3<R> num = cross(Rm.c[0], A[0]) + cross(Rm.c[1], A[1]) + cross(Rm.c[2], A[2]);
-        R den = R(1) / std::fabs(dot(Rm.c[0], A[0]) + dot(Rm.c[1], A[1]) + dot(Rm.c[2], A[2])) + R(1e-9f);
+        R den = R(1) / fabs_(dot(Rm.c[0], A[0]) + dot(Rm.c[1], A[1]) + dot(Rm.c[2], A[2])) + R(1e-9f);
         V3<R> om = num * den;
         R w2 = length2(om);
         if (w2 <= R(1e-9f)) break;
-        R w = std::sqrt(w2);
+        R w = sqrt_(w2);
         q = qmul(angle_axis(om / w, w), q);
         q = qnormalize(q);
     }
@@ -251,11 +340,20 @@ template <class R> struct Solver : SolverBase {
     std::vector<Q4<R>> rrot;
     std::vector<V3<R>> rcm;
     bool has_solid = false, has_fluid = false;
+    // differentiable parameters (wind vector, drag, lift, XSPH c), from P
+    R th_wind[3] = {R(0), R(0), R(0)}, th_cd = R(0), th_cl = R(0), th_xsph = R(0);
+    void theta_from_params() {
+        for (int a = 0; a < 3; ++a) th_wind[a] = R(P.wind[a]);
+        th_cd = R(P.drag_coeff);
+        th_cl = R(P.lift_coeff);
+        th_xsph = R(P.xsph_c);
+    }
 
     int n() const override { return N; }
 
     void init(const bp_scene_desc &s, const bp_params &p, int m) {
         P = p;
+        theta_from_params();
         mode = m;
         N = s.num_particles;
         for (int a = 0; a < 3; ++a) {
@@ -308,6 +406,7 @@ template <class R> struct Solver : SolverBase {
 
     void set_params(const bp_params &p) override {
         P = p;
+        theta_from_params();
         rest_density = R(p.fluid_rest_density);
     }
 
@@ -387,7 +486,7 @@ template <class R> struct Solver : SolverBase {
         const R e = R(2.7182818284f);
         const R k = R(P.mass_scaling_k);
         for (int i = 0; i < N; ++i) {
-            R scale = std::pow(e, -k * x[i].y);
+            R scale = pow_(e, -k * x[i].y);
             invw[i] = R(1) / (scale * mass[i]);
         }
     }
@@ -416,10 +515,17 @@ template <class R> struct Solver : SolverBase {
             V3<R> resolved = d * nrm + xs[i];
             V3<R> dX = resolved - pos;
             V3<R> tdx = dX - dot(dX, nrm) * nrm;
-            if (dtl < fs * dn)
+            if (dtl < fs * dn) {
                 xs[i] = resolved - tdx;
-            else
+            } else if (dtl == R(0) && !(fd * -dn > R(0))) {
+                // no tangential motion: fminf(NaN, 1) selects 1 but tdx is
+                // zero, and any tangential motion gives the factor fd (-dn) /
+                // dtl <= 0; the value is kept, the derivative is the limit's
+                V3<R> t = tdx * fmin_(fd * -dn / dtl, R(1));
+                xs[i] = resolved - V3<R>{nograd(t.x), nograd(t.y), nograd(t.z)};
+            } else {
                 xs[i] = resolved - tdx * fmin_(fd * -dn / dtl, R(1));
+            }
         }
     }
     // particleParticleCollisionConstraint (particleparticle.cuh:5-101) + Δx apply
@@ -440,7 +546,7 @@ template <class R> struct Solver : SolverBase {
                     V3<R> diff = xip - xjp;
                     R d2 = length2(diff);
                     if (d2 < r * r * R(4) && d2 > R(1e-5f)) {
-                        R dist = std::sqrt(d2);
+                        R dist = sqrt_(d2);
                         R wj = invw[j];
                         R w1 = wi / (wi + wj);
                         R w2 = wj / (wi + wj);
@@ -461,7 +567,7 @@ template <class R> struct Solver : SolverBase {
                             if (tl2 <= (fs * fs) * l2)
                                 sumF -= w1 * tan;
                             else
-                                sumF -= w1 * tan * fmin_(fd * std::sqrt(l2 / tl2), R(1));
+                                sumF -= w1 * tan * fmin_(fd * sqrt_(l2 / tl2), R(1));
                         }
                     }
                 }
@@ -511,7 +617,7 @@ template <class R> struct Solver : SolverBase {
     // fluidPosition pair term (fluid.cuh:199-221) for the pair (a,b) seen from a
     inline V3<R> pair_delta(const V3<R> &diff, R d2, R lam_sum, R wdq) const {
         R sl = lam_sum;
-        if (!P.akinci) sl += -R(P.scorr_k) * std::pow(K.poly6(d2) / wdq, R(P.scorr_n));
+        if (!P.akinci) sl += -R(P.scorr_k) * pow_(K.poly6(d2) / wdq, R(P.scorr_n));
         V3<R> delta = sl * K.grad(diff, d2);
         const R clamp = R(P.delta_clamp);
         R len = length(delta);
@@ -521,7 +627,7 @@ template <class R> struct Solver : SolverBase {
     long long missed_pairs = 0;
     void fluid_position() {
         const R inv_rho = R(1) / rest_density;
-        const R wdq = K.poly6(std::pow(R(P.scorr_dq) * K.h, R(2)));
+        const R wdq = K.poly6(pow_(R(P.scorr_dq) * K.h, R(2)));
         for (int i = 0; i < N; ++i) dx[i] = V3<R>{0, 0, 0};
         if (mode == 0) {
             for (int i = 0; i < N; ++i) {
@@ -641,7 +747,7 @@ template <class R> struct Solver : SolverBase {
             V3<R> Q1 = -Q2 - Q3 - Q4v;
             R w1 = invw[id1], w2 = invw[id2], w3 = invw[id3], w4 = invw[id4];
             R den = w1 * length2(Q1) + w2 * length2(Q2) + w3 * length2(Q3) + w4 * length2(Q4v);
-            R num = std::sqrt(R(1) - d * d) * (std::acos(d) - R(3.141592f));
+            R num = sqrt_(R(1) - d * d) * (acos_(d) - R(3.141592f));
             if (den <= R(1e-5f)) continue;
             R sc = num / den;
             dx[id1] += -w1 * Q1 * sc * kb;
@@ -679,8 +785,8 @@ template <class R> struct Solver : SolverBase {
     void wind(R dt) {
         for (int i = 0; i < N; ++i) dx[i] = V3<R>{0, 0, 0};
         const int nf = (int)faces.size() / 3;
-        const V3<R> W = mk(R(P.wind[0]), R(P.wind[1]), R(P.wind[2]));
-        const R cd = R(P.drag_coeff), cl = R(P.lift_coeff);
+        const V3<R> W = mk(th_wind[0], th_wind[1], th_wind[2]);
+        const R cd = th_cd, cl = th_cl;
         for (int f = 0; f < nf; ++f) {
             int id1 = to_new[faces[3 * f]], id2 = to_new[faces[3 * f + 1]], id3 = to_new[faces[3 * f + 2]];
             V3<R> p1 = xs[id1], p2 = xs[id2], p3 = xs[id3];
@@ -779,7 +885,7 @@ template <class R> struct Solver : SolverBase {
                     V3<R> diff = p - pj;
                     R d2 = length2(diff);
                     if (d2 <= K.h2 && d2 > R(0)) {
-                        R dist = std::sqrt(d2);
+                        R dist = sqrt_(d2);
                         V3<R> fcoh = -g * K.akinci(dist) * (p - pj) / dist;
                         V3<R> fcur = -g * (normal[i] - normal[j]);
                         R kij = R(2) * rest_density / (density[i] + density[j]);
@@ -793,7 +899,7 @@ template <class R> struct Solver : SolverBase {
     }
     // fluidXSph (fluid.cuh:338-385) -> vtmp, then swap (particlesolver.cuh:1060)
     void fluid_xsph() {
-        const R c = R(P.xsph_c);
+        const R c = th_xsph;
         for (int i = 0; i < N; ++i) {
             const int pi = phase[i];
             vtmp[i] = v[i];
@@ -933,6 +1039,56 @@ template <class R> struct Solver : SolverBase {
     }
 };
 
+// Loss of a rollout, L = sum_i |x_i(T) - target_i|^2 / norm, from the scene's
+// initial state in reference order (scatter mode).  theta (6 doubles, may be
+// null) replaces wind xyz, drag, lift, XSPH c; v0 (3N doubles, original
+// order, may be null) replaces the initial velocities.  With dirs != null
+// (FP64 tangent): dL along direction k for k < 6 the k-th parameter, and for
+// k = 6, 7 the initial velocities along vdirs[k - 6] (3N each), into dloss.
+template <class R>
+double rollout_loss(const bp_scene_desc *scene, const bp_params *params, int substeps, float subdt,
+                    const double *target, double norm, const double *theta, const double *v0, const double *vdirs,
+                    int ndirs_v, double *dloss) {
+    Solver<R> s;
+    s.init(*scene, *params, 0);
+    if (theta) {
+        for (int a = 0; a < 3; ++a) s.th_wind[a] = R(theta[a]);
+        s.th_cd = R(theta[3]);
+        s.th_cl = R(theta[4]);
+        s.th_xsph = R(theta[5]);
+    }
+    const int N = s.N;
+    if (v0)
+        for (int i = 0; i < N; ++i) s.v[i] = mk(R(v0[3 * i]), R(v0[3 * i + 1]), R(v0[3 * i + 2]));
+    if constexpr (std::is_same<R, Dual>::value) {
+        for (int a = 0; a < 3; ++a) s.th_wind[a].d[a] = 1.0;
+        s.th_cd.d[3] = 1.0;
+        s.th_cl.d[4] = 1.0;
+        s.th_xsph.d[5] = 1.0;
+        for (int k = 0; k < ndirs_v && k < kDirs - 6; ++k)
+            for (int i = 0; i < N; ++i) {  // slot i holds original id i at creation
+                s.v[i].x.d[6 + k] = vdirs[(size_t)k * 3 * N + 3 * i];
+                s.v[i].y.d[6 + k] = vdirs[(size_t)k * 3 * N + 3 * i + 1];
+                s.v[i].z.d[6 + k] = vdirs[(size_t)k * 3 * N + 3 * i + 2];
+            }
+    }
+    for (int k = 0; k < substeps; ++k) s.step(1, subdt, -1, nullptr, nullptr);
+    R L = R(0);
+    for (int i = 0; i < N; ++i) {
+        const int o = s.orig[i];
+        const V3<R> d = s.x[i] - mk(R(target[3 * o]), R(target[3 * o + 1]), R(target[3 * o + 2]));
+        L = L + dot(d, d);
+    }
+    L = L / R(norm);
+    if constexpr (std::is_same<R, Dual>::value) {
+        if (dloss)
+            for (int k = 0; k < 6 + ndirs_v && k < kDirs; ++k) dloss[k] = L.d[k];
+        return L.v;
+    } else {
+        return (double)L;
+    }
+}
+
 }  // namespace
 
 struct or_solver {
@@ -969,6 +1125,17 @@ void or_get_rigid(or_solver *s, double *q, double *cm) { s->impl->get_rigid(q, c
 void or_set_rigid(or_solver *s, const double *q) { s->impl->set_rigid(q); }
 void or_debug_fetch(or_solver *s, int which, double *out) { s->impl->fetch(which, out); }
 void or_debug_fetch_int(or_solver *s, int which, int *out) { s->impl->fetch_int(which, out); }
+
+double or_rollout_loss(const bp_scene_desc *scene, const bp_params *params, int substeps, float subdt,
+                       const double *target, double norm, const double *theta, const double *v0) {
+    return rollout_loss<double>(scene, params, substeps, subdt, target, norm, theta, v0, nullptr, 0, nullptr);
+}
+
+double or_rollout_tangent(const bp_scene_desc *scene, const bp_params *params, int substeps, float subdt,
+                          const double *target, double norm, const double *vdirs, int ndirs_v, double *dloss) {
+    if (ndirs_v > kDirs - 6) ndirs_v = kDirs - 6;
+    return rollout_loss<Dual>(scene, params, substeps, subdt, target, norm, nullptr, nullptr, vdirs, ndirs_v, dloss);
+}
 
 void or_neighbor_grid(const float *xs, int n, const float cs[3], const int dims[3], int *keys, int *perm,
                       int *cell_start, int *cell_end) {

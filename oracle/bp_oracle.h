@@ -10,7 +10,9 @@
  * commented-out projection and wind blocks (:838-858, :881-993) restored, and
  * of the kernels under brasstower/kernel/.  Templated on the real type: the
  * FP32 instance is the parity oracle, the FP64 instance is the error
- * reference.  Grid hashing is always done in FP32 on the (rounded) position,
+ * reference, and a forward-mode dual-number instance gives exact FP64
+ * derivatives of a rollout (or_rollout_tangent).  Grid hashing is always done
+ * in FP32 on the (rounded) position,
  * exactly as the reference does, so both instances see the same neighbour
  * candidate sets.
  *
@@ -52,6 +54,18 @@ void or_set_rigid(or_solver *s, const double *quats);
 void or_debug_fetch(or_solver *s, int which, double *out);
 /* which: 2 sorted keys, 3 permutation */
 void or_debug_fetch_int(or_solver *s, int which, int *out);
+
+/* Differentiable-rollout checks.  Loss L = sum_i |x_i(T) - target_i|^2 / norm
+ * after `substeps` substeps of `subdt` from the scene's initial state (FP64,
+ * reference order).  or_rollout_loss: theta (6: wind xyz, drag, lift, xsph_c;
+ * NULL = params) and v0 (3N, original order; NULL = scene) override the
+ * inputs, for central finite differences.  or_rollout_tangent: the FP64
+ * derivative by forward-mode dual numbers -- dloss[k] = dL/dtheta_k for
+ * k < 6, and dloss[6 + j] = dL/dv0 . vdirs[j] for j < ndirs_v <= 2. */
+double or_rollout_loss(const bp_scene_desc *scene, const bp_params *params, int substeps, float subdt,
+                       const double *target, double norm, const double *theta, const double *v0);
+double or_rollout_tangent(const bp_scene_desc *scene, const bp_params *params, int substeps, float subdt,
+                          const double *target, double norm, const double *vdirs, int ndirs_v, double *dloss);
 
 /* updateGrid stand-alone (grid.cuh:30-99, particlesolver.cuh:712-739). */
 void or_neighbor_grid(const float *xs, int n, const float cell_size[3],

@@ -41,6 +41,9 @@ def load():
             "or_grad_spiky": (None, [_D, C.c_double, C.c_double, C.c_int, _D]),
             "or_akinci_c": (C.c_double, [C.c_double, C.c_double, C.c_int]),
             "or_extract_rotation": (None, [_D, _D, C.c_int, C.c_int]),
+            "or_rollout_loss": (C.c_double, [C.c_void_p, C.c_void_p, C.c_int, C.c_float, _D, C.c_double, _D, _D]),
+            "or_rollout_tangent": (C.c_double, [C.c_void_p, C.c_void_p, C.c_int, C.c_float, _D, C.c_double, _D,
+                                                C.c_int, _D]),
         }
         for k, (r, a) in sig.items():
             f = getattr(lib, k)
@@ -172,3 +175,25 @@ def extract_rotation(A_cols, q, iterations=20, fp64=True):
     qq = np.ascontiguousarray(np.asarray(q, np.float64).copy())
     load().or_extract_rotation(_d(A), _d(qq), int(iterations), int(fp64))
     return qq
+
+
+def rollout_loss(scene_arrays, params, substeps, subdt, target, norm, theta=None, v0=None):
+    """FP64 loss sum |x_T - target|^2 / norm of a reference-order rollout;
+    theta (6) / v0 (N x 3, original order) override the inputs."""
+    t = np.ascontiguousarray(target, np.float64)
+    th = None if theta is None else np.ascontiguousarray(theta, np.float64)
+    v = None if v0 is None else np.ascontiguousarray(v0, np.float64)
+    return load().or_rollout_loss(C.byref(scene_arrays.desc), C.byref(params), int(substeps), float(subdt), _d(t),
+                                  float(norm), _d(th), _d(v))
+
+
+def rollout_tangent(scene_arrays, params, substeps, subdt, target, norm, vdirs=()):
+    """FP64 forward-mode derivative of rollout_loss: returns (L, dL) with
+    dL[0:6] = dL/d(wind xyz, drag, lift, xsph_c) and dL[6 + k] = dL/dv0 . vdirs[k]."""
+    t = np.ascontiguousarray(target, np.float64)
+    nd = len(vdirs)
+    vd = np.ascontiguousarray(np.stack(vdirs), np.float64) if nd else None
+    out = np.zeros(8, np.float64)
+    L = load().or_rollout_tangent(C.byref(scene_arrays.desc), C.byref(params), int(substeps), float(subdt), _d(t),
+                                  float(norm), _d(vd), nd, _d(out))
+    return L, out[:6 + nd]

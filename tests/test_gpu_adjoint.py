@@ -1,0 +1,133 @@
+"""GPU: the hand-written adjoint (bp_forward_checkpointed / bp_backward)
+against the FP64 host derivative of the same rollout.
+
+The host derivative is the oracle's reference-order FP64 rollout evaluated in
+forward-mode dual numbers (oracle/bp_oracle.cpp, or_rollout_tangent), which
+tests/test_oracle_adjoint.py pins against central finite differences.  The
+GPU rollout is FP32; on a small scene over 8-16 substeps it takes the same
+branches and neighbour sets as the FP64 one, so its gradient must match the
+FP64 derivative to ADJ_TOL relative error: every parameter component
+(wind xyz, drag, lift, XSPH c) and the initial velocities along random
+directions."""
+import numpy as np
+import pytest
+
+import brasspan as bp
+import oracle_py as orc
+import scenes
+
+pytestmark = pytest.mark.gpu
+
+ADJ_TOL = 1e-3   # relative error of each gradient component vs the FP64 derivative
+DT = scenes.ADJOINT_SUBDT
+
+
+def case(msg):
+    print(f"[gpu case] {msg}", flush=True)
+
+
+def small(num=1, seed=5, fluid=(8, 8, 8), cloth=12):
+    return scenes.adjoint_scenes(num, seed=seed, fluid_dims=fluid, cloth_n=cloth, box=3.2, grid=(32, 32, 32))
+
+
+def seeded_target(sc, seed):
+    rng = np.random.default_rng(seed)
+    x0 = np.asarray(sc.positions, np.float64).reshape(-1, 3)
+    return x0 + rng.normal(0.0, 0.05, x0.shape)
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-30)
+
+
+def gpu_gradient(solver, target, norm, substeps, every):
+    solver.forward_checkpointed(substeps, DT, every)
+    xT, _ = solver.state_original_order()
+    gx = 2.0 * (xT.astype(np.float64) - target) / norm
+    L = float(np.sum((xT.astype(np.float64) - target) ** 2) / norm)
+    gx0, gv0, gth = solver.backward(gx.astype(np.float32))
+    return L, gx0, gv0, gth
+
+
+@pytest.mark.parametrize("substeps,every", [(8, 4), (16, 8)])
+def test_adjoint_matches_fp64_derivative(substeps, every):
+    scs, prms = small()
+    sc, prm = scs[0], prms[0]
+    n = sc.num_particles()
+    target = seeded_target(sc, 1)
+    case(f"adjoint n={n} substeps={substeps} every={every}")
+    s = bp.Solver(sc, prm)
+    L, gx0, gv0, gth = gpu_gradient(s, target, n, substeps, every)
+    rng = np.random.default_rng(7)
+    dirs = [rng.normal(0, 1, (n, 3)) for _ in range(2)]
+    Lh, dL = orc.rollout_tangent(bp.SceneArrays(sc), prm, substeps, DT, target, n, dirs)
+    print(f"loss gpu {L:.9e} fp64 {Lh:.9e}")
+    names = ["wind_x", "wind_y", "wind_z", "drag", "lift", "xsph_c"]
+    worst = 0.0
+    for k in range(6):
+        e = rel(float(gth[0, k]), dL[k])
+        worst = max(worst, e)
+        print(f"  d/d{names[k]}: gpu {gth[0, k]:.6e} fp64 {dL[k]:.6e} rel {e:.2e}")
+    for j, d in enumerate(dirs):
+        g = float(np.sum(gv0.astype(np.float64) * d))
+        e = rel(g, dL[6 + j])
+        worst = max(worst, e)
+        print(f"  dL/dv0 . u{j}: gpu {g:.6e} fp64 {dL[6 + j]:.6e} rel {e:.2e}")
+    assert rel(L, Lh) < 1e-4
+    assert worst <= ADJ_TOL, f"worst relative gradient error {worst:.3e}"
+
+
+def test_adjoint_batched_per_scene():
+    scs, prms = small(num=3, seed=9, fluid=(6, 6, 6), cloth=10)
+    ntot = sum(sc.num_particles() for sc in scs)
+    targets = [seeded_target(sc, 2 + k) for k, sc in enumerate(scs)]
+    case(f"batched adjoint scenes=3 n={ntot}")
+    bs = bp.BatchSolver(scs, prms)
+    target = np.concatenate(targets)
+    L, gx0, gv0, gth = gpu_gradient(bs, target, ntot, 8, 4)
+    for k, (sc, prm) in enumerate(zip(scs, prms)):
+        Lh, dL = orc.rollout_tangent(bp.SceneArrays(sc), prm, 8, DT, targets[k], ntot, [])
+        for c in range(6):
+            e = rel(float(gth[k, c]), dL[c])
+            print(f"  scene {k} theta {c}: gpu {gth[k, c]:.6e} fp64 {dL[c]:.6e} rel {e:.2e}")
+            assert e <= ADJ_TOL
+
+
+def test_adjoint_rollout_is_the_plain_rollout_and_repeatable():
+    scs, prms = small()
+    sc, prm = scs[0], prms[0]
+    n = sc.num_particles()
+    target = seeded_target(sc, 3)
+    case("rollout equals plain steps; backward repeatable; state restored")
+    a, b = bp.Solver(sc, prm), bp.Solver(sc, prm)
+    a.forward_checkpointed(8, DT, 4)
+    for _ in range(8):
+        b.step(1, DT)
+    pa, va = a.state_original_order()
+    pb, vb = b.state_original_order()
+    assert np.array_equal(pa, pb) and np.array_equal(va, vb)
+    gx = (2.0 * (pa - target) / n).astype(np.float32)
+    r1 = a.backward(gx)
+    r2 = a.backward(gx)
+    for u, v in zip(r1, r2):
+        assert np.array_equal(u, v)
+    pa2, va2 = a.state_original_order()
+    assert np.array_equal(pa, pa2) and np.array_equal(va, va2)
+    # the adjoint is linear in the seed
+    r3 = a.backward(2.0 * gx)
+    np.testing.assert_allclose(r3[2], 2.0 * r1[2], rtol=1e-5, atol=1e-12)
+    # any state change discards the rollout
+    a.step(1, DT)
+    with pytest.raises(bp.BrassPanError):
+        a.backward(gx)
+
+
+def test_adjoint_refuses_unsupported_scenes():
+    sc = scenes.Scene(particle_radius=0.025, fluid_kernel_radius=0.1, cell_size=0.1, grid_dims=(32, 32, 32))
+    sc.add_rigid_box((4, 4, 4), (1.0, 1.0, 1.0), 0.05, 1.0)
+    sc.add_box_planes((0, 0, 0), (3.2, 3.2, 3.2))
+    s = bp.Solver(sc, bp.default_params(akinci=0, vorticity_eps=0.0))
+    case("adjoint refuses rigid bodies")
+    with pytest.raises(bp.BrassPanError) as e:
+        s.forward_checkpointed(4, DT, 2)
+    assert e.value.status == 6  # BP_ERR_UNSUPPORTED
