@@ -6,7 +6,8 @@ line.  The default workload is the 282,624-particle mixed-phase scene (fluid +
 64 shape-matched boxes + 128x128 cloth under wind); the same line also carries
 the 4,194,304-particle large block, the HBM-roofline case, and the 512-scene
 batch (2k..64k particles per scene) against the plain solver looped over the
-same scenes.
+same scenes, and the adjoint/forward time ratio of the 64-scene
+differentiable rollout (32 substeps, checkpoints every 8).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
@@ -192,6 +193,75 @@ def time_batched(frames=3, warmup=1, num=512):
     }
 
 
+def time_adjoint(num=64):
+    """The differentiable workload: `num` scenes of 16,384 fluid particles and
+    a 64 x 64 cloth (scenes.adjoint_scenes, seed 5), 32 substeps of 1/120 s,
+    checkpoints every 8.  Loss = mean |x_T - x_target|^2 with x_target from a
+    seeded forward run (wind and XSPH c changed, v0 perturbed).  Times the
+    checkpointed forward and bp_backward (replay + adjoint) on the GPU."""
+    scs, prms = scenes.adjoint_scenes(num)
+    K, dt, every = scenes.ADJOINT_SUBSTEPS, scenes.ADJOINT_SUBDT, scenes.ADJOINT_EVERY
+    n = sum(sc.num_particles() for sc in scs)
+    rng = np.random.default_rng(5)
+    # target: seeded forward run with other parameters and initial velocities
+    tprms = []
+    for p in prms:
+        q = bp.bp_params.from_buffer_copy(p)
+        q.wind[0] += 1.0
+        q.wind[2] -= 0.5
+        q.xsph_c *= 1.5
+        tprms.append(q)
+    ts = bp.BatchSolver(scs, tprms)
+    pos, vel, mp = ts.get_state()
+    ts.set_state(pos, rng.normal(0.0, 0.05, vel.shape).astype(np.float32), mp)
+    for _ in range(K):
+        ts.step(1, dt)
+    target = ts.state_original_order()[0].astype(np.float64)
+    ts.close()
+
+    s = bp.BatchSolver(scs, prms)
+    tune = s.autotune(dt, 2)
+    pos0, vel0, mp0 = s.get_state()
+
+    def run_once():
+        s.set_state(pos0, vel0, mp0)
+        s.sync()
+        t0 = time.perf_counter()
+        s.forward_checkpointed(K, dt, every)
+        s.sync()
+        t_f = time.perf_counter() - t0
+        xT = s.state_original_order()[0].astype(np.float64)
+        gx = (2.0 * (xT - target) / n).astype(np.float32)
+        t0 = time.perf_counter()
+        gx0, gv0, gth = s.backward(gx)
+        t_b = time.perf_counter() - t0
+        return t_f, t_b, float(np.sum((xT - target) ** 2) / n), gth, gv0
+
+    run_once()  # warm-up (allocates checkpoints and tape)
+    t_f, t_b, loss, gth, gv0 = run_once()
+    # the plain solver over the same substeps (graphs on), for reference
+    s.set_state(pos0, vel0, mp0)
+    s.set_graph(True)
+    s.step(2, 2 * dt)
+    s.set_state(pos0, vel0, mp0)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(K, K * dt)
+    s.sync()
+    t_plain = time.perf_counter() - t0
+    s.close()
+    return {
+        "adjoint_over_forward": round(t_b / t_f, 3), "forward_ms": round(t_f * 1e3, 2),
+        "backward_ms": round(t_b * 1e3, 2), "plain_forward_ms": round(t_plain * 1e3, 2),
+        "adjoint_over_plain_forward": round(t_b / t_plain, 3),
+        "particle_substeps_per_s_backward": round(n * K / t_b, 1), "loss": loss,
+        "grad_theta_scene0": [float(x) for x in gth[0]], "grad_v0_norm": float(np.linalg.norm(gv0)),
+        "config": {"workload": f"adjoint_{num}_scenes", "scenes": num, "particles": n, "substeps": K,
+                   "substep_dt": dt, "checkpoint_every": every, "iterations": prms[0].iterations,
+                   "block_sizes": tune},
+    }
+
+
 def copy_calibration(mib):
     """A device-to-device copy of `mib` MiB through the HIP runtime: the known
     traffic (mib read + mib written) that scales rocprof FETCH_SIZE and
@@ -219,6 +289,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-large", action="store_true", help="skip the 4M large-block measurement")
     ap.add_argument("--no-batched", action="store_true", help="skip the 512-scene batched measurement")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device (rocprof counter calibration)")
     args = ap.parse_args()
@@ -236,7 +307,7 @@ def main():
     roof, fam_times = kernel_roofline(s, subdt, workload=args.workload)
     s.close()
     out = {
-        "metric": "particle-substeps/s (4 iters) plain vs 512-scene batched", "value": round(value, 1), "unit": "particle-substeps/s",
+        "metric": "particle-substeps/s (4 iters) plain vs 512-scene batched; adjoint/forward time ratio", "value": round(value, 1), "unit": "particle-substeps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded splitmix64 scene, no checkpoint)",
@@ -261,6 +332,8 @@ def main():
         ls.close()
     if not args.no_batched:
         out["batched_512"] = time_batched()
+    if not args.no_adjoint:
+        out["adjoint_64"] = time_adjoint()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, prm, subdt)
     print(json.dumps(out), flush=True)

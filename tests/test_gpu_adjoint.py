@@ -131,3 +131,74 @@ def test_adjoint_refuses_unsupported_scenes():
     with pytest.raises(bp.BrassPanError) as e:
         s.forward_checkpointed(4, DT, 2)
     assert e.value.status == 6  # BP_ERR_UNSUPPORTED
+
+
+def test_torch_harness_gradients():
+    import torch
+
+    from autograd import Rollout
+
+    scs, prms = small(num=2, seed=21, fluid=(6, 6, 6), cloth=10)
+    ntot = sum(sc.num_particles() for sc in scs)
+    targets = [seeded_target(sc, 30 + k) for k, sc in enumerate(scs)]
+    th0 = np.array([[p.wind[0], p.wind[1], p.wind[2], p.drag_coeff, p.lift_coeff, p.xsph_c] for p in prms],
+                   np.float32)
+    theta = torch.tensor(th0, requires_grad=True)
+    v0 = torch.zeros(ntot, 3, requires_grad=True)
+    case(f"torch harness scenes=2 n={ntot}")
+    xT = Rollout.apply(theta, v0, scs, prms, 8, DT, 4)
+    loss = ((xT.double() - torch.from_numpy(np.concatenate(targets))) ** 2).sum() / ntot
+    loss.backward()
+    rng = np.random.default_rng(11)
+    off = 0
+    for k, (sc, prm) in enumerate(zip(scs, prms)):
+        n = sc.num_particles()
+        u = rng.normal(0, 1, (n, 3))
+        _, dL = orc.rollout_tangent(bp.SceneArrays(sc), prm, 8, DT, targets[k], ntot, [u])
+        for c in range(6):
+            assert rel(float(theta.grad[k, c]), dL[c]) <= ADJ_TOL
+        g = float(np.sum(v0.grad[off:off + n].numpy().astype(np.float64) * u))
+        assert rel(g, dL[6]) <= ADJ_TOL
+        off += n
+
+
+def test_full_size_adjoint_against_gpu_differences():
+    """The differentiable workload at full size (64 scenes of 16,384 fluid
+    particles + a 64 x 64 cloth, 32 substeps, checkpoints every 8): scene 0's
+    wind and drag gradients against central differences of the GPU's own FP32
+    rollout.  The cloth does not touch the fluid, so these parameters only
+    move the cloth, whose rollout is smooth; FP32 rounding of the differences
+    bounds the agreement (FD_TOL)."""
+    FD_TOL = 2e-2
+    scs, prms = scenes.adjoint_scenes(64)
+    n = sum(sc.num_particles() for sc in scs)
+    K, every = scenes.ADJOINT_SUBSTEPS, scenes.ADJOINT_EVERY
+    rng = np.random.default_rng(4)
+    target = np.concatenate([np.asarray(sc.positions, np.float64).reshape(-1, 3) for sc in scs])
+    target += rng.normal(0.0, 0.05, target.shape)
+    case(f"full-size adjoint n={n}")
+    s = bp.BatchSolver(scs, prms)
+    L, gx0, gv0, gth = gpu_gradient(s, target, n, K, every)
+    s.close()
+    assert np.all(np.isfinite(gv0)) and np.all(np.isfinite(gth))
+
+    def loss_with(k, eps):
+        p2 = list(prms)
+        q = bp.bp_params.from_buffer_copy(prms[0])
+        if k < 3:
+            q.wind[k] += eps
+        else:
+            q.drag_coeff += eps
+        p2[0] = q
+        t = bp.BatchSolver(scs, p2)
+        for _ in range(K):
+            t.step(1, DT)
+        x = t.state_original_order()[0].astype(np.float64)
+        t.close()
+        return float(np.sum((x - target) ** 2) / n)
+
+    for k, eps in [(0, 0.05), (2, 0.05), (3, 0.5)]:
+        fd = (loss_with(k, eps) - loss_with(k, -eps)) / (2 * eps)
+        e = rel(float(gth[0, k]), fd)
+        print(f"  scene 0 theta {k}: adjoint {gth[0, k]:.6e} fd {fd:.6e} rel {e:.2e}")
+        assert e <= FD_TOL
