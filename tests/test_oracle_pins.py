@@ -237,6 +237,31 @@ def test_xsph_uniform_velocity_unchanged(have_oracle):
     assert np.abs(v1 - v).max() < 1e-8  # (x+v dt - x)/dt with an FP32 1/dt only
 
 
+def test_akinci_normal_direction_and_tension_momentum(have_oracle):
+    """Akinci surface tension (fluid.cuh:387-489).  Normal: n_i = h sum_j
+    (1/rho_j) gradW(x_i - x_j), and gradW(v) is a negative multiple of v, so a
+    particle of the top face of a block (all neighbours level or below) has a
+    normal pointing down and the bottom face one pointing up.  Tension: k_ij is
+    symmetric and both the cohesion term C(r)(x_i - x_j)/r and the curvature
+    term n_i - n_j change sign under i <-> j, so on a block at rest (no
+    pressure correction, no gravity: the pair sets stay symmetric) the forces
+    cancel in pairs and the total momentum does not change."""
+    sc = lattice_scene(n=6, jitter=0.002)
+    dt = 1.0 / 120.0
+    prm = quiet_params(akinci=1, tension_gamma=0.5, lambda_denominator=1e30, scorr_k=0.0)
+    o, x0, v0 = run_one(sc, prm, dt=dt)
+    _, v1 = o.state_original_order()
+    dv = v1 - v0
+    assert np.abs(dv).max() > 1e-4  # the tension acts
+    dp = (sc.masses[:, None] * dv).sum(0)
+    assert np.abs(dp).max() < 1e-12 * (sc.masses[:, None] * np.abs(dv)).sum()
+    nrm = o.debug(5)
+    _, _, mp = o.get_state()
+    y = sc.positions[mp][:, 1]
+    top, bottom = y > y.max() - 0.01, y < y.min() + 0.01
+    assert np.all(nrm[top][:, 1] < 0.0) and np.all(nrm[bottom][:, 1] > 0.0)
+
+
 def test_vorticity_rigid_rotation_closed_form(have_oracle):
     h = 0.1
     sc = lattice_scene(n=7, jitter=0.0)
