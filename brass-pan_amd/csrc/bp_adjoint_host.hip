@@ -216,7 +216,7 @@ bp_status ensure(bp_solver *s, int total, int every) {
             Adjoint::Slot &T = A.tape.back();
             if ((r = A.alloc(&T.x_in, N)) != BP_OK || (r = A.alloc(&T.v_in, N)) != BP_OK ||
                 (r = A.alloc(&T.x1, N)) != BP_OK || (r = A.alloc(&T.xsw, N)) != BP_OK ||
-                (r = A.alloc(&T.xf, N)) != BP_OK || (r = A.alloc(&T.keys, N)) != BP_OK ||
+                (r = A.alloc(&T.xf, (size_t)N + kWalkPad)) != BP_OK || (r = A.alloc(&T.keys, N)) != BP_OK ||
                 (r = A.alloc(&T.perm, N)) != BP_OK || (r = A.alloc(&T.orig, N)) != BP_OK ||
                 (r = A.alloc(&T.m, N)) != BP_OK)
                 return r;
@@ -226,7 +226,7 @@ bp_status ensure(bp_solver *s, int total, int every) {
             T.xd.assign(iters, nullptr);
             T.lam.assign(iters, nullptr);
             for (int it = 0; it < iters; ++it)
-                if ((r = A.alloc(&T.xp[it], N)) != BP_OK || (r = A.alloc(&T.xl[it], N)) != BP_OK ||
+                if ((r = A.alloc(&T.xp[it], N)) != BP_OK || (r = A.alloc(&T.xl[it], (size_t)N + kWalkPad)) != BP_OK ||
                     (r = A.alloc(&T.xd[it], N)) != BP_OK || (r = A.alloc(&T.lam[it], N)) != BP_OK)
                     return r;
         }

@@ -237,6 +237,12 @@ __device__ __forceinline__ void for_each_run(const GridDesc &G, const int2 *__re
 // then `body(j, b)` runs only for the picked candidates, in ascending j.  The
 // expensive pair math therefore executes max(picked) times per wave instead
 // of once per candidate, and the visiting order is unchanged.
+//
+// Candidates are loaded in batches of 8 from one base address with immediate
+// offsets.  A batch may read up to 7 slots past the end of its run, so every
+// x* buffer a walk reads holds kWalkPad slots beyond the last particle; the
+// bits of those slots are cleared once per chunk.
+constexpr int kWalkPad = 8;
 template <class Pick, class Body>
 __device__ __forceinline__ void for_each_neighbor(const GridDesc &G, const int2 *__restrict__ ranges,
                                                   const int g[3], const float4 *__restrict__ xs, int jmin,
@@ -246,21 +252,15 @@ __device__ __forceinline__ void for_each_neighbor(const GridDesc &G, const int2 
         for (int base = s; base < e; base += 32) {
             const int cnt = (e - base) < 32 ? (e - base) : 32;
             uint32_t mask = 0u;
-            // batches of 8 independent loads in flight; indices past the run
-            // end are clamped to its last slot and masked out
             for (int t0 = 0; t0 < cnt; t0 += 8) {
+                const float4 *__restrict__ p = xs + (base + t0);
                 float4 bb[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int j = base + t0 + u;
-                    bb[u] = xs[j < e ? j : e - 1];
-                }
+                for (int u = 0; u < 8; ++u) bb[u] = p[u];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const bool ok = (t0 + u < cnt) & pick(base + t0 + u, bb[u]);
-                    mask |= (uint32_t)ok << (t0 + u);
-                }
+                for (int u = 0; u < 8; ++u) mask |= (uint32_t)pick(base + t0 + u, bb[u]) << (t0 + u);
             }
+            mask &= (cnt >= 32) ? 0xffffffffu : ((1u << cnt) - 1u);
             while (mask) {
                 const int t = __ffs(mask) - 1;
                 mask &= mask - 1u;

@@ -501,7 +501,7 @@ bp_status create_impl(const bp_scene_desc *scs, const bp_params *params, int ns,
     s->scene_of_orig = hsid;
 
     for (int k = 0; k < 2; ++k) {
-        TRY(s->alloc(&s->xs.b[k], N));
+        TRY(s->alloc(&s->xs.b[k], (size_t)N + kWalkPad));  // neighbour walks read past the last slot
         TRY(s->alloc(&s->x.b[k], N));
         TRY(s->alloc(&s->v.b[k], N));
         TRY(s->alloc(&s->m.b[k], N));
