@@ -134,7 +134,7 @@ def hip_stream():
     return h.value
 
 
-def time_batched(frames=3, warmup=1, num=512):
+def time_batched(frames=3, warmup=1, num=512, graphs=True):
     """The 512-scene batched workload (2 substeps per frame of 1/60 s) in one
     bp_batch_create handle with graphs and tuned block sizes, against the
     plain solver looped over the same scenes on one stream, eager and with
@@ -144,7 +144,7 @@ def time_batched(frames=3, warmup=1, num=512):
     n = sum(sc.num_particles() for sc in scs)
     bs = bp.BatchSolver(scs, prms)
     tune = bs.autotune(dt / sub, 2)
-    bs.set_graph(True)
+    bs.set_graph(graphs)
     for _ in range(warmup):
         bs.step(sub, dt)
     bs.sync()
@@ -172,7 +172,7 @@ def time_batched(frames=3, warmup=1, num=512):
     wall_e = time.perf_counter() - t0
     same = all(np.array_equal(bpos[f:f + c], s.state_original_order()[0]) for (f, c), s in zip(ranges, loop))
     for s in loop:
-        s.set_graph(True)
+        s.set_graph(graphs)
     run_loop(warmup)
     t0 = time.perf_counter()
     run_loop(frames)
@@ -193,7 +193,7 @@ def time_batched(frames=3, warmup=1, num=512):
     }
 
 
-def time_adjoint(num=64):
+def time_adjoint(num=64, graphs=True):
     """The differentiable workload: `num` scenes of 16,384 fluid particles and
     a 64 x 64 cloth (scenes.adjoint_scenes, seed 5), 32 substeps of 1/120 s,
     checkpoints every 8.  Loss = mean |x_T - x_target|^2 with x_target from a
@@ -241,7 +241,7 @@ def time_adjoint(num=64):
     t_f, t_b, loss, gth, gv0 = run_once()
     # the plain solver over the same substeps (graphs on), for reference
     s.set_state(pos0, vel0, mp0)
-    s.set_graph(True)
+    s.set_graph(graphs)
     s.step(2, 2 * dt)
     s.set_state(pos0, vel0, mp0)
     s.sync()
@@ -290,6 +290,10 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the 4M large-block measurement")
     ap.add_argument("--no-batched", action="store_true", help="skip the 512-scene batched measurement")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every kernel (no hipGraph replay; for kernel-trace profiling)")
+    ap.add_argument("--leg", choices=("all", "batched", "adjoint"), default="all",
+                    help="batched / adjoint: run only that measurement and print its object (for profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device (rocprof counter calibration)")
     args = ap.parse_args()
@@ -300,6 +304,12 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+    if args.leg == "batched":
+        print(json.dumps({"batched_512": time_batched(graphs=not args.no_graph)}), flush=True)
+        return
+    if args.leg == "adjoint":
+        print(json.dumps({"adjoint_64": time_adjoint(graphs=not args.no_graph)}), flush=True)
+        return
     sc, prm, subdt = build_workload(args.workload)
     s, wall = time_plain(sc, prm, subdt, args.steps, args.warmup)
     n = s.n
@@ -331,9 +341,9 @@ def main():
                                  "warmup": lwarm, "roofline": lroof, "kernel_ms": ltimes}
         ls.close()
     if not args.no_batched:
-        out["batched_512"] = time_batched()
+        out["batched_512"] = time_batched(graphs=not args.no_graph)
     if not args.no_adjoint:
-        out["adjoint_64"] = time_adjoint()
+        out["adjoint_64"] = time_adjoint(graphs=not args.no_graph)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, prm, subdt)
     print(json.dumps(out), flush=True)

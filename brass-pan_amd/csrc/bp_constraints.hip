@@ -23,13 +23,45 @@ __global__ __launch_bounds__(kBlock) void k_plane_collide(float4 *__restrict__ x
     xs[i] = f4(out, a.w);
 }
 
+// Cells holding a solid this substep: cell_stamp[cell] == *stamp.  The stamp
+// advances once per substep (k_stamp_next), so no table needs clearing.
+__global__ void k_stamp_next(uint32_t *__restrict__ stamp) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *stamp = *stamp + 1u;
+}
+__global__ __launch_bounds__(kBlock) void k_mark_solid_cells(const int *__restrict__ sorted_keys,
+                                                             const int *__restrict__ ph, int n,
+                                                             const uint32_t *__restrict__ stamp,
+                                                             uint32_t *__restrict__ cell_stamp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || ph[i] <= 0) return;
+    cell_stamp[sorted_keys[i]] = *stamp;
+}
+
+// True iff one of the 27 cells around g holds a solid this substep.
+__device__ __forceinline__ bool solid_near(const GridDesc &G, const int g[3], const uint32_t *__restrict__ cell_stamp,
+                                           uint32_t s) {
+    bool any = false;
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+            const uint32_t col = G.base + ((uint32_t)(g[0] + dx) & G.mask[0]) * G.stride_x +
+                                 ((uint32_t)(g[1] + dy) & G.mask[1]) * (uint32_t)G.dims[2];
+#pragma unroll
+            for (int dz = -1; dz <= 1; ++dz) any |= cell_stamp[col + ((uint32_t)(g[2] + dz) & G.mask[2])] == s;
+        }
+    return any;
+}
+
 // particleParticleCollisionConstraint (particleparticle.cuh:5-101) + Δx apply.
+// A fluid particle can only touch solids (the pair needs a positive phase),
+// so it skips the walk when none of its 27 cells holds one.
 template <class SP>
 __global__ __launch_bounds__(kMaxBlock) void k_particle_particle(const float4 *__restrict__ xs,
                                                                  const float4 *__restrict__ x,
                                                                  const int2 *__restrict__ ranges,
                                                                  float4 *__restrict__ xs_out, int n, SP sp,
-                                                                 int *__restrict__ contacts) {
+                                                                 int *__restrict__ contacts,
+                                                                 const uint32_t *__restrict__ cell_stamp,
+                                                                 const uint32_t *__restrict__ stamp) {
     const int i = xcd_index();
     if (i >= n) return;
     const SceneConst &S = sp.at(i);
@@ -44,6 +76,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_particle_particle(const float4 *_
     int cnt = 0;
     int g[3];
     G.pos(xip, g);
+    if (cell_stamp && pi <= 0 && !solid_near(G, g, cell_stamp, *stamp)) {
+        xs_out[i] = a;
+        return;
+    }
     const float r2x4 = r * r * 4.0f;
     const float thr = r * r * 0.001f * 0.001f;
     for_each_neighbor(
@@ -362,9 +398,16 @@ void launch_plane_collide(float4 *xs, const float4 *x, int n, const SP &sp, hipS
 }
 template <class SP>
 void launch_particle_particle(const float4 *xs, const float4 *x, const int2 *ranges, float4 *xs_out, int n,
-                              const SP &sp, int block, int *contacts, hipStream_t st) {
+                              const SP &sp, int block, int *contacts, const uint32_t *cell_stamp,
+                              const uint32_t *stamp, hipStream_t st) {
     hipLaunchKernelGGL(k_particle_particle<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, x, ranges,
-                       xs_out, n, sp, contacts);
+                       xs_out, n, sp, contacts, cell_stamp, stamp);
+}
+void launch_mark_solid_cells(const int *sorted_keys, const int *ph, int n, uint32_t *stamp, uint32_t *cell_stamp,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(k_stamp_next, dim3(1), dim3(64), 0, st, stamp);
+    hipLaunchKernelGGL(k_mark_solid_cells, dim3(grid_blocks(n)), dim3(kBlock), 0, st, sorted_keys, ph, n, stamp,
+                       cell_stamp);
 }
 void launch_distance(float4 *xs, const float4 *x, const int2 *pairs, const float2 *params, const int *map_to_new,
                      float4 *kbuf, int nc, const int *verts, const int *offs, const int *ents, int nv,
@@ -408,7 +451,8 @@ void launch_finish(const float4 *xs, float4 *x, float4 *v, int n, float inv_dt, 
 #define BP_INSTANTIATE_CONSTRAINTS(SP)                                                                          \
     template void launch_plane_collide<SP>(float4 *, const float4 *, int, const SP &, hipStream_t);              \
     template void launch_particle_particle<SP>(const float4 *, const float4 *, const int2 *, float4 *, int,      \
-                                               const SP &, int, int *, hipStream_t);                             \
+                                               const SP &, int, int *, const uint32_t *, const uint32_t *,       \
+                                               hipStream_t);                                                     \
     template void launch_wind<SP>(float4 *, const float4 *, const float *, const int4 *, const int *, float4 *,  \
                                   int, const SP &, float, const int *, const int *, const int *, int, hipStream_t);
 BP_INSTANTIATE_CONSTRAINTS(OneScene)

@@ -185,6 +185,7 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
     s->ph.flip();
     s->orig.flip();
     if (s->sid.b[0]) s->sid.flip();
+    if (s->cell_stamp) TIMED(F_PP, launch_mark_solid_cells(sk, s->ph.cur(), N, s->stamp, s->cell_stamp, st));
     HIP_TRY(snap(H.x1, s->x.cur(), N, st));
     HIP_TRY(snap(H.xsw, s->xs.cur(), N, st));
     HIP_TRY(snap(H.m, s->m.cur(), N, st));
@@ -204,7 +205,7 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
         planes_done = false;
         if (s->has_solid) {
             TIMED(F_PP, launch_particle_particle(s->xs.cur(), s->x.cur(), s->ranges, s->xs.alt(), N, sp(), s->blk_pp,
-                                                 H.contacts, st));
+                                                 H.contacts, s->cell_stamp, s->stamp, st));
             s->xs.flip();
         }
         if (H.xl) HIP_TRY(snap(H.xl[it], s->xs.cur(), N, st));
@@ -522,6 +523,12 @@ bp_status create_impl(const bp_scene_desc *scs, const bp_params *params, int ns,
     TRY(s->alloc(&s->corr, 3 * (size_t)N));
     TRY(s->alloc(&s->omega, N));
     TRY(s->alloc(&s->normal, N));
+    if (s->has_solid && s->has_fluid) {
+        TRY(s->alloc(&s->cell_stamp, s->num_cells));
+        TRY(s->alloc(&s->stamp, 1));
+        HTRY(hipMemsetAsync(s->cell_stamp, 0, (size_t)s->num_cells * sizeof(uint32_t), s->st));
+        HTRY(hipMemsetAsync(s->stamp, 0, sizeof(uint32_t), s->st));
+    }
     if (ns > 1) TRY(s->upload(&s->dsc, s->hsc.data(), s->hsc.size()));
     if (N > 0) {
         HTRY(hipMemcpyAsync(s->x.cur(), hx.data(), N * sizeof(float4), hipMemcpyHostToDevice, s->st));

@@ -82,6 +82,7 @@ struct bp_solver {
     float4 *sum_tmp = nullptr;
     unsigned long long *corr = nullptr;  // fixed-point pair corrections, kept zero between passes
     float4 *omega = nullptr, *normal = nullptr;
+    uint32_t *cell_stamp = nullptr, *stamp = nullptr;  // cells holding a solid (fluid + solid scenes)
 
     int nd = 0;
     int2 *dpairs = nullptr;
