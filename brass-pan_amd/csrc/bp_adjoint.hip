@@ -499,19 +499,19 @@ __global__ __launch_bounds__(kBlock) void ka_distance_gather(float4 *__restrict_
     const int slot = map_to_new[verts[t]];
     f3 g = mk3(0.f, 0.f, 0.f);
     float w = 0.0f;
-    for (int e = offs[t]; e < offs[t + 1]; ++e) {
-        const int en = ents[e];
-        const float4 c4 = cb[en >> 1];
-        if (c4.w == 0.0f) continue;
-        const float2 w2 = wb[en >> 1];
-        if (en & 1) {
-            g -= xyz(c4);
-            w += w2.y;
-        } else {
-            g += xyz(c4);
-            w += w2.x;
-        }
-    }
+    gather_entries(
+        ents, offs[t], offs[t + 1], [&](int en) { return cb[en >> 1]; },
+        [&](int en, float4 c4) {
+            if (c4.w == 0.0f) return;
+            const float2 w2 = wb[en >> 1];
+            if (en & 1) {
+                g -= xyz(c4);
+                w += w2.y;
+            } else {
+                g += xyz(c4);
+                w += w2.x;
+            }
+        });
     gxs[slot] = f4(xyz(gxs[slot]) + g, 0.f);
     gw[slot] += w;
 }
@@ -599,12 +599,12 @@ __global__ __launch_bounds__(kBlock) void ka_wind_gather(float4 *__restrict__ gx
     if (t >= nv) return;
     const int slot = map_to_new[verts[t]];
     f3 gx = mk3(0.f, 0.f, 0.f), gv = mk3(0.f, 0.f, 0.f);
-    for (int e = offs[t]; e < offs[t + 1]; ++e) {
-        const int en = ents[e];
-        const int f = en / 3, corner = en - 3 * f;
-        gx += xyz(fbar[4 * f + corner]);
-        gv += xyz(fbar[4 * f + 3]);
-    }
+    gather_entries(
+        ents, offs[t], offs[t + 1], [&](int en) { return fbar[4 * (en / 3) + (en - 3 * (en / 3))]; },
+        [&](int en, float4 p) {
+            gx += xyz(p);
+            gv += xyz(fbar[4 * (en / 3) + 3]);
+        });
     gxs[slot] = f4(xyz(gxs[slot]) + gx, 0.f);
     gv1[slot] = f4(xyz(gv1[slot]) + gv, 0.f);
 }

@@ -162,16 +162,16 @@ __global__ __launch_bounds__(kBlock) void k_distance_apply(float4 *__restrict__ 
     const int slot = map_to_new[verts[t]];
     const float w = x[slot].w;
     f3 dx = mk3(0.f, 0.f, 0.f);
-    for (int e = offs[t]; e < offs[t + 1]; ++e) {
-        const int en = ents[e];
-        const float4 k4 = kbuf[en >> 1];
-        if (k4.w == 0.0f) continue;
-        const f3 k = xyz(k4);
-        if (en & 1)
-            dx += w * k;
-        else
-            dx += -w * k;
-    }
+    gather_entries(
+        ents, offs[t], offs[t + 1], [&](int en) { return kbuf[en >> 1]; },
+        [&](int en, float4 k4) {
+            if (k4.w == 0.0f) return;
+            const f3 k = xyz(k4);
+            if (en & 1)
+                dx += w * k;
+            else
+                dx += -w * k;
+        });
     const float4 a = xs[slot];
     xs[slot] = f4(xyz(a) + dx, a.w);
 }
@@ -226,14 +226,15 @@ __global__ __launch_bounds__(kBlock) void k_gather_apply(float4 *__restrict__ xs
     const int slot = map_to_new[verts[t]];
     const float mass = divide_by_mass ? m[slot] : 1.0f;
     f3 dx = mk3(0.f, 0.f, 0.f);
-    for (int e = offs[t]; e < offs[t + 1]; ++e) {
-        const float4 b = buf[ents[e]];
-        if (b.w == 0.0f) continue;
-        if (divide_by_mass)
-            dx += xyz(b) / mass;
-        else
-            dx += xyz(b);
-    }
+    gather_entries(
+        ents, offs[t], offs[t + 1], [&](int en) { return buf[en]; },
+        [&](int, float4 b) {
+            if (b.w == 0.0f) return;
+            if (divide_by_mass)
+                dx += xyz(b) / mass;
+            else
+                dx += xyz(b);
+        });
     const float4 a = xs[slot];
     xs[slot] = f4(xyz(a) + dx, a.w);
 }

@@ -270,6 +270,24 @@ __device__ __forceinline__ void for_each_neighbor(const GridDesc &G, const int2 
     });
 }
 
+// Walks the CSR entries [e0, e1) in order, eight at a time: the entry
+// indices, then the eight records they name (`load(en)`), are fetched as
+// independent loads before `use(en, rec)` consumes them in entry order.
+template <class Load, class Use>
+__device__ __forceinline__ void gather_entries(const int *__restrict__ ents, int e0, int e1, Load &&load, Use &&use) {
+    for (int b = e0; b < e1; b += 8) {
+        int en[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) en[u] = (b + u < e1) ? ents[b + u] : ents[b];
+        decltype(load(0)) rec[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) rec[u] = load(en[u]);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (b + u < e1) use(en[u], rec[u]);
+    }
+}
+
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
 // b % 8).  Neighbour passes renumber them so that each XCD walks one
 // contiguous range of sorted slots and its private L2 holds the neighbour
