@@ -12,23 +12,14 @@
 #include "../../../include/brasspan.h"
 #include "scene.h"
 
-struct ParticleSolver {
-    // particlesolver.cuh:148-215.  `params` defaults to the reference literals;
-    // `hipStream` (a hipStream_t) defaults to the null stream.
-    explicit ParticleSolver(const std::shared_ptr<Scene> &scene, const bp_params *params = nullptr,
-                            void *hipStream = nullptr) {
-        bp_params p;
-        if (params) p = *params; else check(bp_default_params(&p));
-        const Scene &s = *scene;
-        planeO.clear();
-        planeN.clear();
+// Flat C-ABI description of a Scene (keeps the arrays it points into).
+struct SceneDesc {
+    explicit SceneDesc(const Scene &s) {
         for (const Plane &pl : s.planes) {
             planeO.insert(planeO.end(), {pl.origin.x, pl.origin.y, pl.origin.z});
             planeN.insert(planeN.end(), {pl.normal.x, pl.normal.y, pl.normal.z});
         }
-        std::vector<int> faces3;
         for (const int3 &f : s.faces) faces3.insert(faces3.end(), {f.x, f.y, f.z});
-        bp_scene_desc d{};
         d.num_particles = (int)s.positions.size();
         d.positions = reinterpret_cast<const float *>(s.positions.data());
         d.masses = s.masses.data();
@@ -54,14 +45,85 @@ struct ParticleSolver {
         d.rigid_rest_positions = reinterpret_cast<const float *>(s.rigidbodyInitialPositions.data());
         d.num_immovables = (int)s.immovableConstraints.size();
         d.immovables = s.immovableConstraints.data();
-        check(bp_solver_create(&d, &p, hipStream, &handle));
-        numParticles = (size_t)d.num_particles;
+    }
+    SceneDesc(const SceneDesc &) = delete;
+    SceneDesc &operator=(const SceneDesc &) = delete;
+    bp_scene_desc d{};
+
+  private:
+    std::vector<float> planeO, planeN;
+    std::vector<int> faces3;
+};
+
+inline void bp_check(bp_status s) {
+    if (s != BP_OK) throw std::runtime_error(std::string("brasspan: ") + bp_last_error_string());
+}
+
+// What the plain and the batched solver share: state export, graphs, the
+// block-size autotuner and the differentiable rollout (no counterparts in
+// the reference).
+struct SolverHandle {
+    SolverHandle() = default;
+    SolverHandle(const SolverHandle &) = delete;
+    SolverHandle &operator=(const SolverHandle &) = delete;
+    ~SolverHandle() { bp_solver_destroy(handle); }
+
+    // Positions and velocities in original-id order (headless export).
+    void copyStateOriginalOrder(std::vector<float3> &pos, std::vector<float3> &vel) {
+        pos.resize(numParticles);
+        vel.resize(numParticles);
+        bp_check(bp_copy_state_original_order(handle, reinterpret_cast<float *>(pos.data()),
+                                              reinterpret_cast<float *>(vel.data())));
+    }
+    void synchronize() { bp_check(bp_solver_sync(handle)); }
+    // Replays pairs of substeps from a captured hipGraph.
+    void setGraph(bool enable) { bp_check(bp_solver_set_graph(handle, enable ? 1 : 0)); }
+    // Chooses the neighbour-pass workgroup sizes; returns the JSON report.
+    std::string autotune(float subDeltaTime, int substepsPerTrial = 2) {
+        char buf[512] = {0};
+        bp_check(bp_solver_autotune(handle, subDeltaTime, substepsPerTrial, buf, (int)sizeof(buf)));
+        return std::string(buf);
+    }
+    // Differentiable rollout: `substeps` substeps of `subDeltaTime`, full
+    // state kept every `checkpointEvery` substeps.
+    void forwardCheckpointed(int substeps, float subDeltaTime, int checkpointEvery = 8) {
+        bp_check(bp_forward_checkpointed(handle, substeps, subDeltaTime, checkpointEvery));
+    }
+    // Adjoint of the last rollout.  gradX/gradV: dL/dx_T, dL/dv_T (original
+    // order; gradV may be empty).  Outputs dL/dx_0, dL/dv_0 and, per scene,
+    // dL/d(wind x, y, z, drag, lift, xsph c).
+    void backward(const std::vector<float3> &gradX, const std::vector<float3> &gradV, std::vector<float3> &gradX0,
+                  std::vector<float3> &gradV0, std::vector<float> &gradTheta) {
+        if (gradX.size() != numParticles || (!gradV.empty() && gradV.size() != numParticles))
+            throw std::invalid_argument("brasspan: gradient seeds need one float3 per particle");
+        gradX0.resize(numParticles);
+        gradV0.resize(numParticles);
+        gradTheta.resize(6 * (size_t)numScenes);
+        bp_check(bp_backward(handle, reinterpret_cast<const float *>(gradX.data()),
+                             gradV.empty() ? nullptr : reinterpret_cast<const float *>(gradV.data()),
+                             reinterpret_cast<float *>(gradX0.data()), reinterpret_cast<float *>(gradV0.data()),
+                             gradTheta.data()));
+    }
+
+    size_t numParticles = 0;
+    int numScenes = 1;
+    bp_solver *handle = nullptr;
+};
+
+struct ParticleSolver : SolverHandle {
+    // particlesolver.cuh:148-215.  `params` defaults to the reference literals;
+    // `hipStream` (a hipStream_t) defaults to the null stream.
+    explicit ParticleSolver(const std::shared_ptr<Scene> &scene, const bp_params *params = nullptr,
+                            void *hipStream = nullptr) {
+        bp_params p;
+        if (params) p = *params; else check(bp_default_params(&p));
+        const Scene &s = *scene;
+        SceneDesc d(s);
+        check(bp_solver_create(&d.d, &p, hipStream, &handle));
+        numParticles = (size_t)d.d.num_particles;
         solidRadius = s.particleRadius;
         fluidKernelRadius = s.fluidKernelRadius;
     }
-    ~ParticleSolver() { bp_solver_destroy(handle); }
-    ParticleSolver(const ParticleSolver &) = delete;
-    ParticleSolver &operator=(const ParticleSolver &) = delete;
 
     // particlesolver.cuh:772-776
     void update(const int numSubTimeStep, const float deltaTime, const int pickedOriginalParticleId = -1,
@@ -100,23 +162,35 @@ struct ParticleSolver {
         check(bp_get_device_arrays(handle, &v));
         return v;
     }
-    // Positions and velocities in original-id order (headless export).
-    void copyStateOriginalOrder(std::vector<float3> &pos, std::vector<float3> &vel) {
-        pos.resize(numParticles);
-        vel.resize(numParticles);
-        check(bp_copy_state_original_order(handle, reinterpret_cast<float *>(pos.data()),
-                                           reinterpret_cast<float *>(vel.data())));
-    }
-    void synchronize() { check(bp_solver_sync(handle)); }
 
-    size_t numParticles = 0;
     float solidRadius = 0.f;
     float fluidKernelRadius = 0.f;
-    bp_solver *handle = nullptr;
 
   private:
-    std::vector<float> planeO, planeN;
-    static void check(bp_status s) {
-        if (s != BP_OK) throw std::runtime_error(std::string("brasspan: ") + bp_last_error_string());
+    static void check(bp_status s) { bp_check(s); }
+};
+
+// Many independent scenes advanced together (bp_batch_create): one launch per
+// pass serves every scene.  No counterpart in the reference.
+struct BatchParticleSolver : SolverHandle {
+    BatchParticleSolver(const std::vector<std::shared_ptr<Scene>> &scenes, const std::vector<bp_params> &params,
+                        void *hipStream = nullptr) {
+        if (scenes.size() != params.size() || scenes.empty())
+            throw std::invalid_argument("brasspan: one bp_params per scene");
+        std::vector<std::unique_ptr<SceneDesc>> keep;
+        std::vector<bp_scene_desc> ds;
+        for (const auto &sc : scenes) {
+            keep.emplace_back(new SceneDesc(*sc));
+            ds.push_back(keep.back()->d);
+        }
+        bp_check(bp_batch_create(ds.data(), params.data(), (int)ds.size(), hipStream, &handle));
+        numScenes = (int)ds.size();
+        for (const auto &d : ds) numParticles += (size_t)d.num_particles;
     }
+    // ParticleSolver::update for all scenes (no picking).
+    void update(const int numSubTimeStep, const float deltaTime) {
+        bp_check(bp_solver_step(handle, numSubTimeStep, deltaTime, -1, nullptr, nullptr));
+    }
+    // Scene k owns original ids [first, first + count).
+    void sceneRange(int k, int &first, int &count) { bp_check(bp_batch_scene_range(handle, k, &first, &count)); }
 };

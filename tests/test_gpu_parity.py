@@ -226,3 +226,8 @@ def test_cpp_api_example_runs():
     r = subprocess.run([exe, "10"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert "particles 4096 frames 10" in r.stdout
+    # batch of three flags and the gradient of a final-state loss (BatchParticleSolver)
+    r = subprocess.run([exe.replace("dam_break", "batch_and_grad")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "batch_and_grad ok" in r.stdout, r.stdout
+    print(r.stdout)

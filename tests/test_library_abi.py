@@ -94,12 +94,13 @@ def test_create_without_device_fails_loudly():
         bp.Solver(small_scene(grid_dims=(64, 64, 64)))
 
 
-EXAMPLE = os.path.join(ROOT, "brass-pan_amd", "bin", "dam_break")
+EXAMPLES = [os.path.join(ROOT, "brass-pan_amd", "bin", name) for name in ("dam_break", "batch_and_grad")]
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK), reason="device present")
-def test_cpp_api_example_builds_and_fails_loudly_without_device():
+@pytest.mark.parametrize("exe", EXAMPLES)
+def test_cpp_api_example_builds_and_fails_loudly_without_device(exe):
     import subprocess
-    assert os.path.exists(EXAMPLE), "brass-pan_amd/bin/dam_break not built"
-    r = subprocess.run([EXAMPLE, "1"], capture_output=True, text=True, timeout=60)
+    assert os.path.exists(exe), f"{exe} not built"
+    r = subprocess.run([exe, "1"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "no HIP device" in r.stderr
