@@ -231,3 +231,39 @@ def test_cpp_api_example_runs():
     assert r.returncode == 0, r.stderr
     assert "batch_and_grad ok" in r.stdout, r.stdout
     print(r.stdout)
+
+
+def test_edge_cases_empty_single_and_one_scene_batch():
+    """An empty scene steps as a no-op (particlesolver.cuh:778); one free
+    particle follows x(T) = x0 + v0 K dt + g dt^2 K(K+1)/2 exactly as the
+    oracle computes it; a batch of one scene equals the plain solver; a batch
+    may hold an empty scene between two others."""
+    case("edge cases")
+    empty = scenes.Scene(particle_radius=0.025, fluid_kernel_radius=0.1, cell_size=0.1, grid_dims=(16, 16, 16))
+    s = bp.Solver(empty, bp.default_params())
+    s.step(2, DT)
+    assert s.n == 0
+
+    one = scenes.Scene(particle_radius=0.025, fluid_kernel_radius=0.1, cell_size=0.1, grid_dims=(16, 16, 16))
+    one.add_granulars_block((1, 1, 1), (0.8, 1.2, 0.8), 0.05, 1.0)
+    prm = bp.default_params(wind_enabled=0)
+    g = bp.Solver(one, prm)
+    o = orc.Oracle(bp.SceneArrays(one), prm, fp64=False, mode=orc.GATHER)
+    for _ in range(5):
+        g.step(1, DT)
+        o.step(1, DT)
+    gp, gv = g.state_original_order()
+    op, ov = o.state_original_order()
+    assert np.array_equal(gp, op.astype(np.float32)) and np.array_equal(gv, ov.astype(np.float32))
+
+    sc, prm2 = scenes.dam_break(dims=(8, 8, 8)), scenes.dam_break_params()
+    a, b = bp.Solver(sc, prm2), bp.BatchSolver([sc], [prm2])
+    a.step(2, DT)
+    b.step(2, DT)
+    assert np.array_equal(a.state_original_order()[0], b.state_original_order()[0])
+
+    scs, prms = scenes.batched_scenes(num=2, seed=8, lo=2048, hi=3000)
+    bs = bp.BatchSolver([scs[0], empty, scs[1]], [prms[0], prms[0], prms[1]])
+    assert bs.ranges[1] == (scs[0].num_particles(), 0)
+    bs.step(2, DT)
+    assert np.all(np.isfinite(bs.state_original_order()[0]))
