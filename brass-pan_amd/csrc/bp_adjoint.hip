@@ -229,7 +229,7 @@ __global__ __launch_bounds__(kMaxBlock) void ka_xsph(const float4 *__restrict__ 
     float wsum = 0.0f;
     for_each_neighbor(
         G, ranges, g, xf, 0,
-        [&](int j, float4 b) { return (j != i) & (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
+        [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
         [&](int j, float4 b) {
             const f3 pj = xyz(b);
             const float d2 = dist2(pi, pj);
@@ -249,7 +249,8 @@ __global__ __launch_bounds__(kMaxBlock) void ka_xsph(const float4 *__restrict__ 
                 deliver3(corr, j, 0, (c * W) * yi);
                 deliver3(corr, j, 3, -sa * diff);
             }
-        });
+        },
+        AllowNotSelf{i});
     gv2_own[i] = f4(yi - (c * wsum) * yi + gv, 0.f);
     gx_own[i] = f4(gx, 0.f);
     cacc[i] += dot(yi, vn);
@@ -315,12 +316,10 @@ __global__ __launch_bounds__(kMaxBlock) void ka_position(const float4 *__restric
     float gl = 0.0f;
     for_each_neighbor(
         G, ranges, g, xs, 0,
+        [&](int, float4 b) { return dist2(pk, xyz(b)) <= K.h2; },
         [&](int j, float4 b) {
-            const bool own = (j != k) & (kf | (j < k)) & ((j > k) | (phase_of(b) <= 0));
-            const float d2 = dist2(pk, xyz(b));
-            return own & (d2 <= K.h2) & (d2 > 0.0f);
-        },
-        [&](int j, float4 b) {
+            // coincident partners are skipped; a solid partner's pair is k's only if j > k
+            if (!(dist2(pk, xyz(b)) > 0.0f) | ((phase_of(b) > 0) & !(kf & (j > k)))) return;
             const bool partner_sees_k = G.in_block(ccell[j], bk);
             if (j < k && !partner_sees_k) return;
             const f3 diff = pk - xyz(b);
@@ -336,7 +335,8 @@ __global__ __launch_bounds__(kMaxBlock) void ka_position(const float4 *__restric
                 deliver3(corr, j, 0, -gdiff);
                 atomicAdd(&corr[kCorrStride * j + 3], afix(gls));
             }
-        });
+        },
+        [&](int base) { return kf ? ~chunk_bit(k, base) : chunk_below(k, base); });
     gx_own[k] = f4(gx, 0.f);
     glam_own[k] = gl;
 }
@@ -388,11 +388,10 @@ __global__ __launch_bounds__(kMaxBlock) void ka_lambda(const float4 *__restrict_
     f3 gx = mk3(0.f, 0.f, 0.f);
     for_each_neighbor(
         G, ranges, g, xs, 0,
+        [&](int, float4 b) { return dist2(pk, xyz(b)) <= K.h2; },
         [&](int j, float4 b) {
-            const float d2 = dist2(pk, xyz(b));
-            return (j != k) & (((j > k) & kf) | ((j < k) & (phase_of(b) <= 0))) & (d2 <= K.h2) & (d2 > 0.0f);
-        },
-        [&](int j, float4 b) {
+            // coincident partners are skipped; a solid partner's pair is k's only if j > k
+            if (!(dist2(pk, xyz(b)) > 0.0f) | ((phase_of(b) > 0) & !(kf & (j > k)))) return;
             const bool partner_sees_k = G.in_block(ccell[j], bk);
             const f3 diff = pk - xyz(b);
             const float Wp2 = 2.0f * poly6_d(K, dist2(pk, xyz(b)));
@@ -404,7 +403,8 @@ __global__ __launch_bounds__(kMaxBlock) void ka_lambda(const float4 *__restrict_
             } else if (partner_sees_k) {  // pair of rho_j
                 gx += (grho[j] * sk * mk * Wp2) * diff;
             }
-        });
+        },
+        [&](int base) { return kf ? ~chunk_bit(k, base) : chunk_below(k, base); });
     gx_own[k] = f4(gx, 0.f);
 }
 

@@ -84,10 +84,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_particle_particle(const float4 *_
     const float thr = r * r * 0.001f * 0.001f;
     for_each_neighbor(
         G, ranges, g, xs, 0,
-        [&](int j, float4 b) {
+        [&](int, float4 b) {  // pi != pj also excludes i itself
             const int pj = phase_of(b);
             const float d2 = dist2(xip, xyz(b));
-            return (i != j) & (pi != pj) & ((pi > 0) | (pj > 0)) & (d2 < r2x4) & (d2 > 1e-5f);
+            return (pi != pj) & ((pi > 0) | (pj > 0)) & (d2 < r2x4) & (d2 > 1e-5f);
         },
         [&](int j, float4 b) {
             const f3 xjp = xyz(b);

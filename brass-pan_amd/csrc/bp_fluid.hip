@@ -14,8 +14,8 @@ namespace bp {
 template <class SP>
 __global__ __launch_bounds__(kMaxBlock) void k_lambda(const float4 *__restrict__ xs, const float *__restrict__ m,
                                                    const int2 *__restrict__ ranges, float *__restrict__ lambda,
-                                                   float *__restrict__ density, uint32_t *__restrict__ ccell, int n,
-                                                   SP sp) {
+                                                   float *__restrict__ density, uint32_t *__restrict__ ccell,
+                                                   uint32_t *__restrict__ srec, int n, SP sp) {
     const int i = xcd_index();
     if (i >= n) return;
     const SceneConst &S = sp.at(i);
@@ -35,13 +35,24 @@ __global__ __launch_bounds__(kMaxBlock) void k_lambda(const float4 *__restrict__
     float dens = 0.0f, sumg2 = 0.0f;
     f3 gi = mk3(0.f, 0.f, 0.f);
     const float sds = F.solid_density_scale;
-    // Q1: only j >= i (fluid.cuh:117)
+    // Q1: only j >= i (fluid.cuh:117).  With a screen record the walk covers
+    // every j (the position pass replays it) and j < i are masked off.
     const float inv_rho0 = F.inv_rest_density;
-    for_each_neighbor(
-        G, ranges, g, xs, i, [&](int, float4 b) { return dist2(pi, xyz(b)) <= K.h2; },
-        [&](int j, float4 b) {
+    struct Partner {
+        float4 b;
+        float m;
+    };
+    ScreenRec rec;
+    rec.out = srec;
+    rec.slot = i;
+    rec.n = n;
+    for_each_neighbor_nb<kBodyBatch>(
+        G, ranges, g, xs, srec ? 0 : i, [&](int, float4 b) { return dist2(pi, xyz(b)) <= K.h2; },
+        [&](int j) { return Partner{xs[j], m[j]}; },
+        [&](int, const Partner &P) {
+            const float4 b = P.b;
             const float d2 = dist2(pi, xyz(b));
-            const float mj = m[j];
+            const float mj = P.m;
             const float ld = mj * K.poly6(d2);
             dens += (phase_of(b) < 0) ? ld : sds * ld;
             if (F.lambda_mode == 1 && d2 > 0.0f) {
@@ -52,7 +63,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_lambda(const float4 *__restrict__
                 sumg2 += gc * gc * d2;
                 gi -= gr;
             }
-        });
+        },
+        [&](int base) { return ~chunk_below(i, base); }, rec);
     if (F.lambda_mode == 1) sumg2 += dot(gi, gi);
     float C = dens / F.rest_density - 1.0f;
     if (F.akinci) C = fmaxf(C, 0.0f);
@@ -94,7 +106,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_position(const float4 *__restrict
                                                      const float *__restrict__ lambda,
                                                      const uint32_t *__restrict__ ccell,
                                                      const int *__restrict__ build_keys,
-                                                     const int2 *__restrict__ ranges, float4 *__restrict__ sum_out,
+                                                     const int2 *__restrict__ ranges,
+                                                     const uint32_t *__restrict__ srec, float4 *__restrict__ sum_out,
                                                      unsigned long long *__restrict__ corr, int n, SP sp) {
     const int k = xcd_index();
     if (k >= n) return;
@@ -112,19 +125,28 @@ __global__ __launch_bounds__(kMaxBlock) void k_position(const float4 *__restrict
     int g[3];
     unpack_cell(ck, g);
     f3 sum = mk3(0.f, 0.f, 0.f);
-    for_each_neighbor(
+    struct Partner {
+        float4 b;
+        uint32_t cell;
+        float lam, m;
+    };
+    ScreenRec rec;  // the lambda pass screened fluid particles at this x*
+    rec.in = kf ? srec : nullptr;
+    rec.slot = k;
+    rec.n = n;
+    for_each_neighbor_nb<kBodyBatch>(
         G, ranges, g, xs, 0,
-        [&](int j, float4 b) {
-            const bool own = (j != k) & (kf | (j < k)) & ((j > k) | (phase_of(b) <= 0));
-            const float d2 = dist2(pk, xyz(b));
-            return own & (d2 <= K.h2) & (d2 > 0.0f);
-        },
-        [&](int j, float4 b) {
-            const bool partner_sees_k = G.in_block(ccell[j], bk);
+        [&](int, float4 b) { return dist2(pk, xyz(b)) <= K.h2; },
+        [&](int j) { return Partner{xs[j], ccell[j], lambda[j], m[j]}; },
+        [&](int j, const Partner &P) {
+            const float4 b = P.b;
+            // coincident partners are skipped; a solid partner's pair is k's only if j > k
+            if (!(dist2(pk, xyz(b)) > 0.0f) | ((phase_of(b) > 0) & !(kf & (j > k)))) return;
+            const bool partner_sees_k = G.in_block(P.cell, bk);
             if (j < k && !partner_sees_k) return;
             const f3 diff = pk - xyz(b);
-            const float c = pair_coef(dist2(pk, xyz(b)), lk + lambda[j], K, F);
-            const float cj = c * m[j];
+            const float c = pair_coef(dist2(pk, xyz(b)), lk + P.lam, K, F);
+            const float cj = c * P.m;
             sum.x = fmaf(cj, diff.x, sum.x);
             sum.y = fmaf(cj, diff.y, sum.y);
             sum.z = fmaf(cj, diff.z, sum.z);
@@ -134,7 +156,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_position(const float4 *__restrict
                 atomicAdd(&corr[3 * j + 1], to_fixed(ck2 * diff.y));
                 atomicAdd(&corr[3 * j + 2], to_fixed(ck2 * diff.z));
             }
-        });
+        },
+        // partners j != k if k is fluid, only j < k if k is a solid
+        [&](int base) { return kf ? ~chunk_bit(k, base) : chunk_below(k, base); }, rec);
     sum_out[k] = f4(sum, 0.0f);
 }
 
@@ -182,8 +206,9 @@ template <class SP>
 __global__ __launch_bounds__(kMaxBlock) void k_post_a(const float4 *__restrict__ xs, const float4 *__restrict__ v,
                                                    const float *__restrict__ density,
                                                    const int2 *__restrict__ ranges, float4 *__restrict__ omega,
-                                                   float4 *__restrict__ normal, uint32_t *__restrict__ ccell, int n,
-                                                   SP sp, int do_omega, int do_normal) {
+                                                   float4 *__restrict__ normal, uint32_t *__restrict__ ccell,
+                                                   uint32_t *__restrict__ srec, int n, SP sp, int do_omega,
+                                                   int do_normal) {
     const int i = xcd_index();
     if (i >= n) return;
     const SceneConst &S = sp.at(i);
@@ -198,19 +223,22 @@ __global__ __launch_bounds__(kMaxBlock) void k_post_a(const float4 *__restrict__
     ccell[i] = G.pack(g);
     const f3 vi = xyz(v[i]);
     f3 om = mk3(0.f, 0.f, 0.f), nn = mk3(0.f, 0.f, 0.f);
+    ScreenRec rec;  // replayed by the vorticity/tension and XSPH passes
+    rec.out = srec;
+    rec.slot = i;
+    rec.n = n;
     for_each_neighbor(
         G, ranges, g, xs, 0,
+        [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
         [&](int j, float4 b) {
-            const float d2 = dist2(pi, xyz(b));
-            return (j != i) & (phase_of(b) == ph) & (d2 <= K.h2) & (d2 > 0.0f);
-        },
-        [&](int j, float4 b) {
+            if (!(dist2(pi, xyz(b)) > 0.0f)) return;  // coincident partners are skipped
             const f3 diff = pi - xyz(b);
             const float r = fsqrt(dist2(pi, xyz(b)));
             const f3 gr = K.grad_coef(r, frcp(r)) * diff;
             if (do_omega) om += cross(xyz(v[j]) - vi, gr);
             if (do_normal) nn += frcp(density[j]) * gr;
-        });
+        },
+        AllowNotSelf{i}, rec);
     if (do_omega) omega[i] = f4(om, fsqrt(dot(om, om)));
     if (do_normal) normal[i] = f4(K.h * nn, 0.0f);
 }
@@ -224,7 +252,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_post_b(const float4 *__restrict__
                                                    const float4 *__restrict__ normal,
                                                    const float *__restrict__ density,
                                                    const int2 *__restrict__ ranges,
-                                                   const uint32_t *__restrict__ ccell, int n, SP sp, int do_vort,
+                                                   const uint32_t *__restrict__ ccell,
+                                                   const uint32_t *__restrict__ srec, int n, SP sp, int do_vort,
                                                    int do_akinci, float dt) {
     const int i = xcd_index();
     if (i >= n) return;
@@ -242,13 +271,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_post_b(const float4 *__restrict__
     const f3 ni = do_akinci ? xyz(normal[i]) : mk3(0.f, 0.f, 0.f);
     const float rhoi = do_akinci ? density[i] : 0.0f;
     const float gam = F.tension_gamma;
+    ScreenRec rec;  // the omega/normal pass screened the same positions
+    rec.in = srec;
+    rec.slot = i;
+    rec.n = n;
     for_each_neighbor(
         G, ranges, g, xs, 0,
+        [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
         [&](int j, float4 b) {
-            const float d2 = dist2(pi, xyz(b));
-            return (j != i) & (phase_of(b) == ph) & (d2 <= K.h2) & (d2 > 0.0f);
-        },
-        [&](int j, float4 b) {
+            if (!(dist2(pi, xyz(b)) > 0.0f)) return;  // coincident partners are skipped
             const f3 diff = pi - xyz(b);
             const float r = fsqrt(dist2(pi, xyz(b)));
             const float ir = frcp(r);
@@ -260,7 +291,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_post_b(const float4 *__restrict__
                 const f3 fcur = -gam * (ni - xyz(normal[j]));
                 f += kij * (fcoh + fcur);
             }
-        });
+        },
+        AllowNotSelf{i}, rec);
     f3 vi = xyz(v[i]);
     if (do_vort && length2(eta) > 1e-3f) {
         const f3 nrm = eta * (1.0f / sqrtf(dot(eta, eta)));
@@ -274,7 +306,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_post_b(const float4 *__restrict__
 template <class SP>
 __global__ __launch_bounds__(kMaxBlock) void k_xsph(const float4 *__restrict__ xs, const float4 *__restrict__ v,
                                                  const int2 *__restrict__ ranges,
-                                                 const uint32_t *__restrict__ ccell, float4 *__restrict__ v_out,
+                                                 const uint32_t *__restrict__ ccell,
+                                                 const uint32_t *__restrict__ srec, float4 *__restrict__ v_out,
                                                  int n, SP sp, int use_ccell) {
     const int i = xcd_index();
     if (i >= n) return;
@@ -297,64 +330,70 @@ __global__ __launch_bounds__(kMaxBlock) void k_xsph(const float4 *__restrict__ x
         G.pos(pi, g);
     const f3 vi = xyz(v4);
     f3 vn = mk3(0.f, 0.f, 0.f);
+    ScreenRec rec;  // after the omega/normal pass: its screen of the same positions
+    rec.in = use_ccell ? srec : nullptr;
+    rec.slot = i;
+    rec.n = n;
     for_each_neighbor(
         G, ranges, g, xs, 0,
-        [&](int j, float4 b) { return (j != i) & (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
-        [&](int j, float4 b) { vn += (xyz(v[j]) - vi) * K.poly6(dist2(pi, xyz(b))); });
+        [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
+        [&](int j, float4 b) { vn += (xyz(v[j]) - vi) * K.poly6(dist2(pi, xyz(b))); }, AllowNotSelf{i}, rec);
     v_out[i] = f4(vi + c * vn, 0.0f);
 }
 
 // ------------------------------------------------------------ launchers
 template <class SP>
 void launch_lambda(const float4 *xs, const float *m, const int2 *ranges, float *lambda, float *density,
-                   uint32_t *ccell, int n, const SP &sp, int block, hipStream_t st) {
+                   uint32_t *ccell, uint32_t *srec, int n, const SP &sp, int block, hipStream_t st) {
     hipLaunchKernelGGL(k_lambda<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, m, ranges, lambda, density,
-                       ccell, n, sp);
+                       ccell, srec, n, sp);
 }
 template <class SP>
 void launch_position(const float4 *xs, const float *m, const float *lambda, const uint32_t *ccell,
-                     const int *build_keys, const int2 *ranges, float4 *sum_tmp, unsigned long long *corr,
-                     float4 *xs_out, float4 *x_io, float4 *v_out, int n, const SP &sp, const Epilogue &E, int block,
-                     hipStream_t st) {
+                     const int *build_keys, const int2 *ranges, const uint32_t *srec, float4 *sum_tmp,
+                     unsigned long long *corr, float4 *xs_out, float4 *x_io, float4 *v_out, int n, const SP &sp,
+                     const Epilogue &E, int block, hipStream_t st) {
     hipLaunchKernelGGL(k_position<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, m, lambda, ccell,
-                       build_keys, ranges, sum_tmp, corr, n, sp);
+                       build_keys, ranges, srec, sum_tmp, corr, n, sp);
     hipLaunchKernelGGL(k_position_apply<SP>, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, sum_tmp, corr, xs_out,
                        x_io, v_out, n, sp, E);
 }
 template <class SP>
 void launch_post_a(const float4 *xs, const float4 *v, const float *density, const int2 *ranges, float4 *omega,
-                   float4 *normal, uint32_t *ccell, int n, const SP &sp, int do_omega, int do_normal, int block,
-                   hipStream_t st) {
+                   float4 *normal, uint32_t *ccell, uint32_t *srec, int n, const SP &sp, int do_omega, int do_normal,
+                   int block, hipStream_t st) {
     hipLaunchKernelGGL(k_post_a<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, v, density, ranges, omega,
-                       normal, ccell, n, sp, do_omega, do_normal);
+                       normal, ccell, srec, n, sp, do_omega, do_normal);
 }
 template <class SP>
 void launch_post_b(const float4 *xs, float4 *v, const float4 *omega, const float4 *normal, const float *density,
-                   const int2 *ranges, const uint32_t *ccell, int n, const SP &sp, int do_vort, int do_akinci,
-                   float dt, int block, hipStream_t st) {
+                   const int2 *ranges, const uint32_t *ccell, const uint32_t *srec, int n, const SP &sp, int do_vort,
+                   int do_akinci, float dt, int block, hipStream_t st) {
     hipLaunchKernelGGL(k_post_b<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, v, omega, normal, density,
-                       ranges, ccell, n, sp, do_vort, do_akinci, dt);
+                       ranges, ccell, srec, n, sp, do_vort, do_akinci, dt);
 }
 template <class SP>
-void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const uint32_t *ccell, float4 *v_out,
-                 int n, const SP &sp, int use_ccell, int block, hipStream_t st) {
-    hipLaunchKernelGGL(k_xsph<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, v, ranges, ccell, v_out, n,
-                       sp, use_ccell);
+void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const uint32_t *ccell, const uint32_t *srec,
+                 float4 *v_out, int n, const SP &sp, int use_ccell, int block, hipStream_t st) {
+    hipLaunchKernelGGL(k_xsph<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, v, ranges, ccell, srec,
+                       v_out, n, sp, use_ccell);
 }
 
 #define BP_INSTANTIATE_FLUID(SP)                                                                                \
     template void launch_lambda<SP>(const float4 *, const float *, const int2 *, float *, float *, uint32_t *,  \
-                                    int, const SP &, int, hipStream_t);                                         \
+                                    uint32_t *, int, const SP &, int, hipStream_t);                             \
     template void launch_position<SP>(const float4 *, const float *, const float *, const uint32_t *,           \
-                                      const int *, const int2 *, float4 *, unsigned long long *, float4 *,      \
-                                      float4 *, float4 *, int, const SP &, const Epilogue &, int, hipStream_t); \
+                                      const int *, const int2 *, const uint32_t *, float4 *,                    \
+                                      unsigned long long *, float4 *, float4 *, float4 *, int, const SP &,      \
+                                      const Epilogue &, int, hipStream_t);                                      \
     template void launch_post_a<SP>(const float4 *, const float4 *, const float *, const int2 *, float4 *,      \
-                                    float4 *, uint32_t *, int, const SP &, int, int, int, hipStream_t);         \
-    template void launch_post_b<SP>(const float4 *, float4 *, const float4 *, const float4 *, const float *,    \
-                                    const int2 *, const uint32_t *, int, const SP &, int, int, float, int,      \
+                                    float4 *, uint32_t *, uint32_t *, int, const SP &, int, int, int,           \
                                     hipStream_t);                                                               \
-    template void launch_xsph<SP>(const float4 *, const float4 *, const int2 *, const uint32_t *, float4 *,     \
-                                  int, const SP &, int, int, hipStream_t);
+    template void launch_post_b<SP>(const float4 *, float4 *, const float4 *, const float4 *, const float *,    \
+                                    const int2 *, const uint32_t *, const uint32_t *, int, const SP &, int,     \
+                                    int, float, int, hipStream_t);                                              \
+    template void launch_xsph<SP>(const float4 *, const float4 *, const int2 *, const uint32_t *,               \
+                                  const uint32_t *, float4 *, int, const SP &, int, int, hipStream_t);
 BP_INSTANTIATE_FLUID(OneScene)
 BP_INSTANTIATE_FLUID(SceneTable)
 

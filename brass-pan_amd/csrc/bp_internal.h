@@ -232,42 +232,119 @@ __device__ __forceinline__ void for_each_run(const GridDesc &G, const int2 *__re
     }
 }
 
+// Index rules of a walk, as masks over a chunk of 32 consecutive slots (bit t
+// stands for slot base + t), so the per-candidate screen holds no index
+// comparisons.
+struct AllowAll {
+    __device__ __forceinline__ uint32_t operator()(int) const { return ~0u; }
+};
+__device__ __forceinline__ uint32_t chunk_bit(int k, int base) {  // bit of slot k, 0 if outside
+    const uint32_t d = (uint32_t)(k - base);
+    return d < 32u ? (1u << d) : 0u;
+}
+__device__ __forceinline__ uint32_t chunk_below(int k, int base) {  // bits of slots < k
+    const int d = k - base;
+    return d <= 0 ? 0u : (d >= 32 ? ~0u : (1u << d) - 1u);
+}
+struct AllowNotSelf {
+    int k;
+    __device__ __forceinline__ uint32_t operator()(int base) const { return ~chunk_bit(k, base); }
+};
+
+// Screen record: the candidate masks one walk computed, replayed by a later
+// walk over the same positions, cell and ranges instead of screening again
+// (lambda -> position of one iteration; omega/normal -> vorticity/tension and
+// XSPH).  Chunk c (in walk order) of slot `slot` is m[c * n + slot], for the
+// first kRecChunks chunks; later chunks are screened by both walks.
+constexpr int kRecChunks = 16;
+struct ScreenRec {
+    uint32_t *out = nullptr;       // record this walk's masks (every chunk is screened)
+    const uint32_t *in = nullptr;  // replay recorded masks
+    int slot = 0, n = 0;
+};
+
 // Two-phase neighbour walk: for every run, chunks of up to 32 candidates are
 // first screened with the cheap test `pick(j, b)` (b = xs[j]) into a bitmask,
-// then `body(j, b)` runs only for the picked candidates, in ascending j.  The
-// expensive pair math therefore executes max(picked) times per wave instead
-// of once per candidate, and the visiting order is unchanged.
+// which `allow(base)` then restricts by slot index.  The candidates left are
+// consumed NB at a time: `fetch(j)` loads what the pair term needs for NB
+// partners as independent loads, then `use(j, rec)` runs for each in
+// ascending j (slots past the last pick fetch the first pick again and are not
+// used).  The expensive pair math therefore executes max(picked) times per
+// wave instead of once per candidate, and the visiting order is unchanged.
 //
 // Candidates are loaded in batches of 8 from one base address with immediate
 // offsets.  A batch may read up to 7 slots past the end of its run, so every
 // x* buffer a walk reads holds kWalkPad slots beyond the last particle; the
 // bits of those slots are cleared once per chunk.
 constexpr int kWalkPad = 8;
+constexpr int kBodyBatch = 2;
+template <int NB, class Pick, class Fetch, class Use, class Allow>
+__device__ __forceinline__ void for_each_neighbor_nb(const GridDesc &G, const int2 *__restrict__ ranges,
+                                                     const int g[3], const float4 *__restrict__ xs, int jmin,
+                                                     Pick &&pick, Fetch &&fetch, Use &&use, Allow &&allow,
+                                                     ScreenRec rec = ScreenRec{}) {
+    int c = 0;  // chunk number in walk order
+    for_each_run(G, ranges, g, [&](int s, int e) {
+        if (s < jmin) s = jmin;
+        for (int base = s; base < e; base += 32, ++c) {
+            const int cnt = (e - base) < 32 ? (e - base) : 32;
+            const uint32_t valid = (cnt >= 32) ? 0xffffffffu : ((1u << cnt) - 1u);
+            const uint32_t keep = allow(base) & valid;
+            uint32_t mask = 0u;
+            if (rec.in && c < kRecChunks) {
+                if (keep == 0u) continue;
+                mask = rec.in[(size_t)c * rec.n + rec.slot];
+            } else {
+                if (keep == 0u && !rec.out) continue;
+                for (int t0 = 0; t0 < cnt; t0 += 8) {
+                    const float4 *__restrict__ p = xs + (base + t0);
+                    float4 bb[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) bb[u] = p[u];
+                    uint32_t m8 = 0u;  // constant shifts within the batch, one variable shift per batch
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) m8 |= (uint32_t)pick(base + t0 + u, bb[u]) << u;
+                    mask |= m8 << t0;
+                }
+                mask &= valid;
+                if (rec.out && c < kRecChunks) rec.out[(size_t)c * rec.n + rec.slot] = mask;
+            }
+            mask &= keep;
+            while (mask) {
+                int j[NB];
+                bool ok[NB];
+                j[0] = base + __ffs(mask) - 1;
+                ok[0] = true;
+                mask &= mask - 1u;
+#pragma unroll
+                for (int u = 1; u < NB; ++u) {
+                    ok[u] = mask != 0u;
+                    j[u] = ok[u] ? base + __ffs(mask) - 1 : j[0];
+                    mask &= mask - 1u;
+                }
+                decltype(fetch(0)) rv[NB];
+#pragma unroll
+                for (int u = 0; u < NB; ++u) rv[u] = fetch(j[u]);
+#pragma unroll
+                for (int u = 0; u < NB; ++u)
+                    if (ok[u]) use(j[u], rv[u]);
+            }
+        }
+    });
+}
+template <class Pick, class Body, class Allow>
+__device__ __forceinline__ void for_each_neighbor(const GridDesc &G, const int2 *__restrict__ ranges,
+                                                  const int g[3], const float4 *__restrict__ xs, int jmin,
+                                                  Pick &&pick, Body &&body, Allow &&allow,
+                                                  ScreenRec rec = ScreenRec{}) {
+    for_each_neighbor_nb<1>(
+        G, ranges, g, xs, jmin, pick, [&](int j) { return xs[j]; }, body, allow, rec);
+}
 template <class Pick, class Body>
 __device__ __forceinline__ void for_each_neighbor(const GridDesc &G, const int2 *__restrict__ ranges,
                                                   const int g[3], const float4 *__restrict__ xs, int jmin,
                                                   Pick &&pick, Body &&body) {
-    for_each_run(G, ranges, g, [&](int s, int e) {
-        if (s < jmin) s = jmin;
-        for (int base = s; base < e; base += 32) {
-            const int cnt = (e - base) < 32 ? (e - base) : 32;
-            uint32_t mask = 0u;
-            for (int t0 = 0; t0 < cnt; t0 += 8) {
-                const float4 *__restrict__ p = xs + (base + t0);
-                float4 bb[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) bb[u] = p[u];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) mask |= (uint32_t)pick(base + t0 + u, bb[u]) << (t0 + u);
-            }
-            mask &= (cnt >= 32) ? 0xffffffffu : ((1u << cnt) - 1u);
-            while (mask) {
-                const int t = __ffs(mask) - 1;
-                mask &= mask - 1u;
-                body(base + t, xs[base + t]);
-            }
-        }
-    });
+    for_each_neighbor(G, ranges, g, xs, jmin, pick, body, AllowAll{});
 }
 
 // Walks the CSR entries [e0, e1) in order, eight at a time: the entry

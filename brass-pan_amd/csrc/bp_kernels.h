@@ -21,26 +21,29 @@ void launch_reorder(const int *perm, const float4 *x_in, const float4 *v_in, con
                     int *ph_out, int *o_out, int *sid_out, int *map_to_new, int n, float dt, f3 gstep, float mass_k,
                     int need_w, int stab_passes, int collide, const SP &sp, hipStream_t st);
 
-// bp_fluid.hip (`block`: threads per workgroup of the neighbour pass, <= kMaxBlock)
+// bp_fluid.hip (`block`: threads per workgroup of the neighbour pass, <= kMaxBlock).
+// `srec` (kRecChunks x n candidate masks) is written by the lambda and
+// omega/normal passes and replayed by the position pass of the same iteration
+// and by the vorticity/tension and XSPH passes.
 template <class SP>
 void launch_lambda(const float4 *xs, const float *m, const int2 *ranges, float *lambda, float *density,
-                   uint32_t *ccell, int n, const SP &sp, int block, hipStream_t st);
+                   uint32_t *ccell, uint32_t *srec, int n, const SP &sp, int block, hipStream_t st);
 template <class SP>
 void launch_position(const float4 *xs, const float *m, const float *lambda, const uint32_t *ccell,
-                     const int *build_keys, const int2 *ranges, float4 *sum_tmp, unsigned long long *corr,
-                     float4 *xs_out, float4 *x_io, float4 *v_out, int n, const SP &sp, const Epilogue &E, int block,
-                     hipStream_t st);
+                     const int *build_keys, const int2 *ranges, const uint32_t *srec, float4 *sum_tmp,
+                     unsigned long long *corr, float4 *xs_out, float4 *x_io, float4 *v_out, int n, const SP &sp,
+                     const Epilogue &E, int block, hipStream_t st);
 template <class SP>
 void launch_post_a(const float4 *xs, const float4 *v, const float *density, const int2 *ranges, float4 *omega,
-                   float4 *normal, uint32_t *ccell, int n, const SP &sp, int do_omega, int do_normal, int block,
-                   hipStream_t st);
+                   float4 *normal, uint32_t *ccell, uint32_t *srec, int n, const SP &sp, int do_omega, int do_normal,
+                   int block, hipStream_t st);
 template <class SP>
 void launch_post_b(const float4 *xs, float4 *v, const float4 *omega, const float4 *normal, const float *density,
-                   const int2 *ranges, const uint32_t *ccell, int n, const SP &sp, int do_vort, int do_akinci,
-                   float dt, int block, hipStream_t st);
+                   const int2 *ranges, const uint32_t *ccell, const uint32_t *srec, int n, const SP &sp, int do_vort,
+                   int do_akinci, float dt, int block, hipStream_t st);
 template <class SP>
-void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const uint32_t *ccell, float4 *v_out,
-                 int n, const SP &sp, int use_ccell, int block, hipStream_t st);
+void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const uint32_t *ccell, const uint32_t *srec,
+                 float4 *v_out, int n, const SP &sp, int use_ccell, int block, hipStream_t st);
 
 // bp_constraints.hip
 template <class SP>

@@ -210,7 +210,7 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
         }
         if (H.xl) HIP_TRY(snap(H.xl[it], s->xs.cur(), N, st));
         if (s->has_fluid) {
-            TIMED(F_LAMBDA, launch_lambda(s->xs.cur(), s->m.cur(), s->ranges, s->lambda, s->density, s->ccell, N, sp(),
+            TIMED(F_LAMBDA, launch_lambda(s->xs.cur(), s->m.cur(), s->ranges, s->lambda, s->density, s->ccell, s->srec, N, sp(),
                                           s->blk_lambda, st));
             if (H.lam) HIP_TRY(snap(H.lam[it], s->lambda, N, st));
             Epilogue E{};
@@ -219,7 +219,7 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
             E.inv_dt = inv_dt;
             E.sleep_eps2 = P.sleep_epsilon * P.sleep_epsilon;
             TIMED(F_POSITION, launch_position(s->xs.cur(), s->m.cur(), s->lambda, s->ccell, sk, s->ranges,
-                                              s->sum_tmp, s->corr, s->xs.alt(), s->x.cur(), s->v.cur(), N, sp(), E,
+                                              s->srec, s->sum_tmp, s->corr, s->xs.alt(), s->x.cur(), s->v.cur(), N, sp(), E,
                                               s->blk_position, st));
             s->xs.flip();
             planes_done = E.planes_next;
@@ -248,12 +248,12 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
         const int do_ak = P.akinci ? 1 : 0;
         if (do_vort || do_ak) {
             TIMED(F_POST_A, launch_post_a(s->xs.cur(), s->v.cur(), s->density, s->ranges, s->omega, s->normal,
-                                          s->ccell, N, sp(), do_vort, do_ak, s->blk_post_a, st));
+                                          s->ccell, s->srec, N, sp(), do_vort, do_ak, s->blk_post_a, st));
             TIMED(F_POST_B, launch_post_b(s->xs.cur(), s->v.cur(), s->omega, s->normal, s->density, s->ranges,
-                                          s->ccell, N, sp(), do_vort, do_ak, subdt, s->blk_post_b, st));
+                                          s->ccell, s->srec, N, sp(), do_vort, do_ak, subdt, s->blk_post_b, st));
         }
         if (P.xsph_c != 0.0f) {
-            TIMED(F_XSPH, launch_xsph(s->xs.cur(), s->v.cur(), s->ranges, s->ccell, s->v.alt(), N, sp(),
+            TIMED(F_XSPH, launch_xsph(s->xs.cur(), s->v.cur(), s->ranges, s->ccell, s->srec, s->v.alt(), N, sp(),
                                       (do_vort || do_ak) ? 1 : 0, s->blk_xsph, st));
             s->v.flip();
         }
@@ -519,6 +519,7 @@ bp_status create_impl(const bp_scene_desc *scs, const bp_params *params, int ns,
     TRY(s->alloc(&s->lambda, N));
     TRY(s->alloc(&s->density, N));
     TRY(s->alloc(&s->ccell, N));
+    if (s->has_fluid) TRY(s->alloc(&s->srec, (size_t)kRecChunks * N));
     TRY(s->alloc(&s->sum_tmp, N));
     TRY(s->alloc(&s->corr, 3 * (size_t)N));
     TRY(s->alloc(&s->omega, N));

@@ -79,6 +79,7 @@ struct bp_solver {
 
     float *lambda = nullptr, *density = nullptr;
     uint32_t *ccell = nullptr;
+    uint32_t *srec = nullptr;  // screen record of the lambda and omega/normal passes (kRecChunks x N masks)
     float4 *sum_tmp = nullptr;
     unsigned long long *corr = nullptr;  // fixed-point pair corrections, kept zero between passes
     float4 *omega = nullptr, *normal = nullptr;
