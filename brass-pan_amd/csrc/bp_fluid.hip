@@ -275,20 +275,30 @@ __global__ __launch_bounds__(kMaxBlock) void k_post_b(const float4 *__restrict__
     rec.in = srec;
     rec.slot = i;
     rec.n = n;
-    for_each_neighbor(
+    struct Partner {
+        float4 b, nrm;
+        float om, rho;
+    };
+    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    for_each_neighbor_nb<kBodyBatch>(
         G, ranges, g, xs, 0,
         [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
-        [&](int j, float4 b) {
+        [&](int j) {
+            return Partner{xs[j], do_akinci ? normal[j] : zero4, do_vort ? omega[j].w : 0.0f,
+                           do_akinci ? density[j] : 0.0f};
+        },
+        [&](int, const Partner &P) {
+            const float4 b = P.b;
             if (!(dist2(pi, xyz(b)) > 0.0f)) return;  // coincident partners are skipped
             const f3 diff = pi - xyz(b);
             const float r = fsqrt(dist2(pi, xyz(b)));
             const float ir = frcp(r);
-            if (do_vort) eta += (omega[j].w * K.grad_coef(r, ir)) * diff;
+            if (do_vort) eta += (P.om * K.grad_coef(r, ir)) * diff;
             if (do_akinci) {
                 // k_ij (-gamma C(r) diff / r - gamma (n_i - n_j)) (fluid.cuh:476-483)
-                const float kij = 2.0f * F.rest_density * frcp(rhoi + density[j]);
+                const float kij = 2.0f * F.rest_density * frcp(rhoi + P.rho);
                 const f3 fcoh = (-gam * K.akinci(r) * ir) * diff;
-                const f3 fcur = -gam * (ni - xyz(normal[j]));
+                const f3 fcur = -gam * (ni - xyz(P.nrm));
                 f += kij * (fcoh + fcur);
             }
         },
@@ -334,10 +344,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_xsph(const float4 *__restrict__ x
     rec.in = use_ccell ? srec : nullptr;
     rec.slot = i;
     rec.n = n;
-    for_each_neighbor(
+    struct Partner {
+        float4 b, v;
+    };
+    for_each_neighbor_nb<kBodyBatch>(
         G, ranges, g, xs, 0,
         [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
-        [&](int j, float4 b) { vn += (xyz(v[j]) - vi) * K.poly6(dist2(pi, xyz(b))); }, AllowNotSelf{i}, rec);
+        [&](int j) { return Partner{xs[j], v[j]}; },
+        [&](int, const Partner &P) { vn += (xyz(P.v) - vi) * K.poly6(dist2(pi, xyz(P.b))); }, AllowNotSelf{i},
+        rec);
     v_out[i] = f4(vi + c * vn, 0.0f);
 }
 

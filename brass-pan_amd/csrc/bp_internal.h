@@ -47,6 +47,10 @@ __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 // x^n for a small non-negative integer n (sCorr exponent, fluid.cuh:208)
 __device__ __forceinline__ float ipowf(float x, int n) {
+    if (n == 4) {  // the loop's own product for n = 4, without the loop
+        const float x2 = x * x;
+        return x2 * x2;
+    }
     float r = 1.0f;
     while (n > 0) {
         if (n & 1) r *= x;
@@ -265,24 +269,18 @@ struct ScreenRec {
 
 // Two-phase neighbour walk: for every run, chunks of up to 32 candidates are
 // first screened with the cheap test `pick(j, b)` (b = xs[j]) into a bitmask,
-// which `allow(base)` then restricts by slot index.  The candidates left are
-// consumed NB at a time: `fetch(j)` loads what the pair term needs for NB
-// partners as independent loads, then `use(j, rec)` runs for each in
-// ascending j (slots past the last pick fetch the first pick again and are not
-// used).  The expensive pair math therefore executes max(picked) times per
-// wave instead of once per candidate, and the visiting order is unchanged.
+// which `allow(base)` then restricts by slot index; `consume(base, mask)`
+// receives each chunk's partners.
 //
 // Candidates are loaded in batches of 8 from one base address with immediate
 // offsets.  A batch may read up to 7 slots past the end of its run, so every
 // x* buffer a walk reads holds kWalkPad slots beyond the last particle; the
 // bits of those slots are cleared once per chunk.
 constexpr int kWalkPad = 8;
-constexpr int kBodyBatch = 2;
-template <int NB, class Pick, class Fetch, class Use, class Allow>
-__device__ __forceinline__ void for_each_neighbor_nb(const GridDesc &G, const int2 *__restrict__ ranges,
-                                                     const int g[3], const float4 *__restrict__ xs, int jmin,
-                                                     Pick &&pick, Fetch &&fetch, Use &&use, Allow &&allow,
-                                                     ScreenRec rec = ScreenRec{}) {
+template <class Pick, class Allow, class Consume>
+__device__ __forceinline__ void walk_chunks(const GridDesc &G, const int2 *__restrict__ ranges, const int g[3],
+                                            const float4 *__restrict__ xs, int jmin, Pick &&pick, Allow &&allow,
+                                            ScreenRec rec, Consume &&consume) {
     int c = 0;  // chunk number in walk order
     for_each_run(G, ranges, g, [&](int s, int e) {
         if (s < jmin) s = jmin;
@@ -310,25 +308,64 @@ __device__ __forceinline__ void for_each_neighbor_nb(const GridDesc &G, const in
                 if (rec.out && c < kRecChunks) rec.out[(size_t)c * rec.n + rec.slot] = mask;
             }
             mask &= keep;
-            while (mask) {
-                int j[NB];
-                bool ok[NB];
-                j[0] = base + __ffs(mask) - 1;
-                ok[0] = true;
-                mask &= mask - 1u;
+            if (mask) consume(base, mask);
+        }
+    });
+}
+
+// Marks every word of a fetched record as used here, so the compiler keeps its
+// loads where they were issued instead of sinking each into the branch of the
+// pair term that reads it (which costs one memory round trip per branch).
+template <class T> __device__ __forceinline__ void pin_loaded(const T &v) {
+    static_assert(sizeof(T) % 4 == 0, "records are whole words");
+    const uint32_t *w = reinterpret_cast<const uint32_t *>(&v);
 #pragma unroll
-                for (int u = 1; u < NB; ++u) {
-                    ok[u] = mask != 0u;
-                    j[u] = ok[u] ? base + __ffs(mask) - 1 : j[0];
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) asm volatile("" ::"v"(w[i]));
+}
+
+// Pair bodies, NB partners at a time: `fetch(j)` loads everything the pair
+// term needs, for NB partners, as independent loads that complete together;
+// then `use(j, rec)` runs for each in ascending j.  next(u, ok) yields the
+// u-th partner and whether it exists (a missing one fetches the first partner
+// again and is not used).
+template <int NB, class Next, class Fetch, class Use>
+__device__ __forceinline__ void run_bodies(Next &&next, Fetch &&fetch, Use &&use) {
+    int j[NB];
+    bool ok[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+        j[u] = next(u, ok[u]);
+        if (u > 0 && !ok[u]) j[u] = j[0];
+    }
+    decltype(fetch(0)) rv[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) rv[u] = fetch(j[u]);
+#pragma unroll
+    for (int u = 0; u < NB; ++u) pin_loaded(rv[u]);
+#pragma unroll
+    for (int u = 0; u < NB; ++u)
+        if (ok[u]) use(j[u], rv[u]);
+}
+
+// The walk with its pair bodies, run per chunk: a wave executes the body
+// sum-over-chunks(max over lanes of the chunk's partners) times, in ascending
+// j per particle.
+constexpr int kBodyBatch = 2;
+template <int NB, class Pick, class Fetch, class Use, class Allow>
+__device__ __forceinline__ void for_each_neighbor_nb(const GridDesc &G, const int2 *__restrict__ ranges,
+                                                     const int g[3], const float4 *__restrict__ xs, int jmin,
+                                                     Pick &&pick, Fetch &&fetch, Use &&use, Allow &&allow,
+                                                     ScreenRec rec = ScreenRec{}) {
+    walk_chunks(G, ranges, g, xs, jmin, pick, allow, rec, [&](int base, uint32_t mask) {
+        while (mask) {
+            run_bodies<NB>(
+                [&](int, bool &ok) {
+                    ok = mask != 0u;
+                    const int j = base + __ffs(mask) - 1;
                     mask &= mask - 1u;
-                }
-                decltype(fetch(0)) rv[NB];
-#pragma unroll
-                for (int u = 0; u < NB; ++u) rv[u] = fetch(j[u]);
-#pragma unroll
-                for (int u = 0; u < NB; ++u)
-                    if (ok[u]) use(j[u], rv[u]);
-            }
+                    return j;
+                },
+                fetch, use);
         }
     });
 }
