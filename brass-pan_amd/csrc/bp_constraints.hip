@@ -23,32 +23,20 @@ __global__ __launch_bounds__(kBlock) void k_plane_collide(float4 *__restrict__ x
     xs[i] = f4(out, a.w);
 }
 
-// Cells holding a solid this substep: cell_stamp[cell] == *stamp.  The stamp
-// advances once per substep (k_stamp_next), so no table needs clearing.
+// Cells holding a solid this substep: cell_stamp[cell] == *stamp; cells
+// holding a fluid particle: fluid_stamp[cell] == *stamp.  The stamp advances
+// once per substep (k_stamp_next), so no table needs clearing.
 __global__ void k_stamp_next(uint32_t *__restrict__ stamp) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *stamp = *stamp + 1u;
 }
-__global__ __launch_bounds__(kBlock) void k_mark_solid_cells(const int *__restrict__ sorted_keys,
-                                                             const int *__restrict__ ph, int n,
-                                                             const uint32_t *__restrict__ stamp,
-                                                             uint32_t *__restrict__ cell_stamp) {
+__global__ __launch_bounds__(kBlock) void k_mark_cells(const int *__restrict__ sorted_keys,
+                                                       const int *__restrict__ ph, int n,
+                                                       const uint32_t *__restrict__ stamp,
+                                                       uint32_t *__restrict__ cell_stamp,
+                                                       uint32_t *__restrict__ fluid_stamp) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || ph[i] <= 0) return;
-    cell_stamp[sorted_keys[i]] = *stamp;
-}
-
-// True iff one of the 27 cells around g holds a solid this substep.
-__device__ __forceinline__ bool solid_near(const GridDesc &G, const int g[3], const uint32_t *__restrict__ cell_stamp,
-                                           uint32_t s) {
-    bool any = false;
-    for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
-            const uint32_t col = G.base + ((uint32_t)(g[0] + dx) & G.mask[0]) * G.stride_x +
-                                 ((uint32_t)(g[1] + dy) & G.mask[1]) * (uint32_t)G.dims[2];
-#pragma unroll
-            for (int dz = -1; dz <= 1; ++dz) any |= cell_stamp[col + ((uint32_t)(g[2] + dz) & G.mask[2])] == s;
-        }
-    return any;
+    if (i >= n) return;
+    (ph[i] > 0 ? cell_stamp : fluid_stamp)[sorted_keys[i]] = *stamp;
 }
 
 // particleParticleCollisionConstraint (particleparticle.cuh:5-101) + Δx apply.
@@ -64,65 +52,66 @@ __global__ __launch_bounds__(kMaxBlock) void k_particle_particle(const float4 *_
                                                                  const uint32_t *__restrict__ stamp) {
     const int i = xcd_index();
     if (i >= n) return;
-    const SceneConst &S = sp.at(i);
-    const GridDesc &G = S.G;
-    const float r = S.radius, fs = S.friction_static, fd = S.friction_dynamic;
-    const float4 a = xs[i];
-    const float4 xa = x[i];
-    const f3 xi = xyz(xa), xip = xyz(a);
-    const float wi = xa.w;
-    const int pi = phase_of(a);
-    f3 sumD = mk3(0.f, 0.f, 0.f), sumF = mk3(0.f, 0.f, 0.f);
-    int cnt = 0;
-    int g[3];
-    G.pos(xip, g);
-    if (cell_stamp && pi <= 0 && !solid_near(G, g, cell_stamp, *stamp)) {
-        xs_out[i] = a;
-        return;
-    }
-    const float r2x4 = r * r * 4.0f;
-    const float thr = r * r * 0.001f * 0.001f;
-    for_each_neighbor(
-        G, ranges, g, xs, 0,
-        [&](int, float4 b) {  // pi != pj also excludes i itself
-            const int pj = phase_of(b);
-            const float d2 = dist2(xip, xyz(b));
-            return (pi != pj) & ((pi > 0) | (pj > 0)) & (d2 < r2x4) & (d2 > 1e-5f);
-        },
-        [&](int j, float4 b) {
-            const f3 xjp = xyz(b);
-            const f3 diff = xip - xjp;
-            const float dist = fsqrt(dist2(xip, xjp));
-            const float idist = frcp(dist);
-            const float4 xb = x[j];
-            const float wj = xb.w;
-            const float iw = frcp(wi + wj);
-            const float w1 = wi * iw;
-            const float w2 = wj * iw;
-            const f3 proj = diff * (2.0f * r * idist - 1.0f);
-            const f3 dxi = w1 * proj;
-            const f3 xistar = dxi + xip;
-            sumD += dxi;
-            const f3 dxj = -w2 * proj;
-            const float l2 = length2(dxi);
-            if (l2 > thr) {
-                cnt += 1;
-                const f3 xjstar = dxj + xjp;
-                const f3 term1 = (xistar - xi) - (xjstar - xyz(xb));
-                const f3 nn = diff * idist;
-                const f3 tan = term1 - dot(term1, nn) * nn;
-                const float tl2 = length2(tan);
-                if (tl2 <= (fs * fs) * l2)
-                    sumF -= w1 * tan;
-                else
-                    sumF -= w1 * tan * fminf(fd * fsqrt(l2 * frcp(tl2)), 1.0f);
-            }
-        });
-    const f3 d = (cnt == 0) ? sumD : sumD + sumF / (float)cnt;
-    xs_out[i] = f4(xip + d, a.w);
-    // the adjoint treats this pass as the identity: it counts the particles
-    // whose contact set is not empty
-    if (contacts && (sumD.x != 0.0f || sumD.y != 0.0f || sumD.z != 0.0f || cnt > 0)) atomicAdd(contacts, 1);
+    sp.with_scene(i, [&](const SceneConst &S) {
+        const GridDesc &G = S.G;
+        const float r = S.radius, fs = S.friction_static, fd = S.friction_dynamic;
+        const float4 a = xs[i];
+        const float4 xa = x[i];
+        const f3 xi = xyz(xa), xip = xyz(a);
+        const float wi = xa.w;
+        const int pi = phase_of(a);
+        f3 sumD = mk3(0.f, 0.f, 0.f), sumF = mk3(0.f, 0.f, 0.f);
+        int cnt = 0;
+        int g[3];
+        G.pos(xip, g);
+        if (cell_stamp && pi <= 0 && !stamp_near(G, g, cell_stamp, *stamp)) {
+            xs_out[i] = a;
+            return;
+        }
+        const float r2x4 = r * r * 4.0f;
+        const float thr = r * r * 0.001f * 0.001f;
+        for_each_neighbor(
+            G, ranges, g, xs, 0,
+            [&](int, float4 b) {  // pi != pj also excludes i itself
+                const int pj = phase_of(b);
+                const float d2 = dist2(xip, xyz(b));
+                return (pi != pj) & ((pi > 0) | (pj > 0)) & (d2 < r2x4) & (d2 > 1e-5f);
+            },
+            [&](int j, float4 b) {
+                const f3 xjp = xyz(b);
+                const f3 diff = xip - xjp;
+                const float dist = fsqrt(dist2(xip, xjp));
+                const float idist = frcp(dist);
+                const float4 xb = x[j];
+                const float wj = xb.w;
+                const float iw = frcp(wi + wj);
+                const float w1 = wi * iw;
+                const float w2 = wj * iw;
+                const f3 proj = diff * (2.0f * r * idist - 1.0f);
+                const f3 dxi = w1 * proj;
+                const f3 xistar = dxi + xip;
+                sumD += dxi;
+                const f3 dxj = -w2 * proj;
+                const float l2 = length2(dxi);
+                if (l2 > thr) {
+                    cnt += 1;
+                    const f3 xjstar = dxj + xjp;
+                    const f3 term1 = (xistar - xi) - (xjstar - xyz(xb));
+                    const f3 nn = diff * idist;
+                    const f3 tan = term1 - dot(term1, nn) * nn;
+                    const float tl2 = length2(tan);
+                    if (tl2 <= (fs * fs) * l2)
+                        sumF -= w1 * tan;
+                    else
+                        sumF -= w1 * tan * fminf(fd * fsqrt(l2 * frcp(tl2)), 1.0f);
+                }
+            });
+        const f3 d = (cnt == 0) ? sumD : sumD + sumF / (float)cnt;
+        xs_out[i] = f4(xip + d, a.w);
+        // the adjoint treats this pass as the identity: it counts the particles
+        // whose contact set is not empty
+        if (contacts && (sumD.x != 0.0f || sumD.y != 0.0f || sumD.z != 0.0f || cnt > 0)) atomicAdd(contacts, 1);
+    });
 }
 
 // distanceConstraints (distance.cuh:3-41): k of each pair
@@ -404,11 +393,11 @@ void launch_particle_particle(const float4 *xs, const float4 *x, const int2 *ran
     hipLaunchKernelGGL(k_particle_particle<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, x, ranges,
                        xs_out, n, sp, contacts, cell_stamp, stamp);
 }
-void launch_mark_solid_cells(const int *sorted_keys, const int *ph, int n, uint32_t *stamp, uint32_t *cell_stamp,
-                             hipStream_t st) {
+void launch_mark_cells(const int *sorted_keys, const int *ph, int n, uint32_t *stamp, uint32_t *cell_stamp,
+                       uint32_t *fluid_stamp, hipStream_t st) {
     hipLaunchKernelGGL(k_stamp_next, dim3(1), dim3(64), 0, st, stamp);
-    hipLaunchKernelGGL(k_mark_solid_cells, dim3(grid_blocks(n)), dim3(kBlock), 0, st, sorted_keys, ph, n, stamp,
-                       cell_stamp);
+    hipLaunchKernelGGL(k_mark_cells, dim3(grid_blocks(n)), dim3(kBlock), 0, st, sorted_keys, ph, n, stamp,
+                       cell_stamp, fluid_stamp);
 }
 void launch_distance(float4 *xs, const float4 *x, const int2 *pairs, const float2 *params, const int *map_to_new,
                      float4 *kbuf, int nc, const int *verts, const int *offs, const int *ents, int nv,

@@ -18,58 +18,59 @@ __global__ __launch_bounds__(kMaxBlock) void k_lambda(const float4 *__restrict__
                                                    uint32_t *__restrict__ srec, int n, SP sp) {
     const int i = xcd_index();
     if (i >= n) return;
-    const SceneConst &S = sp.at(i);
-    const GridDesc &G = S.G;
-    const SphConst &K = S.K;
-    const FluidParams &F = S.F;
-    const float4 a = xs[i];
-    const f3 pi = xyz(a);
-    int g[3];
-    G.pos(pi, g);
-    ccell[i] = G.pack(g);
-    if (phase_of(a) > 0) {  // solids: lambda 0 (the reference never writes theirs)
-        lambda[i] = 0.0f;
-        density[i] = 0.0f;
-        return;
-    }
-    float dens = 0.0f, sumg2 = 0.0f;
-    f3 gi = mk3(0.f, 0.f, 0.f);
-    const float sds = F.solid_density_scale;
-    // Q1: only j >= i (fluid.cuh:117).  With a screen record the walk covers
-    // every j (the position pass replays it) and j < i are masked off.
-    const float inv_rho0 = F.inv_rest_density;
-    struct Partner {
-        float4 b;
-        float m;
-    };
-    ScreenRec rec;
-    rec.out = srec;
-    rec.slot = i;
-    rec.n = n;
-    for_each_neighbor_nb<kBodyBatch>(
-        G, ranges, g, xs, srec ? 0 : i, [&](int, float4 b) { return dist2(pi, xyz(b)) <= K.h2; },
-        [&](int j) { return Partner{xs[j], m[j]}; },
-        [&](int, const Partner &P) {
-            const float4 b = P.b;
-            const float d2 = dist2(pi, xyz(b));
-            const float mj = P.m;
-            const float ld = mj * K.poly6(d2);
-            dens += (phase_of(b) < 0) ? ld : sds * ld;
-            if (F.lambda_mode == 1 && d2 > 0.0f) {
-                // grad C_j = -m_j gradW / rho0 (fluid.cuh:132-137)
-                const float r = fsqrt(d2);
-                const float gc = -mj * K.grad_coef(r, frcp(r)) * inv_rho0;
-                const f3 gr = gc * (pi - xyz(b));
-                sumg2 += gc * gc * d2;
-                gi -= gr;
-            }
-        },
-        [&](int base) { return ~chunk_below(i, base); }, rec);
-    if (F.lambda_mode == 1) sumg2 += dot(gi, gi);
-    float C = dens / F.rest_density - 1.0f;
-    if (F.akinci) C = fmaxf(C, 0.0f);
-    lambda[i] = (F.lambda_mode == 1) ? -C / (sumg2 + F.lambda_relaxation) : -C / F.lambda_denominator;
-    density[i] = dens;
+    sp.with_scene(i, [&](const SceneConst &S) {
+        const GridDesc &G = S.G;
+        const SphConst &K = S.K;
+        const FluidParams &F = S.F;
+        const float4 a = xs[i];
+        const f3 pi = xyz(a);
+        int g[3];
+        G.pos(pi, g);
+        ccell[i] = G.pack(g);
+        if (phase_of(a) > 0) {  // solids: lambda 0 (the reference never writes theirs)
+            lambda[i] = 0.0f;
+            density[i] = 0.0f;
+            return;
+        }
+        float dens = 0.0f, sumg2 = 0.0f;
+        f3 gi = mk3(0.f, 0.f, 0.f);
+        const float sds = F.solid_density_scale;
+        // Q1: only j >= i (fluid.cuh:117).  With a screen record the walk covers
+        // every j (the position pass replays it) and j < i are masked off.
+        const float inv_rho0 = F.inv_rest_density;
+        struct Partner {
+            float4 b;
+            float m;
+        };
+        ScreenRec rec;
+        rec.out = srec;
+        rec.slot = i;
+        rec.n = n;
+        for_each_neighbor_nb<kBodyBatch>(
+            G, ranges, g, xs, srec ? 0 : i, [&](int, float4 b) { return dist2(pi, xyz(b)) <= K.h2; },
+            [&](int j) { return Partner{xs[j], m[j]}; },
+            [&](int, const Partner &P) {
+                const float4 b = P.b;
+                const float d2 = dist2(pi, xyz(b));
+                const float mj = P.m;
+                const float ld = mj * K.poly6(d2);
+                dens += (phase_of(b) < 0) ? ld : sds * ld;
+                if (F.lambda_mode == 1 && d2 > 0.0f) {
+                    // grad C_j = -m_j gradW / rho0 (fluid.cuh:132-137)
+                    const float r = fsqrt(d2);
+                    const float gc = -mj * K.grad_coef(r, frcp(r)) * inv_rho0;
+                    const f3 gr = gc * (pi - xyz(b));
+                    sumg2 += gc * gc * d2;
+                    gi -= gr;
+                }
+            },
+            [&](int base) { return ~chunk_below(i, base); }, rec);
+        if (F.lambda_mode == 1) sumg2 += dot(gi, gi);
+        float C = dens / F.rest_density - 1.0f;
+        if (F.akinci) C = fmaxf(C, 0.0f);
+        lambda[i] = (F.lambda_mode == 1) ? -C / (sumg2 + F.lambda_relaxation) : -C / F.lambda_denominator;
+        density[i] = dens;
+    });
 }
 
 // fluidPosition pair term (fluid.cuh:199-221) of pair (k,j) seen from k, as a
@@ -107,59 +108,70 @@ __global__ __launch_bounds__(kMaxBlock) void k_position(const float4 *__restrict
                                                      const uint32_t *__restrict__ ccell,
                                                      const int *__restrict__ build_keys,
                                                      const int2 *__restrict__ ranges,
-                                                     const uint32_t *__restrict__ srec, float4 *__restrict__ sum_out,
+                                                     const uint32_t *__restrict__ srec,
+                                                     const uint32_t *__restrict__ fluid_stamp,
+                                                     const uint32_t *__restrict__ stamp, float4 *__restrict__ sum_out,
                                                      unsigned long long *__restrict__ corr, int n, SP sp) {
     const int k = xcd_index();
     if (k >= n) return;
-    const SceneConst &S = sp.at(k);
-    const GridDesc &G = S.G;
-    const SphConst &K = S.K;
-    const FluidParams &F = S.F;
-    const float4 a = xs[k];
-    const f3 pk = xyz(a);
-    const bool kf = phase_of(a) <= 0;
-    const float lk = lambda[k];
-    const float mk = m[k];
-    const uint32_t ck = ccell[k];
-    const uint32_t bk = G.pack_addr(build_keys[k]);
-    int g[3];
-    unpack_cell(ck, g);
-    f3 sum = mk3(0.f, 0.f, 0.f);
-    struct Partner {
-        float4 b;
-        uint32_t cell;
-        float lam, m;
-    };
-    ScreenRec rec;  // the lambda pass screened fluid particles at this x*
-    rec.in = kf ? srec : nullptr;
-    rec.slot = k;
-    rec.n = n;
-    for_each_neighbor_nb<kBodyBatch>(
-        G, ranges, g, xs, 0,
-        [&](int, float4 b) { return dist2(pk, xyz(b)) <= K.h2; },
-        [&](int j) { return Partner{xs[j], ccell[j], lambda[j], m[j]}; },
-        [&](int j, const Partner &P) {
-            const float4 b = P.b;
-            // coincident partners are skipped; a solid partner's pair is k's only if j > k
-            if (!(dist2(pk, xyz(b)) > 0.0f) | ((phase_of(b) > 0) & !(kf & (j > k)))) return;
-            const bool partner_sees_k = G.in_block(P.cell, bk);
-            if (j < k && !partner_sees_k) return;
-            const f3 diff = pk - xyz(b);
-            const float c = pair_coef(dist2(pk, xyz(b)), lk + P.lam, K, F);
-            const float cj = c * P.m;
-            sum.x = fmaf(cj, diff.x, sum.x);
-            sum.y = fmaf(cj, diff.y, sum.y);
-            sum.z = fmaf(cj, diff.z, sum.z);
-            if (j > k && !partner_sees_k) {  // j will not find this pair: deliver -m_k delta(k,j)
-                const float ck2 = -c * mk;
-                atomicAdd(&corr[3 * j + 0], to_fixed(ck2 * diff.x));
-                atomicAdd(&corr[3 * j + 1], to_fixed(ck2 * diff.y));
-                atomicAdd(&corr[3 * j + 2], to_fixed(ck2 * diff.z));
-            }
-        },
-        // partners j != k if k is fluid, only j < k if k is a solid
-        [&](int base) { return kf ? ~chunk_bit(k, base) : chunk_below(k, base); }, rec);
-    sum_out[k] = f4(sum, 0.0f);
+    sp.with_scene(k, [&](const SceneConst &S) {
+        const GridDesc &G = S.G;
+        const SphConst &K = S.K;
+        const FluidParams &F = S.F;
+        const float4 a = xs[k];
+        const f3 pk = xyz(a);
+        const bool kf = phase_of(a) <= 0;
+        const float lk = lambda[k];
+        const float mk = m[k];
+        const uint32_t ck = ccell[k];
+        const uint32_t bk = G.pack_addr(build_keys[k]);
+        int g[3];
+        unpack_cell(ck, g);
+        f3 sum = mk3(0.f, 0.f, 0.f);
+        // A solid's pairs all have a fluid partner: with no fluid particle in its
+        // 27 cells the sum stays zero.
+        if (!kf && fluid_stamp && !stamp_near(G, g, fluid_stamp, *stamp)) {
+            sum_out[k] = f4(sum, 0.0f);
+            return;
+        }
+        struct Partner {
+            float4 b;
+            uint32_t cell;
+            float lam, m;
+        };
+        ScreenRec rec;  // the lambda pass screened fluid particles at this x*
+        rec.in = kf ? srec : nullptr;
+        rec.slot = k;
+        rec.n = n;
+        for_each_neighbor_nb<kBodyBatch>(
+            G, ranges, g, xs, 0,
+            // (fluid particles replay the lambda pass's masks; a solid screens for
+            // fluid partners only)
+            [&](int, float4 b) { return (dist2(pk, xyz(b)) <= K.h2) & (kf | (phase_of(b) <= 0)); },
+            [&](int j) { return Partner{xs[j], ccell[j], lambda[j], m[j]}; },
+            [&](int j, const Partner &P) {
+                const float4 b = P.b;
+                // coincident partners are skipped; a solid partner's pair is k's only if j > k
+                if (!(dist2(pk, xyz(b)) > 0.0f) | ((phase_of(b) > 0) & !(kf & (j > k)))) return;
+                const bool partner_sees_k = G.in_block(P.cell, bk);
+                if (j < k && !partner_sees_k) return;
+                const f3 diff = pk - xyz(b);
+                const float c = pair_coef(dist2(pk, xyz(b)), lk + P.lam, K, F);
+                const float cj = c * P.m;
+                sum.x = fmaf(cj, diff.x, sum.x);
+                sum.y = fmaf(cj, diff.y, sum.y);
+                sum.z = fmaf(cj, diff.z, sum.z);
+                if (j > k && !partner_sees_k) {  // j will not find this pair: deliver -m_k delta(k,j)
+                    const float ck2 = -c * mk;
+                    atomicAdd(&corr[3 * j + 0], to_fixed(ck2 * diff.x));
+                    atomicAdd(&corr[3 * j + 1], to_fixed(ck2 * diff.y));
+                    atomicAdd(&corr[3 * j + 2], to_fixed(ck2 * diff.z));
+                }
+            },
+            // partners j != k if k is fluid, only j < k if k is a solid
+            [&](int base) { return kf ? ~chunk_bit(k, base) : chunk_below(k, base); }, rec);
+        sum_out[k] = f4(sum, 0.0f);
+    });
 }
 
 // Δx = (own sum + delivered pairs) / rho0, x* += Δx (particlesolver.cuh:919-932)
@@ -172,32 +184,33 @@ __global__ __launch_bounds__(kBlock) void k_position_apply(const float4 *__restr
                                                            float4 *__restrict__ v_out, int n, SP sp, Epilogue E) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const SceneConst &S = sp.at(k);
-    const FluidParams &F = S.F;
-    const float4 a = xs[k];
-    f3 sum = xyz(sum_in[k]);
-    const unsigned long long cx = corr[3 * k], cy = corr[3 * k + 1], cz = corr[3 * k + 2];
-    if ((cx | cy | cz) != 0ull) {
-        const float inv = (float)(1.0 / kCorrScale);
-        sum.x += (float)(long long)cx * inv;
-        sum.y += (float)(long long)cy * inv;
-        sum.z += (float)(long long)cz * inv;
-        corr[3 * k] = 0ull;
-        corr[3 * k + 1] = 0ull;
-        corr[3 * k + 2] = 0ull;
-    }
-    f3 out = xyz(a) + sum * F.inv_rest_density;
-    if (E.planes_next || E.finish) {
-        float4 x4 = x_io[k];
-        const f3 xp = xyz(x4);
-        if (E.planes_next) out = plane_collide_one(out, xp, S.pl, S.radius, S.friction_static, S.friction_dynamic);
-        if (E.finish) {  // updateVelocity + updatePositions (particlesolver.cuh:71-81,121-137)
-            v_out[k] = f4((out - xp) * E.inv_dt, 0.0f);
-            const float d2 = length2(out - xp);
-            if (d2 >= E.sleep_eps2 || phase_of(a) < 0) x_io[k] = f4(out, x4.w);
+    sp.with_scene(k, [&](const SceneConst &S) {
+        const FluidParams &F = S.F;
+        const float4 a = xs[k];
+        f3 sum = xyz(sum_in[k]);
+        const unsigned long long cx = corr[3 * k], cy = corr[3 * k + 1], cz = corr[3 * k + 2];
+        if ((cx | cy | cz) != 0ull) {
+            const float inv = (float)(1.0 / kCorrScale);
+            sum.x += (float)(long long)cx * inv;
+            sum.y += (float)(long long)cy * inv;
+            sum.z += (float)(long long)cz * inv;
+            corr[3 * k] = 0ull;
+            corr[3 * k + 1] = 0ull;
+            corr[3 * k + 2] = 0ull;
         }
-    }
-    xs_out[k] = f4(out, a.w);
+        f3 out = xyz(a) + sum * F.inv_rest_density;
+        if (E.planes_next || E.finish) {
+            float4 x4 = x_io[k];
+            const f3 xp = xyz(x4);
+            if (E.planes_next) out = plane_collide_one(out, xp, S.pl, S.radius, S.friction_static, S.friction_dynamic);
+            if (E.finish) {  // updateVelocity + updatePositions (particlesolver.cuh:71-81,121-137)
+                v_out[k] = f4((out - xp) * E.inv_dt, 0.0f);
+                const float d2 = length2(out - xp);
+                if (d2 >= E.sleep_eps2 || phase_of(a) < 0) x_io[k] = f4(out, x4.w);
+            }
+        }
+        xs_out[k] = f4(out, a.w);
+    });
 }
 
 // fluidOmega (fluid.cuh:231-279) and fluidNormal (fluid.cuh:387-434) in one
@@ -211,36 +224,37 @@ __global__ __launch_bounds__(kMaxBlock) void k_post_a(const float4 *__restrict__
                                                    int do_normal) {
     const int i = xcd_index();
     if (i >= n) return;
-    const SceneConst &S = sp.at(i);
-    const GridDesc &G = S.G;
-    const SphConst &K = S.K;
-    const float4 a = xs[i];
-    const int ph = phase_of(a);
-    if (ph >= 0) return;
-    const f3 pi = xyz(a);
-    int g[3];
-    G.pos(pi, g);
-    ccell[i] = G.pack(g);
-    const f3 vi = xyz(v[i]);
-    f3 om = mk3(0.f, 0.f, 0.f), nn = mk3(0.f, 0.f, 0.f);
-    ScreenRec rec;  // replayed by the vorticity/tension and XSPH passes
-    rec.out = srec;
-    rec.slot = i;
-    rec.n = n;
-    for_each_neighbor(
-        G, ranges, g, xs, 0,
-        [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
-        [&](int j, float4 b) {
-            if (!(dist2(pi, xyz(b)) > 0.0f)) return;  // coincident partners are skipped
-            const f3 diff = pi - xyz(b);
-            const float r = fsqrt(dist2(pi, xyz(b)));
-            const f3 gr = K.grad_coef(r, frcp(r)) * diff;
-            if (do_omega) om += cross(xyz(v[j]) - vi, gr);
-            if (do_normal) nn += frcp(density[j]) * gr;
-        },
-        AllowNotSelf{i}, rec);
-    if (do_omega) omega[i] = f4(om, fsqrt(dot(om, om)));
-    if (do_normal) normal[i] = f4(K.h * nn, 0.0f);
+    sp.with_scene(i, [&](const SceneConst &S) {
+        const GridDesc &G = S.G;
+        const SphConst &K = S.K;
+        const float4 a = xs[i];
+        const int ph = phase_of(a);
+        if (ph >= 0) return;
+        const f3 pi = xyz(a);
+        int g[3];
+        G.pos(pi, g);
+        ccell[i] = G.pack(g);
+        const f3 vi = xyz(v[i]);
+        f3 om = mk3(0.f, 0.f, 0.f), nn = mk3(0.f, 0.f, 0.f);
+        ScreenRec rec;  // replayed by the vorticity/tension and XSPH passes
+        rec.out = srec;
+        rec.slot = i;
+        rec.n = n;
+        for_each_neighbor(
+            G, ranges, g, xs, 0,
+            [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
+            [&](int j, float4 b) {
+                if (!(dist2(pi, xyz(b)) > 0.0f)) return;  // coincident partners are skipped
+                const f3 diff = pi - xyz(b);
+                const float r = fsqrt(dist2(pi, xyz(b)));
+                const f3 gr = K.grad_coef(r, frcp(r)) * diff;
+                if (do_omega) om += cross(xyz(v[j]) - vi, gr);
+                if (do_normal) nn += frcp(density[j]) * gr;
+            },
+            AllowNotSelf{i}, rec);
+        if (do_omega) omega[i] = f4(om, fsqrt(dot(om, om)));
+        if (do_normal) normal[i] = f4(K.h * nn, 0.0f);
+    });
 }
 
 // fluidVorticity (fluid.cuh:281-336, in place) followed by fluidAkinciTension
@@ -257,59 +271,60 @@ __global__ __launch_bounds__(kMaxBlock) void k_post_b(const float4 *__restrict__
                                                    int do_akinci, float dt) {
     const int i = xcd_index();
     if (i >= n) return;
-    const SceneConst &S = sp.at(i);
-    const GridDesc &G = S.G;
-    const SphConst &K = S.K;
-    const FluidParams &F = S.F;
-    const float4 a = xs[i];
-    const int ph = phase_of(a);
-    if (ph >= 0) return;
-    const f3 pi = xyz(a);
-    int g[3];
-    unpack_cell(ccell[i], g);
-    f3 eta = mk3(0.f, 0.f, 0.f), f = mk3(0.f, 0.f, 0.f);
-    const f3 ni = do_akinci ? xyz(normal[i]) : mk3(0.f, 0.f, 0.f);
-    const float rhoi = do_akinci ? density[i] : 0.0f;
-    const float gam = F.tension_gamma;
-    ScreenRec rec;  // the omega/normal pass screened the same positions
-    rec.in = srec;
-    rec.slot = i;
-    rec.n = n;
-    struct Partner {
-        float4 b, nrm;
-        float om, rho;
-    };
-    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    for_each_neighbor_nb<kBodyBatch>(
-        G, ranges, g, xs, 0,
-        [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
-        [&](int j) {
-            return Partner{xs[j], do_akinci ? normal[j] : zero4, do_vort ? omega[j].w : 0.0f,
-                           do_akinci ? density[j] : 0.0f};
-        },
-        [&](int, const Partner &P) {
-            const float4 b = P.b;
-            if (!(dist2(pi, xyz(b)) > 0.0f)) return;  // coincident partners are skipped
-            const f3 diff = pi - xyz(b);
-            const float r = fsqrt(dist2(pi, xyz(b)));
-            const float ir = frcp(r);
-            if (do_vort) eta += (P.om * K.grad_coef(r, ir)) * diff;
-            if (do_akinci) {
-                // k_ij (-gamma C(r) diff / r - gamma (n_i - n_j)) (fluid.cuh:476-483)
-                const float kij = 2.0f * F.rest_density * frcp(rhoi + P.rho);
-                const f3 fcoh = (-gam * K.akinci(r) * ir) * diff;
-                const f3 fcur = -gam * (ni - xyz(P.nrm));
-                f += kij * (fcoh + fcur);
-            }
-        },
-        AllowNotSelf{i}, rec);
-    f3 vi = xyz(v[i]);
-    if (do_vort && length2(eta) > 1e-3f) {
-        const f3 nrm = eta * (1.0f / sqrtf(dot(eta, eta)));
-        vi += F.vorticity_eps * cross(nrm, xyz(omega[i])) * dt;
-    }
-    if (do_akinci) vi = vi + f * dt;
-    v[i] = f4(vi, 0.0f);
+    sp.with_scene(i, [&](const SceneConst &S) {
+        const GridDesc &G = S.G;
+        const SphConst &K = S.K;
+        const FluidParams &F = S.F;
+        const float4 a = xs[i];
+        const int ph = phase_of(a);
+        if (ph >= 0) return;
+        const f3 pi = xyz(a);
+        int g[3];
+        unpack_cell(ccell[i], g);
+        f3 eta = mk3(0.f, 0.f, 0.f), f = mk3(0.f, 0.f, 0.f);
+        const f3 ni = do_akinci ? xyz(normal[i]) : mk3(0.f, 0.f, 0.f);
+        const float rhoi = do_akinci ? density[i] : 0.0f;
+        const float gam = F.tension_gamma;
+        ScreenRec rec;  // the omega/normal pass screened the same positions
+        rec.in = srec;
+        rec.slot = i;
+        rec.n = n;
+        struct Partner {
+            float4 b, nrm;
+            float om, rho;
+        };
+        const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        for_each_neighbor_nb<kBodyBatch>(
+            G, ranges, g, xs, 0,
+            [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
+            [&](int j) {
+                return Partner{xs[j], do_akinci ? normal[j] : zero4, do_vort ? omega[j].w : 0.0f,
+                               do_akinci ? density[j] : 0.0f};
+            },
+            [&](int, const Partner &P) {
+                const float4 b = P.b;
+                if (!(dist2(pi, xyz(b)) > 0.0f)) return;  // coincident partners are skipped
+                const f3 diff = pi - xyz(b);
+                const float r = fsqrt(dist2(pi, xyz(b)));
+                const float ir = frcp(r);
+                if (do_vort) eta += (P.om * K.grad_coef(r, ir)) * diff;
+                if (do_akinci) {
+                    // k_ij (-gamma C(r) diff / r - gamma (n_i - n_j)) (fluid.cuh:476-483)
+                    const float kij = 2.0f * F.rest_density * frcp(rhoi + P.rho);
+                    const f3 fcoh = (-gam * K.akinci(r) * ir) * diff;
+                    const f3 fcur = -gam * (ni - xyz(P.nrm));
+                    f += kij * (fcoh + fcur);
+                }
+            },
+            AllowNotSelf{i}, rec);
+        f3 vi = xyz(v[i]);
+        if (do_vort && length2(eta) > 1e-3f) {
+            const f3 nrm = eta * (1.0f / sqrtf(dot(eta, eta)));
+            vi += F.vorticity_eps * cross(nrm, xyz(omega[i])) * dt;
+        }
+        if (do_akinci) vi = vi + f * dt;
+        v[i] = f4(vi, 0.0f);
+    });
 }
 
 // fluidXSph (fluid.cuh:338-385).  Non-fluid particles keep their velocity.
@@ -321,39 +336,40 @@ __global__ __launch_bounds__(kMaxBlock) void k_xsph(const float4 *__restrict__ x
                                                  int n, SP sp, int use_ccell) {
     const int i = xcd_index();
     if (i >= n) return;
-    const SceneConst &S = sp.at(i);
-    const GridDesc &G = S.G;
-    const SphConst &K = S.K;
-    const float c = S.F.xsph_c;
-    const float4 a = xs[i];
-    const int ph = phase_of(a);
-    const float4 v4 = v[i];
-    if (ph >= 0) {
-        v_out[i] = v4;
-        return;
-    }
-    const f3 pi = xyz(a);
-    int g[3];
-    if (use_ccell)
-        unpack_cell(ccell[i], g);
-    else
-        G.pos(pi, g);
-    const f3 vi = xyz(v4);
-    f3 vn = mk3(0.f, 0.f, 0.f);
-    ScreenRec rec;  // after the omega/normal pass: its screen of the same positions
-    rec.in = use_ccell ? srec : nullptr;
-    rec.slot = i;
-    rec.n = n;
-    struct Partner {
-        float4 b, v;
-    };
-    for_each_neighbor_nb<kBodyBatch>(
-        G, ranges, g, xs, 0,
-        [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
-        [&](int j) { return Partner{xs[j], v[j]}; },
-        [&](int, const Partner &P) { vn += (xyz(P.v) - vi) * K.poly6(dist2(pi, xyz(P.b))); }, AllowNotSelf{i},
-        rec);
-    v_out[i] = f4(vi + c * vn, 0.0f);
+    sp.with_scene(i, [&](const SceneConst &S) {
+        const GridDesc &G = S.G;
+        const SphConst &K = S.K;
+        const float c = S.F.xsph_c;
+        const float4 a = xs[i];
+        const int ph = phase_of(a);
+        const float4 v4 = v[i];
+        if (ph >= 0) {
+            v_out[i] = v4;
+            return;
+        }
+        const f3 pi = xyz(a);
+        int g[3];
+        if (use_ccell)
+            unpack_cell(ccell[i], g);
+        else
+            G.pos(pi, g);
+        const f3 vi = xyz(v4);
+        f3 vn = mk3(0.f, 0.f, 0.f);
+        ScreenRec rec;  // after the omega/normal pass: its screen of the same positions
+        rec.in = use_ccell ? srec : nullptr;
+        rec.slot = i;
+        rec.n = n;
+        struct Partner {
+            float4 b, v;
+        };
+        for_each_neighbor_nb<kBodyBatch>(
+            G, ranges, g, xs, 0,
+            [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
+            [&](int j) { return Partner{xs[j], v[j]}; },
+            [&](int, const Partner &P) { vn += (xyz(P.v) - vi) * K.poly6(dist2(pi, xyz(P.b))); }, AllowNotSelf{i},
+            rec);
+        v_out[i] = f4(vi + c * vn, 0.0f);
+    });
 }
 
 // ------------------------------------------------------------ launchers
@@ -365,11 +381,11 @@ void launch_lambda(const float4 *xs, const float *m, const int2 *ranges, float *
 }
 template <class SP>
 void launch_position(const float4 *xs, const float *m, const float *lambda, const uint32_t *ccell,
-                     const int *build_keys, const int2 *ranges, const uint32_t *srec, float4 *sum_tmp,
-                     unsigned long long *corr, float4 *xs_out, float4 *x_io, float4 *v_out, int n, const SP &sp,
-                     const Epilogue &E, int block, hipStream_t st) {
+                     const int *build_keys, const int2 *ranges, const uint32_t *srec, const uint32_t *fluid_stamp,
+                     const uint32_t *stamp, float4 *sum_tmp, unsigned long long *corr, float4 *xs_out, float4 *x_io,
+                     float4 *v_out, int n, const SP &sp, const Epilogue &E, int block, hipStream_t st) {
     hipLaunchKernelGGL(k_position<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, m, lambda, ccell,
-                       build_keys, ranges, srec, sum_tmp, corr, n, sp);
+                       build_keys, ranges, srec, fluid_stamp, stamp, sum_tmp, corr, n, sp);
     hipLaunchKernelGGL(k_position_apply<SP>, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, sum_tmp, corr, xs_out,
                        x_io, v_out, n, sp, E);
 }
@@ -398,9 +414,9 @@ void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const ui
     template void launch_lambda<SP>(const float4 *, const float *, const int2 *, float *, float *, uint32_t *,  \
                                     uint32_t *, int, const SP &, int, hipStream_t);                             \
     template void launch_position<SP>(const float4 *, const float *, const float *, const uint32_t *,           \
-                                      const int *, const int2 *, const uint32_t *, float4 *,                    \
-                                      unsigned long long *, float4 *, float4 *, float4 *, int, const SP &,      \
-                                      const Epilogue &, int, hipStream_t);                                      \
+                                      const int *, const int2 *, const uint32_t *, const uint32_t *,            \
+                                      const uint32_t *, float4 *, unsigned long long *, float4 *, float4 *,     \
+                                      float4 *, int, const SP &, const Epilogue &, int, hipStream_t);           \
     template void launch_post_a<SP>(const float4 *, const float4 *, const float *, const int2 *, float4 *,      \
                                     float4 *, uint32_t *, uint32_t *, int, const SP &, int, int, int,           \
                                     hipStream_t);                                                               \

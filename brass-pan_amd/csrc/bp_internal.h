@@ -161,14 +161,45 @@ struct SceneConst {
 
 // Parameter sources: one scene by value (plain solver), or a per-particle
 // lookup in a scene table (batched solver: scenes concatenated slot-wise).
+// with_scene(slot, f) runs f(scene of slot); in a scene table it runs once per
+// distinct scene among the wave's active lanes (scenes are contiguous slot
+// ranges, so nearly always once), with that scene's id made wave-uniform so
+// its constants are scalar loads rather than one load per lane.
 struct OneScene {
     SceneConst c;
     __device__ __forceinline__ const SceneConst &at(int) const { return c; }
+    template <class F> __device__ __forceinline__ void with_scene(int, F &&f) const { f(c); }
 };
 struct SceneTable {
     const SceneConst *__restrict__ table;
     const int *__restrict__ sid;  // slot -> scene
     __device__ __forceinline__ const SceneConst &at(int slot) const { return table[sid[slot]]; }
+    template <class F> __device__ __forceinline__ void with_scene(int slot, F &&f) const {
+        const int s = sid[slot];
+        // a wave-uniform loop over the distinct scenes of the active lanes
+        uint64_t todo = __builtin_amdgcn_ballot_w64(true);
+        while (todo) {
+            const int s0 = __builtin_amdgcn_readlane(s, __builtin_ctzll(todo));
+            // an opaque copy of s0 in a scalar register: inside the branch
+            // below the compiler would otherwise substitute the per-lane s
+            int u = s0;
+            asm volatile("" : "+s"(u));
+            const bool mine = s == s0;
+            if (mine) {
+                // the table is read-only for the whole launch: read the entry
+                // through the constant address space, as scalar loads
+                typedef const __attribute__((address_space(4))) uint32_t cword;
+                static_assert(sizeof(SceneConst) % 4 == 0, "scene records are whole words");
+                const cword *src = (const cword *)(table + u);
+                SceneConst sc;
+                uint32_t *dst = reinterpret_cast<uint32_t *>(&sc);
+#pragma unroll
+                for (int w = 0; w < (int)(sizeof(SceneConst) / 4); ++w) dst[w] = src[w];
+                f(sc);
+            }
+            todo &= ~__builtin_amdgcn_ballot_w64(mine);
+        }
+    }
 };
 
 // ------------------------------------------------------------ device helpers
@@ -204,35 +235,48 @@ __device__ __forceinline__ f3 plane_collide_one(f3 xs, f3 pos, const Planes &pl,
 }
 
 // Iterates the reference's 27-cell candidate set (fluid.cuh:104-115) around
-// grid position g: each of the 9 (x,y) columns is one or two runs of
-// consecutive cell addresses, whose particles are consecutive slots.
+// grid position g: the 9 (x,y) columns in order, each visiting the cells
+// z-1, z, z+1 (wrapped) in that order.  Their particles are consecutive slots
+// except across the wrap, so a column is one run of slots, or two when it
+// wraps.  The next column's three cell ranges are loaded while the current
+// one is walked.  (Grid dimensions are at least 4, so the cells differ.)
 template <class F>
 __device__ __forceinline__ void for_each_run(const GridDesc &G, const int2 *__restrict__ ranges, const int g[3],
                                              F &&f) {
-    const uint32_t z0 = (uint32_t)(g[2] - 1) & G.mask[2];
-    const uint32_t z2 = (uint32_t)(g[2] + 1) & G.mask[2];
-    const bool wrap = z0 > z2;
-    for (int dy = -1; dy <= 1; ++dy) {
-        for (int dx = -1; dx <= 1; ++dx) {
-            const uint32_t col = G.base + ((uint32_t)(g[0] + dx) & G.mask[0]) * G.stride_x +
-                                 ((uint32_t)(g[1] + dy) & G.mask[1]) * (uint32_t)G.dims[2];
-            uint32_t a = z0, b = wrap ? G.mask[2] : z2;
+    const uint32_t zc[3] = {(uint32_t)(g[2] - 1) & G.mask[2], (uint32_t)g[2] & G.mask[2],
+                            (uint32_t)(g[2] + 1) & G.mask[2]};
+    const bool brk01 = zc[1] != zc[0] + 1u, brk12 = zc[2] != zc[1] + 1u;  // the wrap point, if any
+    auto column = [&](int ci) {
+        const int dy = ci / 3 - 1, dx = ci % 3 - 1;
+        return G.base + ((uint32_t)(g[0] + dx) & G.mask[0]) * G.stride_x +
+               ((uint32_t)(g[1] + dy) & G.mask[1]) * (uint32_t)G.dims[2];
+    };
+    int2 nr[3];
+    {
+        const uint32_t col = column(0);
 #pragma unroll
-            for (int part = 0; part < 2; ++part) {
-                int s = -1, e = -1;
-                for (uint32_t c = a; c <= b; ++c) {
-                    int2 r = ranges[col + c];
-                    if (r.x >= 0) {
-                        if (s < 0) s = r.x;
-                        e = r.y;
-                    }
-                }
+        for (int q = 0; q < 3; ++q) nr[q] = ranges[col + zc[q]];
+    }
+    for (int ci = 0; ci < 9; ++ci) {
+        int2 r[3] = {nr[0], nr[1], nr[2]};
+        if (ci + 1 < 9) {
+            const uint32_t col = column(ci + 1);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) nr[q] = ranges[col + zc[q]];
+        }
+        int s = -1, e = -1;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            if ((q == 1 && brk01) || (q == 2 && brk12)) {
                 if (s >= 0) f(s, e);
-                if (!wrap) break;
-                a = 0;
-                b = z2;
+                s = -1;
+            }
+            if (r[q].x >= 0) {
+                if (s < 0) s = r[q].x;
+                e = r[q].y;
             }
         }
+        if (s >= 0) f(s, e);
     }
 }
 
@@ -267,6 +311,21 @@ struct ScreenRec {
     int slot = 0, n = 0;
 };
 
+// True iff one of the 27 cells around g carries stamp s in `table` (a cell
+// table filled once per substep from the grid build's keys).
+__device__ __forceinline__ bool stamp_near(const GridDesc &G, const int g[3], const uint32_t *__restrict__ table,
+                                           uint32_t s) {
+    bool any = false;
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+            const uint32_t col = G.base + ((uint32_t)(g[0] + dx) & G.mask[0]) * G.stride_x +
+                                 ((uint32_t)(g[1] + dy) & G.mask[1]) * (uint32_t)G.dims[2];
+#pragma unroll
+            for (int dz = -1; dz <= 1; ++dz) any |= table[col + ((uint32_t)(g[2] + dz) & G.mask[2])] == s;
+        }
+    return any;
+}
+
 // Two-phase neighbour walk: for every run, chunks of up to 32 candidates are
 // first screened with the cheap test `pick(j, b)` (b = xs[j]) into a bitmask,
 // which `allow(base)` then restricts by slot index; `consume(base, mask)`
@@ -282,6 +341,8 @@ __device__ __forceinline__ void walk_chunks(const GridDesc &G, const int2 *__res
                                             const float4 *__restrict__ xs, int jmin, Pick &&pick, Allow &&allow,
                                             ScreenRec rec, Consume &&consume) {
     int c = 0;  // chunk number in walk order
+    // a replayed chunk's mask is loaded one chunk ahead
+    uint32_t ahead = rec.in ? rec.in[rec.slot] : 0u;
     for_each_run(G, ranges, g, [&](int s, int e) {
         if (s < jmin) s = jmin;
         for (int base = s; base < e; base += 32, ++c) {
@@ -290,8 +351,9 @@ __device__ __forceinline__ void walk_chunks(const GridDesc &G, const int2 *__res
             const uint32_t keep = allow(base) & valid;
             uint32_t mask = 0u;
             if (rec.in && c < kRecChunks) {
+                mask = ahead;
+                if (c + 1 < kRecChunks) ahead = rec.in[(size_t)(c + 1) * rec.n + rec.slot];
                 if (keep == 0u) continue;
-                mask = rec.in[(size_t)c * rec.n + rec.slot];
             } else {
                 if (keep == 0u && !rec.out) continue;
                 for (int t0 = 0; t0 < cnt; t0 += 8) {

@@ -30,9 +30,9 @@ void launch_lambda(const float4 *xs, const float *m, const int2 *ranges, float *
                    uint32_t *ccell, uint32_t *srec, int n, const SP &sp, int block, hipStream_t st);
 template <class SP>
 void launch_position(const float4 *xs, const float *m, const float *lambda, const uint32_t *ccell,
-                     const int *build_keys, const int2 *ranges, const uint32_t *srec, float4 *sum_tmp,
-                     unsigned long long *corr, float4 *xs_out, float4 *x_io, float4 *v_out, int n, const SP &sp,
-                     const Epilogue &E, int block, hipStream_t st);
+                     const int *build_keys, const int2 *ranges, const uint32_t *srec, const uint32_t *fluid_stamp,
+                     const uint32_t *stamp, float4 *sum_tmp, unsigned long long *corr, float4 *xs_out, float4 *x_io,
+                     float4 *v_out, int n, const SP &sp, const Epilogue &E, int block, hipStream_t st);
 template <class SP>
 void launch_post_a(const float4 *xs, const float4 *v, const float *density, const int2 *ranges, float4 *omega,
                    float4 *normal, uint32_t *ccell, uint32_t *srec, int n, const SP &sp, int do_omega, int do_normal,
@@ -52,9 +52,10 @@ template <class SP>
 void launch_particle_particle(const float4 *xs, const float4 *x, const int2 *ranges, float4 *xs_out, int n,
                               const SP &sp, int block, int *contacts, const uint32_t *cell_stamp,
                               const uint32_t *stamp, hipStream_t st);
-// advances the substep stamp and marks the cells holding a solid
-void launch_mark_solid_cells(const int *sorted_keys, const int *ph, int n, uint32_t *stamp, uint32_t *cell_stamp,
-                             hipStream_t st);
+// advances the substep stamp and marks the cells holding a solid (cell_stamp)
+// and those holding a fluid particle (fluid_stamp)
+void launch_mark_cells(const int *sorted_keys, const int *ph, int n, uint32_t *stamp, uint32_t *cell_stamp,
+                       uint32_t *fluid_stamp, hipStream_t st);
 void launch_distance(float4 *xs, const float4 *x, const int2 *pairs, const float2 *params, const int *map_to_new,
                      float4 *kbuf, int nc, const int *verts, const int *offs, const int *ents, int nv,
                      hipStream_t st);

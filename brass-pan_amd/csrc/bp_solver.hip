@@ -185,7 +185,8 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
     s->ph.flip();
     s->orig.flip();
     if (s->sid.b[0]) s->sid.flip();
-    if (s->cell_stamp) TIMED(F_PP, launch_mark_solid_cells(sk, s->ph.cur(), N, s->stamp, s->cell_stamp, st));
+    if (s->cell_stamp)
+        TIMED(F_PP, launch_mark_cells(sk, s->ph.cur(), N, s->stamp, s->cell_stamp, s->fluid_stamp, st));
     HIP_TRY(snap(H.x1, s->x.cur(), N, st));
     HIP_TRY(snap(H.xsw, s->xs.cur(), N, st));
     HIP_TRY(snap(H.m, s->m.cur(), N, st));
@@ -219,7 +220,7 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
             E.inv_dt = inv_dt;
             E.sleep_eps2 = P.sleep_epsilon * P.sleep_epsilon;
             TIMED(F_POSITION, launch_position(s->xs.cur(), s->m.cur(), s->lambda, s->ccell, sk, s->ranges,
-                                              s->srec, s->sum_tmp, s->corr, s->xs.alt(), s->x.cur(), s->v.cur(), N, sp(), E,
+                                              s->srec, s->fluid_stamp, s->stamp, s->sum_tmp, s->corr, s->xs.alt(), s->x.cur(), s->v.cur(), N, sp(), E,
                                               s->blk_position, st));
             s->xs.flip();
             planes_done = E.planes_next;
@@ -526,8 +527,10 @@ bp_status create_impl(const bp_scene_desc *scs, const bp_params *params, int ns,
     TRY(s->alloc(&s->normal, N));
     if (s->has_solid && s->has_fluid) {
         TRY(s->alloc(&s->cell_stamp, s->num_cells));
+        TRY(s->alloc(&s->fluid_stamp, s->num_cells));
         TRY(s->alloc(&s->stamp, 1));
         HTRY(hipMemsetAsync(s->cell_stamp, 0, (size_t)s->num_cells * sizeof(uint32_t), s->st));
+        HTRY(hipMemsetAsync(s->fluid_stamp, 0, (size_t)s->num_cells * sizeof(uint32_t), s->st));
         HTRY(hipMemsetAsync(s->stamp, 0, sizeof(uint32_t), s->st));
     }
     if (ns > 1) TRY(s->upload(&s->dsc, s->hsc.data(), s->hsc.size()));

@@ -84,6 +84,7 @@ struct bp_solver {
     unsigned long long *corr = nullptr;  // fixed-point pair corrections, kept zero between passes
     float4 *omega = nullptr, *normal = nullptr;
     uint32_t *cell_stamp = nullptr, *stamp = nullptr;  // cells holding a solid (fluid + solid scenes)
+    uint32_t *fluid_stamp = nullptr;                    // cells holding a fluid particle (same scenes)
 
     int nd = 0;
     int2 *dpairs = nullptr;
