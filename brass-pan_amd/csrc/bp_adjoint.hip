@@ -208,52 +208,53 @@ __global__ __launch_bounds__(kMaxBlock) void ka_xsph(const float4 *__restrict__ 
                                                      unsigned long long *__restrict__ corr, int n, SP sp) {
     const int i = xcd_index();
     if (i >= n) return;
-    const SceneConst &S = sp.at(i);
-    const GridDesc &G = S.G;
-    const SphConst &K = S.K;
-    const float c = S.F.xsph_c;
-    const float4 a = xf[i];
-    const int ph = phase_of(a);
-    const f3 yi = xyz(gv3[i]);
-    if (ph >= 0) {
-        gv2_own[i] = f4(yi, 0.f);
-        gx_own[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        return;
-    }
-    const f3 pi = xyz(a);
-    const f3 v2i = (pi - xyz(x1[i])) * inv_dt;
-    const uint32_t bi = G.pack_addr(keys[i]);
-    int g[3];
-    unpack_cell(ccell[i], g);
-    f3 gv = mk3(0.f, 0.f, 0.f), gx = mk3(0.f, 0.f, 0.f), vn = mk3(0.f, 0.f, 0.f);
-    float wsum = 0.0f;
-    for_each_neighbor(
-        G, ranges, g, xf, 0,
-        [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
-        [&](int j, float4 b) {
-            const f3 pj = xyz(b);
-            const float d2 = dist2(pi, pj);
-            const f3 diff = pi - pj;
-            const float W = K.poly6(d2);
-            const float Wp2 = 2.0f * poly6_d(K, d2);
-            const f3 dv = (pj - xyz(x1[j])) * inv_dt - v2i;
-            vn += dv * W;
-            wsum += W;
-            const float sa = c * dot(yi, dv) * Wp2;
-            gx += sa * diff;
-            if (G.in_block(ccell[j], bi)) {  // j's equation contains i
-                const f3 yj = xyz(gv3[j]);
-                gv += (c * W) * yj;
-                gx += (c * dot(yj, -dv) * Wp2) * diff;
-            } else {  // j never sees this pair: deliver i's terms for j
-                deliver3(corr, j, 0, (c * W) * yi);
-                deliver3(corr, j, 3, -sa * diff);
-            }
-        },
-        AllowNotSelf{i});
-    gv2_own[i] = f4(yi - (c * wsum) * yi + gv, 0.f);
-    gx_own[i] = f4(gx, 0.f);
-    cacc[i] += dot(yi, vn);
+    sp.with_scene(i, [&](const SceneConst &S) {
+        const GridDesc &G = S.G;
+        const SphConst &K = S.K;
+        const float c = S.F.xsph_c;
+        const float4 a = xf[i];
+        const int ph = phase_of(a);
+        const f3 yi = xyz(gv3[i]);
+        if (ph >= 0) {
+            gv2_own[i] = f4(yi, 0.f);
+            gx_own[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            return;
+        }
+        const f3 pi = xyz(a);
+        const f3 v2i = (pi - xyz(x1[i])) * inv_dt;
+        const uint32_t bi = G.pack_addr(keys[i]);
+        int g[3];
+        unpack_cell(ccell[i], g);
+        f3 gv = mk3(0.f, 0.f, 0.f), gx = mk3(0.f, 0.f, 0.f), vn = mk3(0.f, 0.f, 0.f);
+        float wsum = 0.0f;
+        for_each_neighbor(
+            G, ranges, g, xf, 0,
+            [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
+            [&](int j, float4 b) {
+                const f3 pj = xyz(b);
+                const float d2 = dist2(pi, pj);
+                const f3 diff = pi - pj;
+                const float W = K.poly6(d2);
+                const float Wp2 = 2.0f * poly6_d(K, d2);
+                const f3 dv = (pj - xyz(x1[j])) * inv_dt - v2i;
+                vn += dv * W;
+                wsum += W;
+                const float sa = c * dot(yi, dv) * Wp2;
+                gx += sa * diff;
+                if (G.in_block(ccell[j], bi)) {  // j's equation contains i
+                    const f3 yj = xyz(gv3[j]);
+                    gv += (c * W) * yj;
+                    gx += (c * dot(yj, -dv) * Wp2) * diff;
+                } else {  // j never sees this pair: deliver i's terms for j
+                    deliver3(corr, j, 0, (c * W) * yi);
+                    deliver3(corr, j, 3, -sa * diff);
+                }
+            },
+            AllowNotSelf{i});
+        gv2_own[i] = f4(yi - (c * wsum) * yi + gv, 0.f);
+        gx_own[i] = f4(gx, 0.f);
+        cacc[i] += dot(yi, vn);
+    });
 }
 
 // Completes the XSPH adjoint and takes it through updateVelocity +
@@ -295,50 +296,64 @@ __global__ __launch_bounds__(kMaxBlock) void ka_position(const float4 *__restric
                                                          const int2 *__restrict__ ranges,
                                                          const float4 *__restrict__ gpost, float4 *__restrict__ gx_own,
                                                          float *__restrict__ glam_own,
-                                                         unsigned long long *__restrict__ corr, int n, SP sp) {
+                                                         unsigned long long *__restrict__ corr,
+                                                         uint32_t *__restrict__ srec, int n, SP sp) {
     const int k = xcd_index();
     if (k >= n) return;
-    const SceneConst &S = sp.at(k);
-    const GridDesc &G = S.G;
-    const SphConst &K = S.K;
-    const FluidParams &F = S.F;
-    const float4 a = xs[k];
-    const f3 pk = xyz(a);
-    const bool kf = phase_of(a) <= 0;
-    const float lk = lambda[k];
-    const float mk = m[k];
-    const uint32_t bk = G.pack_addr(keys[k]);
-    const float ir0 = F.inv_rest_density;
-    const f3 yk = xyz(gpost[k]) * ir0;
-    int g[3];
-    unpack_cell(ccell[k], g);
-    f3 gx = mk3(0.f, 0.f, 0.f);
-    float gl = 0.0f;
-    for_each_neighbor(
-        G, ranges, g, xs, 0,
-        [&](int, float4 b) { return dist2(pk, xyz(b)) <= K.h2; },
-        [&](int j, float4 b) {
-            // coincident partners are skipped; a solid partner's pair is k's only if j > k
-            if (!(dist2(pk, xyz(b)) > 0.0f) | ((phase_of(b) > 0) & !(kf & (j > k)))) return;
-            const bool partner_sees_k = G.in_block(ccell[j], bk);
-            if (j < k && !partner_sees_k) return;
-            const f3 diff = pk - xyz(b);
-            float c, dc_dd2, dc_dls;
-            pair_coef_grad(dist2(pk, xyz(b)), lk + lambda[j], K, F, c, dc_dd2, dc_dls);
-            const f3 u = m[j] * yk - mk * (xyz(gpost[j]) * ir0);
-            const float cbar = dot(u, diff);
-            const f3 gdiff = c * u + (2.0f * cbar * dc_dd2) * diff;
-            const float gls = cbar * dc_dls;
-            gx += gdiff;
-            gl += gls;
-            if (j > k && !partner_sees_k) {
-                deliver3(corr, j, 0, -gdiff);
-                atomicAdd(&corr[kCorrStride * j + 3], afix(gls));
-            }
-        },
-        [&](int base) { return kf ? ~chunk_bit(k, base) : chunk_below(k, base); });
-    gx_own[k] = f4(gx, 0.f);
-    glam_own[k] = gl;
+    sp.with_scene(k, [&](const SceneConst &S) {
+        const GridDesc &G = S.G;
+        const SphConst &K = S.K;
+        const FluidParams &F = S.F;
+        const float4 a = xs[k];
+        const f3 pk = xyz(a);
+        const bool kf = phase_of(a) <= 0;
+        const float lk = lambda[k];
+        const float mk = m[k];
+        const uint32_t bk = G.pack_addr(keys[k]);
+        const float ir0 = F.inv_rest_density;
+        const f3 yk = xyz(gpost[k]) * ir0;
+        int g[3];
+        unpack_cell(ccell[k], g);
+        f3 gx = mk3(0.f, 0.f, 0.f);
+        float gl = 0.0f;
+        struct Partner {
+            float4 b, gp;
+            uint32_t cell;
+            float lam, m;
+        };
+        ScreenRec rec;  // replayed by the density adjoint of the same x*
+        rec.out = srec;
+        rec.slot = k;
+        rec.n = n;
+        for_each_neighbor_nb<kBodyBatch>(
+            G, ranges, g, xs, 0,
+            // a solid screens for fluid partners only
+            [&](int, float4 b) { return (dist2(pk, xyz(b)) <= K.h2) & (kf | (phase_of(b) <= 0)); },
+            [&](int j) { return Partner{xs[j], gpost[j], ccell[j], lambda[j], m[j]}; },
+            [&](int j, const Partner &P) {
+                const float4 b = P.b;
+                // coincident partners are skipped; a solid partner's pair is k's only if j > k
+                if (!(dist2(pk, xyz(b)) > 0.0f) | ((phase_of(b) > 0) & !(kf & (j > k)))) return;
+                const bool partner_sees_k = G.in_block(P.cell, bk);
+                if (j < k && !partner_sees_k) return;
+                const f3 diff = pk - xyz(b);
+                float c, dc_dd2, dc_dls;
+                pair_coef_grad(dist2(pk, xyz(b)), lk + P.lam, K, F, c, dc_dd2, dc_dls);
+                const f3 u = P.m * yk - mk * (xyz(P.gp) * ir0);
+                const float cbar = dot(u, diff);
+                const f3 gdiff = c * u + (2.0f * cbar * dc_dd2) * diff;
+                const float gls = cbar * dc_dls;
+                gx += gdiff;
+                gl += gls;
+                if (j > k && !partner_sees_k) {
+                    deliver3(corr, j, 0, -gdiff);
+                    atomicAdd(&corr[kCorrStride * j + 3], afix(gls));
+                }
+            },
+            [&](int base) { return kf ? ~chunk_bit(k, base) : chunk_below(k, base); }, rec);
+        gx_own[k] = f4(gx, 0.f);
+        glam_own[k] = gl;
+    });
 }
 
 // x* adjoint before the correction = after + own + delivered terms; lambda
@@ -369,43 +384,56 @@ __global__ __launch_bounds__(kMaxBlock) void ka_lambda(const float4 *__restrict_
                                                        const uint32_t *__restrict__ ccell,
                                                        const int *__restrict__ keys, const int2 *__restrict__ ranges,
                                                        float4 *__restrict__ gx_own,
-                                                       unsigned long long *__restrict__ corr, int n, SP sp) {
+                                                       unsigned long long *__restrict__ corr,
+                                                       const uint32_t *__restrict__ srec, int n, SP sp) {
     const int k = xcd_index();
     if (k >= n) return;
-    const SceneConst &S = sp.at(k);
-    const GridDesc &G = S.G;
-    const SphConst &K = S.K;
-    const float sds = S.F.solid_density_scale;
-    const float4 a = xs[k];
-    const f3 pk = xyz(a);
-    const bool kf = phase_of(a) <= 0;
-    const float rk = grho[k];
-    const float mk = m[k];
-    const float sk = (phase_of(a) < 0) ? 1.0f : sds;
-    const uint32_t bk = G.pack_addr(keys[k]);
-    int g[3];
-    unpack_cell(ccell[k], g);
-    f3 gx = mk3(0.f, 0.f, 0.f);
-    for_each_neighbor(
-        G, ranges, g, xs, 0,
-        [&](int, float4 b) { return dist2(pk, xyz(b)) <= K.h2; },
-        [&](int j, float4 b) {
-            // coincident partners are skipped; a solid partner's pair is k's only if j > k
-            if (!(dist2(pk, xyz(b)) > 0.0f) | ((phase_of(b) > 0) & !(kf & (j > k)))) return;
-            const bool partner_sees_k = G.in_block(ccell[j], bk);
-            const f3 diff = pk - xyz(b);
-            const float Wp2 = 2.0f * poly6_d(K, dist2(pk, xyz(b)));
-            if (j > k) {  // pair of rho_k
-                const float sj = (phase_of(b) < 0) ? 1.0f : sds;
-                const f3 t = (rk * sj * m[j] * Wp2) * diff;
-                gx += t;
-                if (!partner_sees_k) deliver3(corr, j, 0, -t);
-            } else if (partner_sees_k) {  // pair of rho_j
-                gx += (grho[j] * sk * mk * Wp2) * diff;
-            }
-        },
-        [&](int base) { return kf ? ~chunk_bit(k, base) : chunk_below(k, base); });
-    gx_own[k] = f4(gx, 0.f);
+    sp.with_scene(k, [&](const SceneConst &S) {
+        const GridDesc &G = S.G;
+        const SphConst &K = S.K;
+        const float sds = S.F.solid_density_scale;
+        const float4 a = xs[k];
+        const f3 pk = xyz(a);
+        const bool kf = phase_of(a) <= 0;
+        const float rk = grho[k];
+        const float mk = m[k];
+        const float sk = (phase_of(a) < 0) ? 1.0f : sds;
+        const uint32_t bk = G.pack_addr(keys[k]);
+        int g[3];
+        unpack_cell(ccell[k], g);
+        f3 gx = mk3(0.f, 0.f, 0.f);
+        struct Partner {
+            float4 b;
+            uint32_t cell;
+            float m, grho;
+        };
+        ScreenRec rec;  // the position adjoint screened this x* with the same test
+        rec.in = srec;
+        rec.slot = k;
+        rec.n = n;
+        for_each_neighbor_nb<kBodyBatch>(
+            G, ranges, g, xs, 0,
+            [&](int, float4 b) { return (dist2(pk, xyz(b)) <= K.h2) & (kf | (phase_of(b) <= 0)); },
+            [&](int j) { return Partner{xs[j], ccell[j], m[j], grho[j]}; },
+            [&](int j, const Partner &P) {
+                const float4 b = P.b;
+                // coincident partners are skipped; a solid partner's pair is k's only if j > k
+                if (!(dist2(pk, xyz(b)) > 0.0f) | ((phase_of(b) > 0) & !(kf & (j > k)))) return;
+                const bool partner_sees_k = G.in_block(P.cell, bk);
+                const f3 diff = pk - xyz(b);
+                const float Wp2 = 2.0f * poly6_d(K, dist2(pk, xyz(b)));
+                if (j > k) {  // pair of rho_k
+                    const float sj = (phase_of(b) < 0) ? 1.0f : sds;
+                    const f3 t = (rk * sj * P.m * Wp2) * diff;
+                    gx += t;
+                    if (!partner_sees_k) deliver3(corr, j, 0, -t);
+                } else if (partner_sees_k) {  // pair of rho_j
+                    gx += (P.grho * sk * mk * Wp2) * diff;
+                }
+            },
+            [&](int base) { return kf ? ~chunk_bit(k, base) : chunk_below(k, base); }, rec);
+        gx_own[k] = f4(gx, 0.f);
+    });
 }
 
 __global__ __launch_bounds__(kBlock) void ka_add_own(float4 *__restrict__ gxs, const float4 *__restrict__ gx_own,
@@ -423,22 +451,24 @@ __global__ __launch_bounds__(kBlock) void ka_planes(const float4 *__restrict__ x
                                                     float4 *__restrict__ gxs, float4 *__restrict__ gx1, int n, SP sp) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const SceneConst &S = sp.at(k);
-    const Planes &pl = S.pl;
-    if (pl.n == 0) return;
-    const f3 x0 = xyz(xp[k]), pos = xyz(x1[k]);
-    f3 y = xyz(gxs[k]), gpos = xyz(gx1[k]);
-    for (int q = pl.n - 1; q >= 0; --q) {
-        f3 xin = x0;
-        for (int u = 0; u < q; ++u)
-            xin = plane_fwd(xin, pos, mk3(pl.o[u][0], pl.o[u][1], pl.o[u][2]),
-                            mk3(pl.nrm[u][0], pl.nrm[u][1], pl.nrm[u][2]), S.radius, S.friction_static,
-                            S.friction_dynamic);
-        y = plane_adj(xin, pos, mk3(pl.o[q][0], pl.o[q][1], pl.o[q][2]), mk3(pl.nrm[q][0], pl.nrm[q][1], pl.nrm[q][2]),
-                      S.radius, S.friction_static, S.friction_dynamic, y, &gpos);
-    }
-    gxs[k] = f4(y, 0.f);
-    gx1[k] = f4(gpos, 0.f);
+    sp.with_scene(k, [&](const SceneConst &S) {
+        const Planes &pl = S.pl;
+        if (pl.n == 0) return;
+        const f3 x0 = xyz(xp[k]), pos = xyz(x1[k]);
+        f3 y = xyz(gxs[k]), gpos = xyz(gx1[k]);
+        for (int q = pl.n - 1; q >= 0; --q) {
+            f3 xin = x0;
+            for (int u = 0; u < q; ++u)
+                xin = plane_fwd(xin, pos, mk3(pl.o[u][0], pl.o[u][1], pl.o[u][2]),
+                                mk3(pl.nrm[u][0], pl.nrm[u][1], pl.nrm[u][2]), S.radius, S.friction_static,
+                                S.friction_dynamic);
+            y = plane_adj(xin, pos, mk3(pl.o[q][0], pl.o[q][1], pl.o[q][2]),
+                          mk3(pl.nrm[q][0], pl.nrm[q][1], pl.nrm[q][2]), S.radius, S.friction_static,
+                          S.friction_dynamic, y, &gpos);
+        }
+        gxs[k] = f4(y, 0.f);
+        gx1[k] = f4(gpos, 0.f);
+    });
 }
 
 // ------------------------------------------------------------ pins, distance
@@ -534,60 +564,61 @@ __global__ __launch_bounds__(kBlock) void ka_wind_face(const float4 *__restrict_
     if (f >= nf) return;
     const int4 ids = faces[f];
     const int id1 = map_to_new[ids.x], id2 = map_to_new[ids.y], id3 = map_to_new[ids.z];
-    const SceneConst &S = sp.at(id1);
-    const float cd = S.drag_coeff, cl = S.lift_coeff;
-    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    fbar[4 * f] = z4;
-    fbar[4 * f + 1] = z4;
-    fbar[4 * f + 2] = z4;
-    fbar[4 * f + 3] = z4;
-    const f3 p1 = xyz(xsw[id1]), p2 = xyz(xsw[id2]), p3 = xyz(xsw[id3]);
-    const f3 a = p2 - p1, b = p3 - p1;
-    f3 vel = (xyz(v_in[perm[id1]]) + xyz(v_in[perm[id2]]) + xyz(v_in[perm[id3]])) / 3.0f;
-    vel = vel - S.wind;
-    const float lv = length(vel);
-    if (lv <= 1e-5f) return;
-    const f3 vh = vel / lv;
-    const f3 fr = cross(a, b);
-    const float lfn = length(fr);
-    if (lfn <= 1e-5f) return;
-    const f3 fu = fr / lfn;
-    const float sg = (dot(fu, vel) > 0.0f) ? 1.0f : -1.0f;
-    const f3 fn = sg * fu;
-    const float area = 0.5f * lfn;
-    const float nv = dot(fn, vh);
-    const float A = lv * lv * area * nv;
-    const f3 wv = cross(fn, vh);
-    const f3 uh = cross(wv, vh);
-    // adjoint of the force
-    const f3 F = dt * dt * (xyz(gxs[id1]) / m[id1] + xyz(gxs[id2]) / m[id2] + xyz(gxs[id3]) / m[id3]);
-    const float gcd = -A * dot(F, vh);
-    const float gcl = A * dot(F, uh);
-    const float gA = dot(F, -cd * vh + cl * uh);
-    f3 gvh = (-cd * A) * F;
-    const f3 gu = (cl * A) * F;
-    const f3 gwv = cross(vh, gu);
-    gvh += cross(gu, wv);
-    f3 gfn = cross(vh, gwv);
-    gvh += cross(gwv, fn);
-    const float glv = gA * 2.0f * lv * area * nv;
-    const float garea = gA * lv * lv * nv;
-    const float gnv = gA * lv * lv * area;
-    gfn += gnv * vh;
-    gvh += gnv * fn;
-    const f3 gfu = sg * gfn;
-    const float glfn = 0.5f * garea;
-    const f3 gfr = (gfu - dot(gfu, fu) * fu) / lfn + glfn * fu;
-    const f3 ga = cross(b, gfr);
-    const f3 gb = cross(gfr, a);
-    const f3 gvel = (gvh - dot(gvh, vh) * vh) / lv + glv * vh;
-    fbar[4 * f] = f4(-ga - gb, 1.0f);
-    fbar[4 * f + 1] = f4(ga, 1.0f);
-    fbar[4 * f + 2] = f4(gb, 1.0f);
-    fbar[4 * f + 3] = f4(gvel / 3.0f, 1.0f);
-    const float4 acc0 = facc[2 * f], acc1 = facc[2 * f + 1];
-    facc[2 * f] = make_float4(acc0.x - gvel.x, acc0.y - gvel.y, acc0.z - gvel.z, acc0.w + gcd);
-    facc[2 * f + 1] = make_float4(acc1.x + gcl, acc1.y, acc1.z, acc1.w);
+    sp.with_scene(id1, [&](const SceneConst &S) {
+        const float cd = S.drag_coeff, cl = S.lift_coeff;
+        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        fbar[4 * f] = z4;
+        fbar[4 * f + 1] = z4;
+        fbar[4 * f + 2] = z4;
+        fbar[4 * f + 3] = z4;
+        const f3 p1 = xyz(xsw[id1]), p2 = xyz(xsw[id2]), p3 = xyz(xsw[id3]);
+        const f3 a = p2 - p1, b = p3 - p1;
+        f3 vel = (xyz(v_in[perm[id1]]) + xyz(v_in[perm[id2]]) + xyz(v_in[perm[id3]])) / 3.0f;
+        vel = vel - S.wind;
+        const float lv = length(vel);
+        if (lv <= 1e-5f) return;
+        const f3 vh = vel / lv;
+        const f3 fr = cross(a, b);
+        const float lfn = length(fr);
+        if (lfn <= 1e-5f) return;
+        const f3 fu = fr / lfn;
+        const float sg = (dot(fu, vel) > 0.0f) ? 1.0f : -1.0f;
+        const f3 fn = sg * fu;
+        const float area = 0.5f * lfn;
+        const float nv = dot(fn, vh);
+        const float A = lv * lv * area * nv;
+        const f3 wv = cross(fn, vh);
+        const f3 uh = cross(wv, vh);
+        // adjoint of the force
+        const f3 F = dt * dt * (xyz(gxs[id1]) / m[id1] + xyz(gxs[id2]) / m[id2] + xyz(gxs[id3]) / m[id3]);
+        const float gcd = -A * dot(F, vh);
+        const float gcl = A * dot(F, uh);
+        const float gA = dot(F, -cd * vh + cl * uh);
+        f3 gvh = (-cd * A) * F;
+        const f3 gu = (cl * A) * F;
+        const f3 gwv = cross(vh, gu);
+        gvh += cross(gu, wv);
+        f3 gfn = cross(vh, gwv);
+        gvh += cross(gwv, fn);
+        const float glv = gA * 2.0f * lv * area * nv;
+        const float garea = gA * lv * lv * nv;
+        const float gnv = gA * lv * lv * area;
+        gfn += gnv * vh;
+        gvh += gnv * fn;
+        const f3 gfu = sg * gfn;
+        const float glfn = 0.5f * garea;
+        const f3 gfr = (gfu - dot(gfu, fu) * fu) / lfn + glfn * fu;
+        const f3 ga = cross(b, gfr);
+        const f3 gb = cross(gfr, a);
+        const f3 gvel = (gvh - dot(gvh, vh) * vh) / lv + glv * vh;
+        fbar[4 * f] = f4(-ga - gb, 1.0f);
+        fbar[4 * f + 1] = f4(ga, 1.0f);
+        fbar[4 * f + 2] = f4(gb, 1.0f);
+        fbar[4 * f + 3] = f4(gvel / 3.0f, 1.0f);
+        const float4 acc0 = facc[2 * f], acc1 = facc[2 * f + 1];
+        facc[2 * f] = make_float4(acc0.x - gvel.x, acc0.y - gvel.y, acc0.z - gvel.z, acc0.w + gcd);
+        facc[2 * f + 1] = make_float4(acc1.x + gcl, acc1.y, acc1.z, acc1.w);
+    });
 }
 // per vertex (CSR of (face, corner) entries): position and velocity adjoints
 __global__ __launch_bounds__(kBlock) void ka_wind_gather(float4 *__restrict__ gxs, float4 *__restrict__ gv1,
@@ -624,34 +655,35 @@ __global__ __launch_bounds__(kBlock) void ka_reorder(const int *__restrict__ per
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int o = perm[p];
-    const SceneConst &S = sp.at(p);
-    const Planes &pl = S.pl;
-    f3 xp = xyz(x_in[o]);
-    unsigned long long flags = 0ull;
-    int step = 0;
-    for (int s = 0; s < stab_passes; ++s)
-        for (int q = 0; q < pl.n; ++q, ++step) {
-            const f3 on = mk3(pl.o[q][0], pl.o[q][1], pl.o[q][2]);
-            const f3 nn = mk3(pl.nrm[q][0], pl.nrm[q][1], pl.nrm[q][2]);
-            const float d = dot(on - xp, nn) + S.radius;
-            if (d <= 0.0f) continue;
-            xp += d * nn;
-            flags |= 1ull << step;
-        }
-    const f3 gs = xyz(gxs[p]);
-    f3 gx = xyz(gx1[p]);
-    for (int s = stab_passes - 1; s >= 0; --s)
-        for (int q = pl.n - 1; q >= 0; --q) {
-            const int st = s * pl.n + q;
-            if (!((flags >> st) & 1ull)) continue;
-            const f3 nn = mk3(pl.nrm[q][0], pl.nrm[q][1], pl.nrm[q][2]);
-            gx -= dot(nn, gx + gs) * nn;
-        }
-    gx.y += gw[p] * mass_k * x1[p].w;  // w = e^(k y) / m
-    gx += gs;
-    const f3 gv = xyz(gv1[p]) + gs * dt;
-    gx_old[o] = f4(gx, 0.f);
-    gv_old[o] = f4(gv, 0.f);
+    sp.with_scene(p, [&](const SceneConst &S) {
+        const Planes &pl = S.pl;
+        f3 xp = xyz(x_in[o]);
+        unsigned long long flags = 0ull;
+        int step = 0;
+        for (int s = 0; s < stab_passes; ++s)
+            for (int q = 0; q < pl.n; ++q, ++step) {
+                const f3 on = mk3(pl.o[q][0], pl.o[q][1], pl.o[q][2]);
+                const f3 nn = mk3(pl.nrm[q][0], pl.nrm[q][1], pl.nrm[q][2]);
+                const float d = dot(on - xp, nn) + S.radius;
+                if (d <= 0.0f) continue;
+                xp += d * nn;
+                flags |= 1ull << step;
+            }
+        const f3 gs = xyz(gxs[p]);
+        f3 gx = xyz(gx1[p]);
+        for (int s = stab_passes - 1; s >= 0; --s)
+            for (int q = pl.n - 1; q >= 0; --q) {
+                const int st = s * pl.n + q;
+                if (!((flags >> st) & 1ull)) continue;
+                const f3 nn = mk3(pl.nrm[q][0], pl.nrm[q][1], pl.nrm[q][2]);
+                gx -= dot(nn, gx + gs) * nn;
+            }
+        gx.y += gw[p] * mass_k * x1[p].w;  // w = e^(k y) / m
+        gx += gs;
+        const f3 gv = xyz(gv1[p]) + gs * dt;
+        gx_old[o] = f4(gx, 0.f);
+        gv_old[o] = f4(gv, 0.f);
+    });
 }
 
 // Per-scene sums of the parameter adjoints: slots [poff[s], poff[s+1]) hold
@@ -709,13 +741,14 @@ void launch_adj_xsph(const float4 *xf, const float4 *x1, const float4 *gx2, cons
 template <class SP>
 void launch_adj_projection(const float4 *xs, const float *m, const float *lambda, const uint32_t *ccell,
                            const int *keys, const int2 *ranges, float4 *gxs, float4 *gx_own, float *glam_own,
-                           float *grho, unsigned long long *corr, int n, const SP &sp, int block, hipStream_t st) {
+                           float *grho, unsigned long long *corr, uint32_t *srec, int n, const SP &sp, int block,
+                           hipStream_t st) {
     hipLaunchKernelGGL(ka_position<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, m, lambda, ccell, keys,
-                       ranges, gxs, gx_own, glam_own, corr, n, sp);
+                       ranges, gxs, gx_own, glam_own, corr, srec, n, sp);
     hipLaunchKernelGGL(ka_position_apply<SP>, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, gx_own, glam_own, corr,
                        gxs, grho, n, sp);
     hipLaunchKernelGGL(ka_lambda<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, m, grho, ccell, keys,
-                       ranges, gx_own, corr, n, sp);
+                       ranges, gx_own, corr, srec, n, sp);
     hipLaunchKernelGGL(ka_add_own, dim3(grid_blocks(n)), dim3(kBlock), 0, st, gxs, gx_own, corr, n);
 }
 template <class SP>
@@ -763,7 +796,7 @@ void launch_adj_scene_reduce(const float4 *facc, const float *cacc, const int *p
                                       const SP &, int, hipStream_t);                                             \
     template void launch_adj_projection<SP>(const float4 *, const float *, const float *, const uint32_t *,     \
                                             const int *, const int2 *, float4 *, float4 *, float *, float *,     \
-                                            unsigned long long *, int, const SP &, int, hipStream_t);            \
+                                            unsigned long long *, uint32_t *, int, const SP &, int, hipStream_t); \
     template void launch_adj_planes<SP>(const float4 *, const float4 *, float4 *, float4 *, int, const SP &,    \
                                         hipStream_t);                                                           \
     template void launch_adj_wind<SP>(const float4 *, const float4 *, const int *, const float *, const int4 *, \

@@ -3,12 +3,13 @@
 // reverse sweep through the adjoint kernels (bp_adjoint.hip).
 //
 // bp_forward_checkpointed keeps the full particle state (x, v, mass, phase,
-// slot order) entering every `every`-th substep.  bp_backward walks the
-// segments last to first: it restores a segment's checkpoint, replays its
-// substeps with the same kernels (bit-identical), recording the state before
-// each pass on a tape, then runs the adjoint of those substeps in reverse.
-// Memory: one segment of tape (about 300 B per particle per substep at 4
-// iterations) plus the checkpoints.
+// slot order) entering every `every`-th substep, and records the last
+// segment's substeps on the tape as it runs them.  bp_backward walks the
+// segments last to first: unless the tape already holds a segment, it restores
+// the segment's checkpoint and replays its substeps with the same kernels
+// (bit-identical), recording the state before each pass on the tape; then it
+// runs the adjoint of those substeps in reverse.  Memory: one segment of tape
+// (about 300 B per particle per substep at 4 iterations) plus the checkpoints.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,6 +43,7 @@ struct Adjoint {
     std::vector<State> cps;  // cps[c]: entering substep c * every
     State last;               // after the rollout
     std::vector<Slot> tape;
+    int tape_seg = -1;  // the segment whose substeps the tape holds (-1: none)
     // adjoint buffers (slot order of the substep being reversed)
     float4 *gX[2] = {nullptr, nullptr}, *gV[2] = {nullptr, nullptr};
     float4 *gxs = nullptr, *gx1 = nullptr, *gv1 = nullptr, *gown = nullptr;
@@ -235,6 +237,28 @@ bp_status ensure(bp_solver *s, int total, int every) {
     return BP_OK;
 }
 
+// hooks that record substep t of a segment into tape slot t
+SubstepHooks tape_hooks(Adjoint &A, int t) {
+    Adjoint::Slot &T = A.tape[t];
+    SubstepHooks H;
+    H.x_in = T.x_in;
+    H.v_in = T.v_in;
+    H.keys = T.keys;
+    H.perm = T.perm;
+    H.x1 = T.x1;
+    H.xsw = T.xsw;
+    H.m = T.m;
+    H.orig = T.orig;
+    H.sid = T.sid;
+    H.xp = T.xp.data();
+    H.xl = T.xl.data();
+    H.xd = T.xd.data();
+    H.lam = T.lam.data();
+    H.xf = T.xf;
+    H.contacts = A.contacts;
+    return H;
+}
+
 template <class SP>
 void reverse_substep(bp_solver *s, Adjoint &A, const Adjoint::Slot &T, int &cur, const SP &sp) {
     const int N = s->N;
@@ -258,7 +282,7 @@ void reverse_substep(bp_solver *s, Adjoint &A, const Adjoint::Slot &T, int &cur,
         if (s->has_fluid) {
             launch_adj_cells(T.xl[it], A.ccell, N, sp, st);
             launch_adj_projection(T.xl[it], T.m, T.lam[it], A.ccell, T.keys, s->ranges, A.gxs, A.gown, A.glam,
-                                  A.grho, A.corr, N, sp, s->blk_position, st);
+                                  A.grho, A.corr, s->srec, N, sp, s->blk_position, st);
         }
         if (s->any_planes) launch_adj_planes(T.xp[it], T.x1, A.gxs, A.gx1, N, sp, st);
     }
@@ -291,15 +315,26 @@ bp_status bp_forward_checkpointed(bp_solver *s, int substeps, float subdt, int c
     A.every = checkpoint_every;
     A.total = substeps;
     A.subdt = subdt;
+    A.tape_seg = -1;
     s->drop_graph();
     const float inv_dt = 1.0f / subdt;
     const f3 gstep = gravity_step(s->P, subdt);
+    // contacts are counted over the whole rollout (bp_backward refuses it if
+    // any occurred); the last segment is recorded on the tape as it runs
+    HIP_TRY(hipMemsetAsync(A.contacts, 0, sizeof(int), s->st));
+    const int last_seg = (substeps - 1) / checkpoint_every;
     for (int k = 0; k < substeps; ++k) {
         if (k % checkpoint_every == 0 && (r = copy_state(s, A.cps[k / checkpoint_every], true)) != BP_OK) return r;
-        if ((r = substep(s, subdt, inv_dt, gstep, false, -1, mk3(0.f, 0.f, 0.f), true, nullptr)) != BP_OK) return r;
+        SubstepHooks H;
+        if (k / checkpoint_every == last_seg)
+            H = tape_hooks(A, k - last_seg * checkpoint_every);
+        else
+            H.contacts = A.contacts;
+        if ((r = substep(s, subdt, inv_dt, gstep, false, -1, mk3(0.f, 0.f, 0.f), true, &H)) != BP_OK) return r;
     }
     if ((r = copy_state(s, A.last, true)) != BP_OK) return r;
     HIP_TRY(hipGetLastError());
+    A.tape_seg = last_seg;
     A.recorded = true;
     return BP_OK;
 }
@@ -328,7 +363,6 @@ bp_status bp_backward(bp_solver *s, const float *grad_x, const float *grad_v, fl
     launch_adj_seed(grad_v ? A.seed : nullptr, orig_T, A.gV[0], scale, N, st);
     HIP_TRY(hipMemsetAsync(A.cacc, 0, (size_t)N * sizeof(float), st));
     if (s->nf > 0) HIP_TRY(hipMemsetAsync(A.facc, 0, 2 * (size_t)s->nf * sizeof(float4), st));
-    HIP_TRY(hipMemsetAsync(A.contacts, 0, sizeof(int), st));
 
     const float inv_dt = 1.0f / A.subdt;
     const f3 gstep = gravity_step(s->P, A.subdt);
@@ -338,26 +372,14 @@ bp_status bp_backward(bp_solver *s, const float *grad_x, const float *grad_v, fl
     for (int seg = nseg - 1; seg >= 0 && r == BP_OK; --seg) {
         const int k0 = seg * A.every;
         const int cnt = std::min(A.every, A.total - k0);
-        if ((r = copy_state(s, A.cps[seg], false)) != BP_OK) break;
-        for (int t = 0; t < cnt && r == BP_OK; ++t) {
-            Adjoint::Slot &T = A.tape[t];
-            SubstepHooks H;
-            H.x_in = T.x_in;
-            H.v_in = T.v_in;
-            H.keys = T.keys;
-            H.perm = T.perm;
-            H.x1 = T.x1;
-            H.xsw = T.xsw;
-            H.m = T.m;
-            H.orig = T.orig;
-            H.sid = T.sid;
-            H.xp = T.xp.data();
-            H.xl = T.xl.data();
-            H.xd = T.xd.data();
-            H.lam = T.lam.data();
-            H.xf = T.xf;
-            H.contacts = A.contacts;
-            r = substep(s, A.subdt, inv_dt, gstep, false, -1, mk3(0.f, 0.f, 0.f), true, &H);
+        if (seg != A.tape_seg) {  // replay the segment from its checkpoint onto the tape
+            A.tape_seg = -1;
+            if ((r = copy_state(s, A.cps[seg], false)) != BP_OK) break;
+            for (int t = 0; t < cnt && r == BP_OK; ++t) {
+                const SubstepHooks H = tape_hooks(A, t);
+                r = substep(s, A.subdt, inv_dt, gstep, false, -1, mk3(0.f, 0.f, 0.f), true, &H);
+            }
+            if (r == BP_OK) A.tape_seg = seg;
         }
         for (int t = cnt - 1; t >= 0 && r == BP_OK; --t) {
             const Adjoint::Slot &T = A.tape[t];
