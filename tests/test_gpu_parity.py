@@ -137,6 +137,30 @@ def test_substep_parity_lambda_mode1_and_solid_layer():
     per_substep_parity(sc, prm, 4, "fluid on granular layer, lambda mode 1")
 
 
+def test_substep_parity_dense_block_long_walks():
+    """Spacing 0.3h: about 37 particles per cell, so an interior particle's
+    walk has about 36 chunks of 32 candidates, past the 16 the lambda and
+    omega/normal passes record for the position, vorticity/tension and XSPH
+    passes to replay; the chunks past the record are screened by both walks.
+    (At this packing lambda mode 0's constant denominator overshoots and the
+    block flies apart after two substeps, so two are compared.)"""
+    sp, h = 0.03, 0.1
+    sc = scenes.Scene(particle_radius=0.025, fluid_kernel_radius=h, cell_size=h, grid_dims=(32, 32, 32))
+    sc.add_fluid_block((12, 12, 12), (0.3, 0.3, 0.3), sp, 1.0, jitter=0.004, seed=4)
+    sc.add_box_planes((0, 0, 0), (3.2, 3.2, 3.2))
+    x = np.asarray(sc.positions, np.float64).reshape(-1, 3)
+    cells = np.floor(x / h).astype(np.int64)
+    occ = {}
+    for c in map(tuple, cells):
+        occ[c] = occ.get(c, 0) + 1
+    cand = [sum(occ.get((c[0] + a, c[1] + b, c[2] + d), 0) for a in (-1, 0, 1) for b in (-1, 0, 1)
+                for d in (-1, 0, 1)) for c in map(tuple, cells)]
+    assert max(cand) > 16 * 32, "walks must run past the recorded chunks"
+    prm = scenes.dam_break_params(fluid_rest_density=scenes.lattice_rest_density(sp, h, 1.0), xsph_c=0.0002,
+                                  vorticity_eps=1e-3, akinci=1)
+    per_substep_parity(sc, prm, 2, "dense block 1.7k, long walks", warm=0)
+
+
 def test_substep_parity_mixed_phase_small():
     sc = scenes.mixed_phase(fluid_dims=(16, 16, 16), num_boxes=6, cloth_n=16, grid=(128, 128, 128))
     per_substep_parity(sc, scenes.mixed_phase_params(), 4, "mixed phase small")
