@@ -219,6 +219,7 @@ template <class SP>
 __global__ __launch_bounds__(kMaxBlock) void k_post_a(const float4 *__restrict__ xs, const float4 *__restrict__ v,
                                                    const float *__restrict__ density,
                                                    const int2 *__restrict__ ranges, float4 *__restrict__ omega,
+                                                   float *__restrict__ omag,
                                                    float4 *__restrict__ normal, uint32_t *__restrict__ ccell,
                                                    uint32_t *__restrict__ srec, int n, SP sp, int do_omega,
                                                    int do_normal) {
@@ -252,8 +253,14 @@ __global__ __launch_bounds__(kMaxBlock) void k_post_a(const float4 *__restrict__
                 if (do_normal) nn += frcp(density[j]) * gr;
             },
             AllowNotSelf{i}, rec);
-        if (do_omega) omega[i] = f4(om, fsqrt(dot(om, om)));
-        if (do_normal) normal[i] = f4(K.h * nn, 0.0f);
+        // |omega| in its own array and the density in normal.w: the
+        // vorticity/tension pass then fetches 36 B per partner, not 52
+        if (do_omega) {
+            const float mag = fsqrt(dot(om, om));
+            omega[i] = f4(om, mag);
+            omag[i] = mag;
+        }
+        if (do_normal) normal[i] = f4(K.h * nn, density[i]);
     });
 }
 
@@ -263,6 +270,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_post_a(const float4 *__restrict__
 template <class SP>
 __global__ __launch_bounds__(kMaxBlock) void k_post_b(const float4 *__restrict__ xs, float4 *__restrict__ v,
                                                    const float4 *__restrict__ omega,
+                                                   const float *__restrict__ omag,
                                                    const float4 *__restrict__ normal,
                                                    const float *__restrict__ density,
                                                    const int2 *__restrict__ ranges,
@@ -290,16 +298,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_post_b(const float4 *__restrict__
         rec.slot = i;
         rec.n = n;
         struct Partner {
-            float4 b, nrm;
-            float om, rho;
+            float4 b, nrm;  // nrm.w: the partner's density
+            float om;       // |omega| of the partner
         };
         const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
         for_each_neighbor_nb<kBodyBatch>(
             G, ranges, g, xs, 0,
             [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
             [&](int j) {
-                return Partner{xs[j], do_akinci ? normal[j] : zero4, do_vort ? omega[j].w : 0.0f,
-                               do_akinci ? density[j] : 0.0f};
+                return Partner{xs[j], do_akinci ? normal[j] : zero4, do_vort ? omag[j] : 0.0f};
             },
             [&](int, const Partner &P) {
                 const float4 b = P.b;
@@ -310,7 +317,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_post_b(const float4 *__restrict__
                 if (do_vort) eta += (P.om * K.grad_coef(r, ir)) * diff;
                 if (do_akinci) {
                     // k_ij (-gamma C(r) diff / r - gamma (n_i - n_j)) (fluid.cuh:476-483)
-                    const float kij = 2.0f * F.rest_density * frcp(rhoi + P.rho);
+                    const float kij = 2.0f * F.rest_density * frcp(rhoi + P.nrm.w);
                     const f3 fcoh = (-gam * K.akinci(r) * ir) * diff;
                     const f3 fcur = -gam * (ni - xyz(P.nrm));
                     f += kij * (fcoh + fcur);
@@ -391,17 +398,17 @@ void launch_position(const float4 *xs, const float *m, const float *lambda, cons
 }
 template <class SP>
 void launch_post_a(const float4 *xs, const float4 *v, const float *density, const int2 *ranges, float4 *omega,
-                   float4 *normal, uint32_t *ccell, uint32_t *srec, int n, const SP &sp, int do_omega, int do_normal,
-                   int block, hipStream_t st) {
+                   float *omag, float4 *normal, uint32_t *ccell, uint32_t *srec, int n, const SP &sp, int do_omega,
+                   int do_normal, int block, hipStream_t st) {
     hipLaunchKernelGGL(k_post_a<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, v, density, ranges, omega,
-                       normal, ccell, srec, n, sp, do_omega, do_normal);
+                       omag, normal, ccell, srec, n, sp, do_omega, do_normal);
 }
 template <class SP>
-void launch_post_b(const float4 *xs, float4 *v, const float4 *omega, const float4 *normal, const float *density,
-                   const int2 *ranges, const uint32_t *ccell, const uint32_t *srec, int n, const SP &sp, int do_vort,
-                   int do_akinci, float dt, int block, hipStream_t st) {
-    hipLaunchKernelGGL(k_post_b<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, v, omega, normal, density,
-                       ranges, ccell, srec, n, sp, do_vort, do_akinci, dt);
+void launch_post_b(const float4 *xs, float4 *v, const float4 *omega, const float *omag, const float4 *normal,
+                   const float *density, const int2 *ranges, const uint32_t *ccell, const uint32_t *srec, int n,
+                   const SP &sp, int do_vort, int do_akinci, float dt, int block, hipStream_t st) {
+    hipLaunchKernelGGL(k_post_b<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, v, omega, omag, normal,
+                       density, ranges, ccell, srec, n, sp, do_vort, do_akinci, dt);
 }
 template <class SP>
 void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const uint32_t *ccell, const uint32_t *srec,
@@ -418,11 +425,11 @@ void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const ui
                                       const uint32_t *, float4 *, unsigned long long *, float4 *, float4 *,     \
                                       float4 *, int, const SP &, const Epilogue &, int, hipStream_t);           \
     template void launch_post_a<SP>(const float4 *, const float4 *, const float *, const int2 *, float4 *,      \
-                                    float4 *, uint32_t *, uint32_t *, int, const SP &, int, int, int,           \
+                                    float *, float4 *, uint32_t *, uint32_t *, int, const SP &, int, int, int,  \
                                     hipStream_t);                                                               \
-    template void launch_post_b<SP>(const float4 *, float4 *, const float4 *, const float4 *, const float *,    \
-                                    const int2 *, const uint32_t *, const uint32_t *, int, const SP &, int,     \
-                                    int, float, int, hipStream_t);                                              \
+    template void launch_post_b<SP>(const float4 *, float4 *, const float4 *, const float *, const float4 *,    \
+                                    const float *, const int2 *, const uint32_t *, const uint32_t *, int,       \
+                                    const SP &, int, int, float, int, hipStream_t);                             \
     template void launch_xsph<SP>(const float4 *, const float4 *, const int2 *, const uint32_t *,               \
                                   const uint32_t *, float4 *, int, const SP &, int, int, hipStream_t);
 BP_INSTANTIATE_FLUID(OneScene)

@@ -35,12 +35,12 @@ void launch_position(const float4 *xs, const float *m, const float *lambda, cons
                      float4 *v_out, int n, const SP &sp, const Epilogue &E, int block, hipStream_t st);
 template <class SP>
 void launch_post_a(const float4 *xs, const float4 *v, const float *density, const int2 *ranges, float4 *omega,
-                   float4 *normal, uint32_t *ccell, uint32_t *srec, int n, const SP &sp, int do_omega, int do_normal,
-                   int block, hipStream_t st);
+                   float *omag, float4 *normal, uint32_t *ccell, uint32_t *srec, int n, const SP &sp, int do_omega,
+                   int do_normal, int block, hipStream_t st);
 template <class SP>
-void launch_post_b(const float4 *xs, float4 *v, const float4 *omega, const float4 *normal, const float *density,
-                   const int2 *ranges, const uint32_t *ccell, const uint32_t *srec, int n, const SP &sp, int do_vort,
-                   int do_akinci, float dt, int block, hipStream_t st);
+void launch_post_b(const float4 *xs, float4 *v, const float4 *omega, const float *omag, const float4 *normal,
+                   const float *density, const int2 *ranges, const uint32_t *ccell, const uint32_t *srec, int n,
+                   const SP &sp, int do_vort, int do_akinci, float dt, int block, hipStream_t st);
 template <class SP>
 void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const uint32_t *ccell, const uint32_t *srec,
                  float4 *v_out, int n, const SP &sp, int use_ccell, int block, hipStream_t st);

@@ -248,9 +248,9 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
         const int do_vort = P.vorticity_eps != 0.0f ? 1 : 0;
         const int do_ak = P.akinci ? 1 : 0;
         if (do_vort || do_ak) {
-            TIMED(F_POST_A, launch_post_a(s->xs.cur(), s->v.cur(), s->density, s->ranges, s->omega, s->normal,
+            TIMED(F_POST_A, launch_post_a(s->xs.cur(), s->v.cur(), s->density, s->ranges, s->omega, s->omag, s->normal,
                                           s->ccell, s->srec, N, sp(), do_vort, do_ak, s->blk_post_a, st));
-            TIMED(F_POST_B, launch_post_b(s->xs.cur(), s->v.cur(), s->omega, s->normal, s->density, s->ranges,
+            TIMED(F_POST_B, launch_post_b(s->xs.cur(), s->v.cur(), s->omega, s->omag, s->normal, s->density, s->ranges,
                                           s->ccell, s->srec, N, sp(), do_vort, do_ak, subdt, s->blk_post_b, st));
         }
         if (P.xsph_c != 0.0f) {
@@ -524,6 +524,7 @@ bp_status create_impl(const bp_scene_desc *scs, const bp_params *params, int ns,
     TRY(s->alloc(&s->sum_tmp, N));
     TRY(s->alloc(&s->corr, 3 * (size_t)N));
     TRY(s->alloc(&s->omega, N));
+    TRY(s->alloc(&s->omag, N));
     TRY(s->alloc(&s->normal, N));
     if (s->has_solid && s->has_fluid) {
         TRY(s->alloc(&s->cell_stamp, s->num_cells));
@@ -545,6 +546,7 @@ bp_status create_impl(const bp_scene_desc *scs, const bp_params *params, int ns,
         HTRY(hipMemsetAsync(s->lambda, 0, N * sizeof(float), s->st));
         HTRY(hipMemsetAsync(s->density, 0, N * sizeof(float), s->st));
         HTRY(hipMemsetAsync(s->omega, 0, N * sizeof(float4), s->st));
+        HTRY(hipMemsetAsync(s->omag, 0, N * sizeof(float), s->st));
         HTRY(hipMemsetAsync(s->normal, 0, N * sizeof(float4), s->st));
         HTRY(hipMemsetAsync(s->corr, 0, 3 * (size_t)N * sizeof(unsigned long long), s->st));
     }

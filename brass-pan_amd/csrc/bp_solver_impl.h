@@ -82,7 +82,8 @@ struct bp_solver {
     uint32_t *srec = nullptr;  // screen record of the lambda and omega/normal passes (kRecChunks x N masks)
     float4 *sum_tmp = nullptr;
     unsigned long long *corr = nullptr;  // fixed-point pair corrections, kept zero between passes
-    float4 *omega = nullptr, *normal = nullptr;
+    float4 *omega = nullptr, *normal = nullptr;  // normal.w: the density it was computed with
+    float *omag = nullptr;                        // |omega|
     uint32_t *cell_stamp = nullptr, *stamp = nullptr;  // cells holding a solid (fluid + solid scenes)
     uint32_t *fluid_stamp = nullptr;                    // cells holding a fluid particle (same scenes)
 
