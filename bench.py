@@ -55,7 +55,10 @@ def build_workload(name):
 
 
 def time_plain(sc, prm, subdt, steps, warmup):
+    """Workgroup sizes from bp_solver_autotune (which restores the state it
+    started from), then `warmup` untimed and `steps` timed substeps."""
     s = bp.Solver(sc, prm)
+    s.block_sizes = s.autotune(subdt, 2)
     for _ in range(warmup):
         s.step(1, subdt)
     s.sync()
@@ -324,7 +327,7 @@ def main():
         "config": {"workload": args.workload, "particles": n, "iterations": prm.iterations,
                    "substep_dt": subdt, "distance_constraints": len(sc.distance_pairs),
                    "wind_faces": len(sc.faces), "rigid_bodies": len(sc.rigidbody_particle_id_ranges),
-                   "parallelism": "single GPU"},
+                   "block_sizes": s.block_sizes, "parallelism": "single GPU"},
         "roofline": roof,
         "kernel_ms": fam_times,
     }
@@ -338,7 +341,8 @@ def main():
         lroof, ltimes = kernel_roofline(ls, lsub, substeps=2, workload="large_block_4m")
         out["large_block_4m"] = {"value": round(ls.n * lsteps / lwall, 1),
                                  "ms_per_step": round(lwall / lsteps * 1e3, 4), "steps": lsteps,
-                                 "warmup": lwarm, "roofline": lroof, "kernel_ms": ltimes}
+                                 "warmup": lwarm, "block_sizes": ls.block_sizes, "roofline": lroof,
+                                 "kernel_ms": ltimes}
         ls.close()
     if not args.no_batched:
         out["batched_512"] = time_batched(graphs=not args.no_graph)

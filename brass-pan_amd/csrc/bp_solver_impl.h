@@ -111,7 +111,7 @@ struct bp_solver {
     std::vector<void *> allocs;
 
     // neighbour-pass workgroup sizes (bp_solver_autotune)
-    int blk_lambda = 256, blk_position = 256, blk_post_a = 256, blk_post_b = 256, blk_xsph = 256, blk_pp = 256;
+    int blk_lambda = 128, blk_position = 128, blk_post_a = 128, blk_post_b = 128, blk_xsph = 128, blk_pp = 128;
 
     // hipGraph of two substeps (the ping-pong buffers return to their start)
     bool use_graph = false;
