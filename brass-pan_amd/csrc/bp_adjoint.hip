@@ -451,24 +451,23 @@ __global__ __launch_bounds__(kBlock) void ka_planes(const float4 *__restrict__ x
                                                     float4 *__restrict__ gxs, float4 *__restrict__ gx1, int n, SP sp) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    sp.with_scene(k, [&](const SceneConst &S) {
-        const Planes &pl = S.pl;
-        if (pl.n == 0) return;
-        const f3 x0 = xyz(xp[k]), pos = xyz(x1[k]);
-        f3 y = xyz(gxs[k]), gpos = xyz(gx1[k]);
-        for (int q = pl.n - 1; q >= 0; --q) {
-            f3 xin = x0;
-            for (int u = 0; u < q; ++u)
-                xin = plane_fwd(xin, pos, mk3(pl.o[u][0], pl.o[u][1], pl.o[u][2]),
-                                mk3(pl.nrm[u][0], pl.nrm[u][1], pl.nrm[u][2]), S.radius, S.friction_static,
-                                S.friction_dynamic);
-            y = plane_adj(xin, pos, mk3(pl.o[q][0], pl.o[q][1], pl.o[q][2]),
-                          mk3(pl.nrm[q][0], pl.nrm[q][1], pl.nrm[q][2]), S.radius, S.friction_static,
-                          S.friction_dynamic, y, &gpos);
-        }
-        gxs[k] = f4(y, 0.f);
-        gx1[k] = f4(gpos, 0.f);
-    });
+    const SceneConst &S = sp.at(k);
+    const Planes &pl = S.pl;
+    if (pl.n == 0) return;
+    const f3 x0 = xyz(xp[k]), pos = xyz(x1[k]);
+    f3 y = xyz(gxs[k]), gpos = xyz(gx1[k]);
+    for (int q = pl.n - 1; q >= 0; --q) {
+        f3 xin = x0;
+        for (int u = 0; u < q; ++u)
+            xin = plane_fwd(xin, pos, mk3(pl.o[u][0], pl.o[u][1], pl.o[u][2]),
+                            mk3(pl.nrm[u][0], pl.nrm[u][1], pl.nrm[u][2]), S.radius, S.friction_static,
+                            S.friction_dynamic);
+        y = plane_adj(xin, pos, mk3(pl.o[q][0], pl.o[q][1], pl.o[q][2]),
+                      mk3(pl.nrm[q][0], pl.nrm[q][1], pl.nrm[q][2]), S.radius, S.friction_static,
+                      S.friction_dynamic, y, &gpos);
+    }
+    gxs[k] = f4(y, 0.f);
+    gx1[k] = f4(gpos, 0.f);
 }
 
 // ------------------------------------------------------------ pins, distance
@@ -655,35 +654,34 @@ __global__ __launch_bounds__(kBlock) void ka_reorder(const int *__restrict__ per
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int o = perm[p];
-    sp.with_scene(p, [&](const SceneConst &S) {
-        const Planes &pl = S.pl;
-        f3 xp = xyz(x_in[o]);
-        unsigned long long flags = 0ull;
-        int step = 0;
-        for (int s = 0; s < stab_passes; ++s)
-            for (int q = 0; q < pl.n; ++q, ++step) {
-                const f3 on = mk3(pl.o[q][0], pl.o[q][1], pl.o[q][2]);
-                const f3 nn = mk3(pl.nrm[q][0], pl.nrm[q][1], pl.nrm[q][2]);
-                const float d = dot(on - xp, nn) + S.radius;
-                if (d <= 0.0f) continue;
-                xp += d * nn;
-                flags |= 1ull << step;
-            }
-        const f3 gs = xyz(gxs[p]);
-        f3 gx = xyz(gx1[p]);
-        for (int s = stab_passes - 1; s >= 0; --s)
-            for (int q = pl.n - 1; q >= 0; --q) {
-                const int st = s * pl.n + q;
-                if (!((flags >> st) & 1ull)) continue;
-                const f3 nn = mk3(pl.nrm[q][0], pl.nrm[q][1], pl.nrm[q][2]);
-                gx -= dot(nn, gx + gs) * nn;
-            }
-        gx.y += gw[p] * mass_k * x1[p].w;  // w = e^(k y) / m
-        gx += gs;
-        const f3 gv = xyz(gv1[p]) + gs * dt;
-        gx_old[o] = f4(gx, 0.f);
-        gv_old[o] = f4(gv, 0.f);
-    });
+    const SceneConst &S = sp.at(p);
+    const Planes &pl = S.pl;
+    f3 xp = xyz(x_in[o]);
+    unsigned long long flags = 0ull;
+    int step = 0;
+    for (int s = 0; s < stab_passes; ++s)
+        for (int q = 0; q < pl.n; ++q, ++step) {
+            const f3 on = mk3(pl.o[q][0], pl.o[q][1], pl.o[q][2]);
+            const f3 nn = mk3(pl.nrm[q][0], pl.nrm[q][1], pl.nrm[q][2]);
+            const float d = dot(on - xp, nn) + S.radius;
+            if (d <= 0.0f) continue;
+            xp += d * nn;
+            flags |= 1ull << step;
+        }
+    const f3 gs = xyz(gxs[p]);
+    f3 gx = xyz(gx1[p]);
+    for (int s = stab_passes - 1; s >= 0; --s)
+        for (int q = pl.n - 1; q >= 0; --q) {
+            const int st = s * pl.n + q;
+            if (!((flags >> st) & 1ull)) continue;
+            const f3 nn = mk3(pl.nrm[q][0], pl.nrm[q][1], pl.nrm[q][2]);
+            gx -= dot(nn, gx + gs) * nn;
+        }
+    gx.y += gw[p] * mass_k * x1[p].w;  // w = e^(k y) / m
+    gx += gs;
+    const f3 gv = xyz(gv1[p]) + gs * dt;
+    gx_old[o] = f4(gx, 0.f);
+    gv_old[o] = f4(gv, 0.f);
 }
 
 // Per-scene sums of the parameter adjoints: slots [poff[s], poff[s+1]) hold

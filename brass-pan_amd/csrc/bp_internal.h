@@ -164,7 +164,10 @@ struct SceneConst {
 // with_scene(slot, f) runs f(scene of slot); in a scene table it runs once per
 // distinct scene among the wave's active lanes (scenes are contiguous slot
 // ranges, so nearly always once), with that scene's id made wave-uniform so
-// its constants are scalar loads rather than one load per lane.
+// its constants are scalar loads rather than one load per lane.  f gets a
+// local copy of the entry: for passes that read scalar fields only.  A pass
+// that indexes the plane arrays with a loop variable keeps at(slot), since
+// the copy would then live in scratch memory.
 struct OneScene {
     SceneConst c;
     __device__ __forceinline__ const SceneConst &at(int) const { return c; }
