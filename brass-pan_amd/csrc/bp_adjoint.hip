@@ -297,6 +297,7 @@ __global__ __launch_bounds__(kMaxBlock) void ka_position(const float4 *__restric
                                                          const float4 *__restrict__ gpost, float4 *__restrict__ gx_own,
                                                          float *__restrict__ glam_own,
                                                          unsigned long long *__restrict__ corr,
+                                                         const uint32_t *__restrict__ trec,
                                                          uint32_t *__restrict__ srec, int n, SP sp) {
     const int k = xcd_index();
     if (k >= n) return;
@@ -321,8 +322,11 @@ __global__ __launch_bounds__(kMaxBlock) void ka_position(const float4 *__restric
             uint32_t cell;
             float lam, m;
         };
-        ScreenRec rec;  // replayed by the density adjoint of the same x*
-        rec.out = srec;
+        // A fluid particle replays the forward lambda pass's screen of this x*
+        // (taped); a solid screens and records for the density adjoint.
+        ScreenRec rec;
+        rec.in = kf ? trec : nullptr;
+        rec.out = kf ? nullptr : srec;
         rec.slot = k;
         rec.n = n;
         for_each_neighbor_nb<kBodyBatch>(
@@ -385,6 +389,7 @@ __global__ __launch_bounds__(kMaxBlock) void ka_lambda(const float4 *__restrict_
                                                        const int *__restrict__ keys, const int2 *__restrict__ ranges,
                                                        float4 *__restrict__ gx_own,
                                                        unsigned long long *__restrict__ corr,
+                                                       const uint32_t *__restrict__ trec,
                                                        const uint32_t *__restrict__ srec, int n, SP sp) {
     const int k = xcd_index();
     if (k >= n) return;
@@ -407,8 +412,8 @@ __global__ __launch_bounds__(kMaxBlock) void ka_lambda(const float4 *__restrict_
             uint32_t cell;
             float m, grho;
         };
-        ScreenRec rec;  // the position adjoint screened this x* with the same test
-        rec.in = srec;
+        ScreenRec rec;  // the records the position adjoint replayed or wrote for this x*
+        rec.in = kf ? trec : srec;
         rec.slot = k;
         rec.n = n;
         for_each_neighbor_nb<kBodyBatch>(
@@ -739,14 +744,14 @@ void launch_adj_xsph(const float4 *xf, const float4 *x1, const float4 *gx2, cons
 template <class SP>
 void launch_adj_projection(const float4 *xs, const float *m, const float *lambda, const uint32_t *ccell,
                            const int *keys, const int2 *ranges, float4 *gxs, float4 *gx_own, float *glam_own,
-                           float *grho, unsigned long long *corr, uint32_t *srec, int n, const SP &sp, int block,
-                           hipStream_t st) {
+                           float *grho, unsigned long long *corr, const uint32_t *trec, uint32_t *srec, int n,
+                           const SP &sp, int block, hipStream_t st) {
     hipLaunchKernelGGL(ka_position<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, m, lambda, ccell, keys,
-                       ranges, gxs, gx_own, glam_own, corr, srec, n, sp);
+                       ranges, gxs, gx_own, glam_own, corr, trec, srec, n, sp);
     hipLaunchKernelGGL(ka_position_apply<SP>, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, gx_own, glam_own, corr,
                        gxs, grho, n, sp);
     hipLaunchKernelGGL(ka_lambda<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, m, grho, ccell, keys,
-                       ranges, gx_own, corr, srec, n, sp);
+                       ranges, gx_own, corr, trec, srec, n, sp);
     hipLaunchKernelGGL(ka_add_own, dim3(grid_blocks(n)), dim3(kBlock), 0, st, gxs, gx_own, corr, n);
 }
 template <class SP>
@@ -794,7 +799,8 @@ void launch_adj_scene_reduce(const float4 *facc, const float *cacc, const int *p
                                       const SP &, int, hipStream_t);                                             \
     template void launch_adj_projection<SP>(const float4 *, const float *, const float *, const uint32_t *,     \
                                             const int *, const int2 *, float4 *, float4 *, float *, float *,     \
-                                            unsigned long long *, uint32_t *, int, const SP &, int, hipStream_t); \
+                                            unsigned long long *, const uint32_t *, uint32_t *, int, const SP &, \
+                                            int, hipStream_t);                                                   \
     template void launch_adj_planes<SP>(const float4 *, const float4 *, float4 *, float4 *, int, const SP &,    \
                                         hipStream_t);                                                           \
     template void launch_adj_wind<SP>(const float4 *, const float4 *, const int *, const float *, const int4 *, \

@@ -9,7 +9,9 @@
 // the segment's checkpoint and replays its substeps with the same kernels
 // (bit-identical), recording the state before each pass on the tape; then it
 // runs the adjoint of those substeps in reverse.  Memory: one segment of tape
-// (about 300 B per particle per substep at 4 iterations) plus the checkpoints.
+// (about 300 B per particle per substep at 4 iterations, plus the lambda
+// passes' screen records, 64 B per particle per iteration) and the
+// checkpoints.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,6 +38,7 @@ struct Adjoint {
         float *m = nullptr;
         std::vector<float4 *> xp, xl, xd;
         std::vector<float *> lam;
+        std::vector<uint32_t *> rec;  // per iteration: the lambda pass's screen record of xl
     };
     int every = 0, total = 0, iters = 0;
     float subdt = 0.f;
@@ -227,9 +230,11 @@ bp_status ensure(bp_solver *s, int total, int every) {
             T.xl.assign(iters, nullptr);
             T.xd.assign(iters, nullptr);
             T.lam.assign(iters, nullptr);
+            T.rec.assign(iters, nullptr);
             for (int it = 0; it < iters; ++it)
                 if ((r = A.alloc(&T.xp[it], N)) != BP_OK || (r = A.alloc(&T.xl[it], (size_t)N + kWalkPad)) != BP_OK ||
-                    (r = A.alloc(&T.xd[it], N)) != BP_OK || (r = A.alloc(&T.lam[it], N)) != BP_OK)
+                    (r = A.alloc(&T.xd[it], N)) != BP_OK || (r = A.alloc(&T.lam[it], N)) != BP_OK ||
+                    (s->has_fluid && (r = A.alloc(&T.rec[it], (size_t)kRecChunks * N)) != BP_OK))
                     return r;
         }
     }
@@ -254,6 +259,7 @@ SubstepHooks tape_hooks(Adjoint &A, int t) {
     H.xl = T.xl.data();
     H.xd = T.xd.data();
     H.lam = T.lam.data();
+    H.rec = T.rec.data();
     H.xf = T.xf;
     H.contacts = A.contacts;
     return H;
@@ -282,7 +288,7 @@ void reverse_substep(bp_solver *s, Adjoint &A, const Adjoint::Slot &T, int &cur,
         if (s->has_fluid) {
             launch_adj_cells(T.xl[it], A.ccell, N, sp, st);
             launch_adj_projection(T.xl[it], T.m, T.lam[it], A.ccell, T.keys, s->ranges, A.gxs, A.gown, A.glam,
-                                  A.grho, A.corr, s->srec, N, sp, s->blk_position, st);
+                                  A.grho, A.corr, T.rec[it], s->srec, N, sp, s->blk_position, st);
         }
         if (s->any_planes) launch_adj_planes(T.xp[it], T.x1, A.gxs, A.gx1, N, sp, st);
     }

@@ -212,7 +212,7 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
         if (H.xl) HIP_TRY(snap(H.xl[it], s->xs.cur(), N, st));
         if (s->has_fluid) {
             TIMED(F_LAMBDA, launch_lambda(s->xs.cur(), s->m.cur(), s->ranges, s->lambda, s->density, s->ccell,
-                                          s->srec, N, sp(), s->blk_lambda, st));
+                                          H.rec ? H.rec[it] : s->srec, N, sp(), s->blk_lambda, st));
             if (H.lam) HIP_TRY(snap(H.lam[it], s->lambda, N, st));
             Epilogue E{};
             E.planes_next = (!no_fuse && s->any_planes && it + 1 < iters && !tail_groups && !pick) ? 1 : 0;
@@ -220,8 +220,9 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
             E.inv_dt = inv_dt;
             E.sleep_eps2 = P.sleep_epsilon * P.sleep_epsilon;
             TIMED(F_POSITION, launch_position(s->xs.cur(), s->m.cur(), s->lambda, s->ccell, sk, s->ranges,
-                                              s->srec, s->fluid_stamp, s->stamp, s->sum_tmp, s->corr, s->xs.alt(),
-                                              s->x.cur(), s->v.cur(), N, sp(), E, s->blk_position, st));
+                                              H.rec ? H.rec[it] : s->srec, s->fluid_stamp, s->stamp, s->sum_tmp,
+                                              s->corr, s->xs.alt(), s->x.cur(), s->v.cur(), N, sp(), E,
+                                              s->blk_position, st));
             s->xs.flip();
             planes_done = E.planes_next;
             finished = E.finish;

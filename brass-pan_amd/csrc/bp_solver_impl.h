@@ -200,6 +200,7 @@ struct SubstepHooks {
     float **lam = nullptr;                    // per iteration
     float4 *xf = nullptr;                     // x* after the last iteration
     int *contacts = nullptr;                  // counts particles with a particle-particle contact
+    uint32_t **rec = nullptr;                 // per iteration: where the lambda pass writes its screen record
 };
 
 // One substep of ParticleSolver::update.  `no_fuse` keeps every pass a
