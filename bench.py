@@ -265,6 +265,35 @@ def time_adjoint(num=64, graphs=True):
     }
 
 
+def copy_bandwidth(mib=1024, reps=5):
+    """Device-to-device copy of `mib` MiB timed with HIP events (best of
+    `reps`): a STREAM-style check of the HBM peak, counting the bytes read
+    plus the bytes written."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    assert hip.hipEventCreate(C.byref(e0)) == 0 and hip.hipEventCreate(C.byref(e1)) == 0
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    best = float("inf")
+    for _ in range(reps + 1):  # the first copy is a warm-up
+        hip.hipEventRecord(e0, None)
+        hip.hipMemcpyAsync(b, a, C.c_size_t(n), 3, None)  # hipMemcpyDeviceToDevice
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = C.c_float()
+        hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+        best = min(best, ms.value) if _ else best
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return {"bytes": 2 * n, "ms": round(best, 4), "GBs": round(2 * n / (best * 1e-3) / 1e9, 1)}
+
+
 def copy_calibration(mib):
     """A device-to-device copy of `mib` MiB through the HIP runtime: the known
     traffic (mib read + mib written) that scales rocprof FETCH_SIZE and
@@ -330,6 +359,7 @@ def main():
                    "block_sizes": s.block_sizes, "parallelism": "single GPU"},
         "roofline": roof,
         "kernel_ms": fam_times,
+        "hbm_copy_check": copy_bandwidth(),
     }
     if not args.no_large and args.workload != "large_block_4m":
         # The large block's Akinci tension (gamma 0.5, no mass weighting,
