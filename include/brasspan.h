@@ -214,12 +214,15 @@ bp_status bp_solver_set_graph(bp_solver *s, int enable);
  *
  * bp_forward_checkpointed advances `substeps` substeps of size `subdt`
  * (every pass a separate kernel, no picking) and keeps the full particle
- * state entering every `checkpoint_every`-th substep. */
+ * state entering every `checkpoint_every`-th substep.  Device memory: the
+ * checkpoints plus one segment of tape, about (300 + 64 x iterations) B per
+ * particle per substep of a segment. */
 bp_status bp_forward_checkpointed(bp_solver *s, int substeps, float subdt, int checkpoint_every);
 
 /* Reverse sweep over the last bp_forward_checkpointed rollout: each segment
  * is replayed from its checkpoint (recording the state before every pass),
- * then differentiated in reverse.  Inputs: dL/dx_T, dL/dv_T (float3 per
+ * except the last, which the forward recorded as it ran it, then
+ * differentiated in reverse.  Inputs: dL/dx_T, dL/dv_T (float3 per
  * particle, original order; either may be NULL = 0).  Outputs (each may be
  * NULL): dL/dx_0, dL/dv_0 (float3, original order) and, per scene s,
  * grad_theta[6 s .. 6 s + 5] = dL/d(wind x, wind y, wind z, drag_coeff,
