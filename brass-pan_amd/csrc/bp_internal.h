@@ -332,7 +332,9 @@ __device__ __forceinline__ bool stamp_near(const GridDesc &G, const int g[3], co
 // Two-phase neighbour walk: for every run, chunks of up to 32 candidates are
 // first screened with the cheap test `pick(j, b)` (b = xs[j]) into a bitmask,
 // which `allow(base)` then restricts by slot index; `consume(base, mask)`
-// receives each chunk's partners.
+// receives each chunk's partners.  With rec.out every chunk is screened (even
+// one `allow` empties) and the first kRecChunks masks are stored before
+// `allow`; with rec.in those chunks take their masks from the record instead.
 //
 // Candidates are loaded in batches of 8 from one base address with immediate
 // offsets.  A batch may read up to 7 slots past the end of its run, so every
