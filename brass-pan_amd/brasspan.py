@@ -97,6 +97,8 @@ SIGNATURES = {
     "bp_backward": (_S, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
                          C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "bp_solver_timed_substeps": (_S, [_P, C.c_int, C.c_float, C.POINTER(C.c_float)]),
+    "bp_export_frame": (_S, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "bp_map_particle_infos": (_S, [_P, C.c_void_p]),
     "bp_solver_enable_kernel_timing": (_S, [_P, C.c_int]),
     "bp_solver_kernel_times": (_S, [_P, C.c_int, C.c_char_p, C.POINTER(C.c_float), C.POINTER(C.c_int),
                                     C.POINTER(C.c_int)]),
@@ -298,6 +300,20 @@ class Solver:
         if self.num_rigid:
             _check(self.lib.bp_get_rigid_state(self.h, _fp(q), _fp(cm)))
         return q, cm
+
+    def export_frame(self):
+        """bp_export_frame: (records [N, 4] float32 in original-id order, x y z
+        and the group id's int bits; matrices [bodies, 4, 4], M[b] @ (p, 1)
+        maps a body's rest offset p to its position)."""
+        rec = np.empty((self.n, 4), np.float32)
+        mats = np.zeros((self.num_rigid, 16), np.float32)
+        _check(self.lib.bp_export_frame(self.h, _fp(rec), _fp(mats) if self.num_rigid else None))
+        return rec, mats.reshape(-1, 4, 4).transpose(0, 2, 1).copy()  # column-major -> M[b][row][col]
+
+    def map_particle_infos(self, device_ptr: int):
+        """bp_map_particle_infos into a device buffer of 16 N bytes (e.g. a
+        torch CUDA tensor's data_ptr()), ordered on the solver's stream."""
+        _check(self.lib.bp_map_particle_infos(self.h, C.c_void_p(device_ptr)))
 
     def set_rigid_state(self, quats):
         q = np.ascontiguousarray(quats, np.float32)

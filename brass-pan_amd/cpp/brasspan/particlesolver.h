@@ -4,6 +4,9 @@
 // the library's message (the reference prints and continues,
 // cuda/helper.cuh:282-290).
 #pragma once
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -75,6 +78,39 @@ struct SolverHandle {
         bp_check(bp_copy_state_original_order(handle, reinterpret_cast<float *>(pos.data()),
                                               reinterpret_cast<float *>(vel.data())));
     }
+    // The renderer boundary, headless (main.cu:273-291): one record per
+    // particle in original-id order, {x, y, z, group id} with the id's int bits
+    // in w, and one column-major 4x4 per rigid body (rotation + centre of mass).
+    void exportFrame(std::vector<float4> &particles, std::vector<float> &matrices) {
+        particles.resize(numParticles);
+        matrices.resize(16 * (size_t)numRigidBodies);
+        bp_check(bp_export_frame(handle, reinterpret_cast<float *>(particles.data()),
+                                 matrices.empty() ? nullptr : matrices.data()));
+    }
+    // The particle records into a device buffer of 16 * numParticles bytes.
+    void mapParticleInfos(void *deviceRecords) { bp_check(bp_map_particle_infos(handle, deviceRecords)); }
+    // Writes the frame file read by brass-pan_amd/frames.py: a 32-byte header
+    // ("BPFR", u32 version 1, u32 particles, u32 bodies, u64 frame, f64 time),
+    // then the particle records, then the matrices (little endian).
+    void writeFrame(const std::string &path, uint64_t frame, double time) {
+        std::vector<float4> particles;
+        std::vector<float> matrices;
+        exportFrame(particles, matrices);
+        unsigned char header[32];
+        const uint32_t version = 1, n = (uint32_t)numParticles, nb = (uint32_t)numRigidBodies;
+        std::memcpy(header, "BPFR", 4);
+        std::memcpy(header + 4, &version, 4);
+        std::memcpy(header + 8, &n, 4);
+        std::memcpy(header + 12, &nb, 4);
+        std::memcpy(header + 16, &frame, 8);
+        std::memcpy(header + 24, &time, 8);
+        std::FILE *f = std::fopen(path.c_str(), "wb");
+        if (!f) throw std::runtime_error("brasspan: cannot open " + path);
+        const bool ok = std::fwrite(header, 1, sizeof(header), f) == sizeof(header) &&
+                        std::fwrite(particles.data(), sizeof(float4), particles.size(), f) == particles.size() &&
+                        std::fwrite(matrices.data(), sizeof(float), matrices.size(), f) == matrices.size();
+        if (std::fclose(f) != 0 || !ok) throw std::runtime_error("brasspan: cannot write " + path);
+    }
     void synchronize() { bp_check(bp_solver_sync(handle)); }
     // Replays pairs of substeps from a captured hipGraph.
     void setGraph(bool enable) { bp_check(bp_solver_set_graph(handle, enable ? 1 : 0)); }
@@ -107,6 +143,7 @@ struct SolverHandle {
 
     size_t numParticles = 0;
     int numScenes = 1;
+    int numRigidBodies = 0;
     bp_solver *handle = nullptr;
 };
 
@@ -121,6 +158,7 @@ struct ParticleSolver : SolverHandle {
         SceneDesc d(s);
         check(bp_solver_create(&d.d, &p, hipStream, &handle));
         numParticles = (size_t)d.d.num_particles;
+        numRigidBodies = d.d.num_rigid;
         solidRadius = s.particleRadius;
         fluidKernelRadius = s.fluidKernelRadius;
     }
@@ -185,7 +223,10 @@ struct BatchParticleSolver : SolverHandle {
         }
         bp_check(bp_batch_create(ds.data(), params.data(), (int)ds.size(), hipStream, &handle));
         numScenes = (int)ds.size();
-        for (const auto &d : ds) numParticles += (size_t)d.num_particles;
+        for (const auto &d : ds) {
+            numParticles += (size_t)d.num_particles;
+            numRigidBodies += d.num_rigid;
+        }
     }
     // ParticleSolver::update for all scenes (no picking).
     void update(const int numSubTimeStep, const float deltaTime) {

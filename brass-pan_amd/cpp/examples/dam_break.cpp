@@ -1,6 +1,9 @@
 // dam_break.cpp -- headless use of the Scene / ParticleSolver API, as
 // main.cu:293-435 drives the reference (without the renderer): build a fluid
 // block in a box, step frames with update(2, 1/60), print the state.
+// Usage: dam_break [frames] [frame file]; with a frame file, the final state
+// is written there in the frame format of brass-pan_amd/frames.py (what the
+// reference would hand its renderer).
 #include <cstdio>
 #include <memory>
 #include <vector>
@@ -36,6 +39,7 @@ int main(int argc, char **argv) {
         for (const float3 &q : pos) ymean += q.y;
         std::printf("particles %zu frames %d mean height %.4f slot(0)->orig %d\n", solver.numParticles, frames,
                     ymean / pos.size(), solver.queryOriginalParticleId(0));
+        if (argc > 2) solver.writeFrame(argv[2], (uint64_t)frames, frames / 60.0);
     } catch (const std::exception &e) {
         std::fprintf(stderr, "%s\n", e.what());
         return 1;

@@ -309,6 +309,25 @@ void launch_reorder(const int *perm, const float4 *x_in, const float4 *v_in, con
                        need_w, stab_passes, collide, sp);
 }
 
+// The renderer boundary without a renderer (mapParticleInfos, main.cu:273-282):
+// one {x, y, z, group id} record per particle, the group id's int bits in the
+// fourth float, written in original-id order so frames line up across
+// substeps.
+__global__ __launch_bounds__(kBlock) void k_map_particle_infos(const float4 *__restrict__ x,
+                                                               const int *__restrict__ orig,
+                                                               const int *__restrict__ group,
+                                                               float4 *__restrict__ out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int o = orig[i];
+    const float4 p = x[i];
+    out[o] = make_float4(p.x, p.y, p.z, __int_as_float(group[o]));
+}
+void launch_map_particle_infos(const float4 *x, const int *orig, const int *group, float4 *out, int n,
+                               hipStream_t st) {
+    hipLaunchKernelGGL(k_map_particle_infos, dim3(grid_blocks(n)), dim3(kBlock), 0, st, x, orig, group, out, n);
+}
+
 #define BP_INSTANTIATE_GRID(SP)                                                                                 \
     template void launch_predict_hash<SP>(const float4 *, const float4 *, int *, int, float, f3, const SP &,    \
                                           const int *, int2 *, hipStream_t);                                    \

@@ -439,7 +439,7 @@ bp_status create_impl(const bp_scene_desc *scs, const bp_params *params, int ns,
     std::vector<float4> hx(N), hv(N);
     std::vector<int> iota(N), hsid(N);
     std::vector<float> hm(N);
-    std::vector<int> hph(N);
+    std::vector<int> hph(N), hgrp(N);
     std::vector<int> dp, bd, fc, rr, im;
     std::vector<float> dpar;
     std::vector<float4> rest;
@@ -458,6 +458,7 @@ bp_status create_impl(const bp_scene_desc *scs, const bp_params *params, int ns,
                                   : make_float4(0.f, 0.f, 0.f, 0.f);
             hm[g] = sc.masses[i];
             hph[g] = sc.phases[i];
+            hgrp[g] = sc.group_ids ? sc.group_ids[i] : 0;
             iota[g] = g;
             hsid[g] = k;
             if (sc.phases[i] > 0) s->has_solid = true; else s->has_fluid = true;
@@ -521,6 +522,7 @@ bp_status create_impl(const bp_scene_desc *scs, const bp_params *params, int ns,
     TRY(s->alloc(&s->lambda, N));
     TRY(s->alloc(&s->density, N));
     TRY(s->alloc(&s->ccell, N));
+    TRY(s->alloc(&s->group, N));
     if (s->has_fluid) TRY(s->alloc(&s->srec, (size_t)kRecChunks * N));
     TRY(s->alloc(&s->sum_tmp, N));
     TRY(s->alloc(&s->corr, 3 * (size_t)N));
@@ -541,6 +543,7 @@ bp_status create_impl(const bp_scene_desc *scs, const bp_params *params, int ns,
         HTRY(hipMemcpyAsync(s->v.cur(), hv.data(), N * sizeof(float4), hipMemcpyHostToDevice, s->st));
         HTRY(hipMemcpyAsync(s->m.cur(), hm.data(), N * sizeof(float), hipMemcpyHostToDevice, s->st));
         HTRY(hipMemcpyAsync(s->ph.cur(), hph.data(), N * sizeof(int), hipMemcpyHostToDevice, s->st));
+        HTRY(hipMemcpyAsync(s->group, hgrp.data(), N * sizeof(int), hipMemcpyHostToDevice, s->st));
         HTRY(hipMemcpyAsync(s->orig.cur(), iota.data(), N * sizeof(int), hipMemcpyHostToDevice, s->st));
         HTRY(hipMemcpyAsync(s->map_to_new, iota.data(), N * sizeof(int), hipMemcpyHostToDevice, s->st));
         if (ns > 1) HTRY(hipMemcpyAsync(s->sid.cur(), hsid.data(), N * sizeof(int), hipMemcpyHostToDevice, s->st));
@@ -929,6 +932,60 @@ bp_status bp_get_rigid_state(bp_solver *s, float *quats, float *cms) {
     for (int b = 0; b < s->nr; ++b) {
         if (quats) { quats[4 * b] = q[b].x; quats[4 * b + 1] = q[b].y; quats[4 * b + 2] = q[b].z; quats[4 * b + 3] = q[b].w; }
         if (cms) { cms[3 * b] = c[b].x; cms[3 * b + 1] = c[b].y; cms[3 * b + 2] = c[b].z; }
+    }
+    return BP_OK;
+}
+
+// Column-major 4x4 of a rigid body: rotation of the unit quaternion (x, y, z, w),
+// translation = centre of mass (mapMatrices, main.cu:284-291; the terms are
+// summed in the order of extract_rotation_matrix4, cudamatrix.cuh:175-199).
+static void rigid_matrix(const float *q, const float *cm, float *m) {
+    const float x = q[0], y = q[1], z = q[2], w = q[3];
+    const float xx = x * x, yy = y * y, zz = z * z;
+    m[0] = 1.0f - 2.0f * yy - 2.0f * zz;
+    m[1] = 2.0f * x * y + 2.0f * w * z;
+    m[2] = 2.0f * x * z - 2.0f * w * y;
+    m[3] = 0.0f;
+    m[4] = 2.0f * x * y - 2.0f * w * z;
+    m[5] = 1.0f - 2.0f * xx - 2.0f * zz;
+    m[6] = 2.0f * y * z + 2.0f * w * x;
+    m[7] = 0.0f;
+    m[8] = 2.0f * x * z + 2.0f * w * y;
+    m[9] = 2.0f * y * z - 2.0f * w * x;
+    m[10] = 1.0f - 2.0f * xx - 2.0f * yy;
+    m[11] = 0.0f;
+    m[12] = cm[0];
+    m[13] = cm[1];
+    m[14] = cm[2];
+    m[15] = 1.0f;
+}
+
+bp_status bp_map_particle_infos(bp_solver *s, void *device_records) {
+    if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
+    if (s->N == 0) return BP_OK;
+    if (!device_records) return fail(BP_ERR_INVALID_ARG, "device_records is NULL");
+    launch_map_particle_infos(s->x.cur(), s->orig.cur(), s->group, (float4 *)device_records, s->N, s->st);
+    HIP_TRY(hipGetLastError());
+    return BP_OK;
+}
+
+bp_status bp_export_frame(bp_solver *s, float *particles, float *matrices) {
+    if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
+    const int N = s->N;
+    if (particles && N > 0) {
+        if (!s->frame_buf) {
+            bp_status r = s->alloc(&s->frame_buf, N);
+            if (r != BP_OK) return r;
+        }
+        launch_map_particle_infos(s->x.cur(), s->orig.cur(), s->group, s->frame_buf, N, s->st);
+        HIP_TRY(hipMemcpyAsync(particles, s->frame_buf, N * sizeof(float4), hipMemcpyDeviceToHost, s->st));
+        HIP_TRY(hipStreamSynchronize(s->st));
+    }
+    if (matrices && s->nr > 0) {
+        std::vector<float> q(4 * (size_t)s->nr), c(3 * (size_t)s->nr);
+        bp_status r = bp_get_rigid_state(s, q.data(), c.data());
+        if (r != BP_OK) return r;
+        for (int b = 0; b < s->nr; ++b) rigid_matrix(&q[4 * (size_t)b], &c[3 * (size_t)b], matrices + 16 * (size_t)b);
     }
     return BP_OK;
 }

@@ -80,6 +80,8 @@ struct bp_solver {
     float *lambda = nullptr, *density = nullptr;
     uint32_t *ccell = nullptr;
     uint32_t *srec = nullptr;  // screen record of the lambda and omega/normal passes (kRecChunks x N masks)
+    int *group = nullptr;      // render group id by original id (bp_scene_desc::group_ids, else 0)
+    float4 *frame_buf = nullptr;  // bp_export_frame staging, allocated on first use
     float4 *sum_tmp = nullptr;
     unsigned long long *corr = nullptr;  // fixed-point pair corrections, kept zero between passes
     float4 *omega = nullptr, *normal = nullptr;  // normal.w: the density it was computed with

@@ -167,6 +167,19 @@ bp_status bp_copy_state_original_order(bp_solver *s, float *pos, float *vel);
 bp_status bp_get_state(bp_solver *s, float *pos, float *vel, int *map_to_original);
 bp_status bp_set_state(bp_solver *s, const float *pos, const float *vel,
                        const int *map_to_original);
+/* The renderer boundary, headless (main.cu:273-291; the reference maps a GL
+ * buffer and launches mapParticleInfos and mapMatrices into it).  One record
+ * per particle in original-id order: {x, y, z, group id}, the group id's int
+ * bits in the fourth float (bp_scene_desc::group_ids, 0 when absent).  One
+ * column-major 4x4 per rigid body: the rotation of its quaternion and its
+ * centre of mass as translation (extract_rotation_matrix4,
+ * cudamatrix.cuh:175-199).
+ * bp_export_frame: particles [4N] and matrices [16 * rigid bodies] are host
+ * buffers, either may be NULL.  bp_map_particle_infos: the particle records
+ * into a device buffer of 16 N bytes on the solver's device (a mapped
+ * graphics buffer, say), ordered on the solver's stream. */
+bp_status bp_export_frame(bp_solver *s, float *particles, float *matrices);
+bp_status bp_map_particle_infos(bp_solver *s, void *device_records);
 /* Rigid-body warm-start state: quats [4*nr] (x,y,z,w), cms [3*nr]; either may be NULL. */
 bp_status bp_get_rigid_state(bp_solver *s, float *quats, float *cms);
 bp_status bp_set_rigid_state(bp_solver *s, const float *quats);

@@ -247,9 +247,20 @@ def test_cpp_api_example_runs():
     exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "brass-pan_amd", "bin",
                        "dam_break")
     case("C++ API example")
-    r = subprocess.run([exe, "10"], capture_output=True, text=True, timeout=120)
-    assert r.returncode == 0, r.stderr
-    assert "particles 4096 frames 10" in r.stdout
+    import tempfile
+
+    import frames
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "last.bpfr")
+        r = subprocess.run([exe, "10", path], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert "particles 4096 frames 10" in r.stdout
+        fr = frames.read_frame(path)  # the C++ writer and the Python reader agree on the format
+        assert fr["frame"] == 10 and abs(fr["time"] - 10 / 60) < 1e-12
+        assert fr["positions"].shape == (4096, 3) and fr["matrices"].shape == (0, 4, 4)
+        ymean = float(fr["positions"][:, 1].astype(np.float64).mean())
+        printed = float(r.stdout.split("mean height ")[1].split()[0])
+        assert abs(printed - ymean) <= 1e-4
     # batch of three flags and the gradient of a final-state loss (BatchParticleSolver)
     r = subprocess.run([exe.replace("dam_break", "batch_and_grad")], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
