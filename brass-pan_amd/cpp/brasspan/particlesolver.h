@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../../include/brasspan.h"
+#include "oldscene.h"
 #include "scene.h"
 
 // Flat C-ABI description of a Scene (keeps the arrays it points into).
@@ -154,7 +155,22 @@ struct ParticleSolver : SolverHandle {
                             void *hipStream = nullptr) {
         bp_params p;
         if (params) p = *params; else check(bp_default_params(&p));
-        const Scene &s = *scene;
+        init(*scene, p, hipStream);
+    }
+    // particlesolver.cuh:218-342: the legacy scene format, assembled as the
+    // reference's constructor assembles it (oldscene.h), with its fluid rest
+    // density (:338; a scene without fluid leaves it unset, and only fluid
+    // reads it); the scene's counters advance as the parts are added.
+    explicit ParticleSolver(const std::shared_ptr<OldSceneFormat> &oldScene, const bp_params *params = nullptr,
+                            void *hipStream = nullptr) {
+        bp_params p;
+        if (params) p = *params; else check(bp_default_params(&p));
+        if (oldScene->fluidRestDensity > 0.0f || !oldScene->fluids.empty())
+            p.fluid_rest_density = oldScene->fluidRestDensity;
+        init(assembleOldScene(*oldScene), p, hipStream);
+    }
+
+    void init(const Scene &s, const bp_params &p, void *hipStream) {
         SceneDesc d(s);
         check(bp_solver_create(&d.d, &p, hipStream, &handle));
         numParticles = (size_t)d.d.num_particles;

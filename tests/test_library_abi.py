@@ -94,7 +94,7 @@ def test_create_without_device_fails_loudly():
         bp.Solver(small_scene(grid_dims=(64, 64, 64)))
 
 
-EXAMPLES = [os.path.join(ROOT, "brass-pan_amd", "bin", name) for name in ("dam_break", "batch_and_grad")]
+EXAMPLES = [os.path.join(ROOT, "brass-pan_amd", "bin", name) for name in ("dam_break", "batch_and_grad", "legacy_scenes")]
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK), reason="device present")
