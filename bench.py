@@ -131,7 +131,7 @@ def hip_stream():
     """A new HIP stream at the default priority (a raw handle for the C ABI)."""
     import ctypes as C
 
-    hip = C.CDLL("libamdhip64.so")
+    hip = bp.hip_runtime()
     h = C.c_void_p()
     assert hip.hipStreamCreate(C.byref(h)) == 0
     return h.value
@@ -271,7 +271,7 @@ def copy_bandwidth(mib=1024, reps=5):
     plus the bytes written."""
     import ctypes as C
 
-    hip = C.CDLL("libamdhip64.so")
+    hip = bp.hip_runtime()
     n = mib << 20
     a, b = C.c_void_p(), C.c_void_p()
     assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
@@ -300,7 +300,7 @@ def copy_calibration(mib):
     WRITE_SIZE readings of the solver kernels."""
     import ctypes as C
 
-    hip = C.CDLL("libamdhip64.so")
+    hip = bp.hip_runtime()
     n = mib << 20
     a, b = C.c_void_p(), C.c_void_p()
     assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0

@@ -123,6 +123,17 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     return lib
 
 
+def hip_runtime() -> C.CDLL:
+    """The HIP runtime libbrasspan.so is bound to, for callers that hand it
+    raw device buffers or streams. Opened by its soname after the library is
+    loaded, so the dynamic linker returns the copy already in the process
+    (torch ships its own with the same soname; opening another file by path
+    would start a second runtime that does not share the library's device
+    state)."""
+    load_library()
+    return C.CDLL("libamdhip64.so.7")
+
+
 def _check(status: int):
     if status != BP_OK:
         msg = load_library().bp_last_error_string().decode(errors="replace")

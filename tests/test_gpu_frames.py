@@ -50,7 +50,7 @@ def test_export_frame_is_the_state():
     case("map particle infos into a device buffer")
     import ctypes as C
 
-    hip = C.CDLL("libamdhip64.so")  # the HIP runtime the library uses
+    hip = bp.hip_runtime()
     nbytes = C.c_size_t(16 * s.n)
     dev = C.c_void_p()
     assert hip.hipMalloc(C.byref(dev), nbytes) == 0
