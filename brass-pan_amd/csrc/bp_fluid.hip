@@ -9,12 +9,14 @@
 
 namespace bp {
 
-// fluidLambda (fluid.cuh:78-159).  Also records the particle's current cell
-// (packed) for the position pass, which searches from the same x*.
+// fluidLambda (fluid.cuh:78-159).  Also writes the record the position pass
+// reads for this particle as self and as partner, (lambda, m, current cell
+// packed, 0), so that pass fetches two words-of-four per partner (x* and this)
+// instead of four separate arrays; it searches from the same x* and cell.
 template <class SP>
 __global__ __launch_bounds__(kMaxBlock) void k_lambda(const float4 *__restrict__ xs, const float *__restrict__ m,
                                                    const int2 *__restrict__ ranges, float *__restrict__ lambda,
-                                                   float *__restrict__ density, uint32_t *__restrict__ ccell,
+                                                   float *__restrict__ density, float4 *__restrict__ lam_rec,
                                                    uint32_t *__restrict__ srec, int n, SP sp) {
     const int i = xcd_index();
     if (i >= n) return;
@@ -26,10 +28,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_lambda(const float4 *__restrict__
         const f3 pi = xyz(a);
         int g[3];
         G.pos(pi, g);
-        ccell[i] = G.pack(g);
+        const float cell_bits = __uint_as_float(G.pack(g));
         if (phase_of(a) > 0) {  // solids: lambda 0 (the reference never writes theirs)
             lambda[i] = 0.0f;
             density[i] = 0.0f;
+            lam_rec[i] = make_float4(0.0f, m[i], cell_bits, 0.0f);
             return;
         }
         float dens = 0.0f, sumg2 = 0.0f;
@@ -68,8 +71,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_lambda(const float4 *__restrict__
         if (F.lambda_mode == 1) sumg2 += dot(gi, gi);
         float C = dens / F.rest_density - 1.0f;
         if (F.akinci) C = fmaxf(C, 0.0f);
-        lambda[i] = (F.lambda_mode == 1) ? -C / (sumg2 + F.lambda_relaxation) : -C / F.lambda_denominator;
+        const float lam = (F.lambda_mode == 1) ? -C / (sumg2 + F.lambda_relaxation) : -C / F.lambda_denominator;
+        lambda[i] = lam;
         density[i] = dens;
+        lam_rec[i] = make_float4(lam, m[i], cell_bits, 0.0f);
     });
 }
 
@@ -103,9 +108,8 @@ __device__ __forceinline__ unsigned long long to_fixed(float v) {
 // A pair k owns whose partner j's search does not reach k's cell is delivered
 // to j through corr[j] (fixed point).  k_position_apply adds both parts.
 template <class SP>
-__global__ __launch_bounds__(kMaxBlock) void k_position(const float4 *__restrict__ xs, const float *__restrict__ m,
-                                                     const float *__restrict__ lambda,
-                                                     const uint32_t *__restrict__ ccell,
+__global__ __launch_bounds__(kMaxBlock) void k_position(const float4 *__restrict__ xs,
+                                                     const float4 *__restrict__ lam_rec,
                                                      const int *__restrict__ build_keys,
                                                      const int2 *__restrict__ ranges,
                                                      const uint32_t *__restrict__ srec,
@@ -121,9 +125,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_position(const float4 *__restrict
         const float4 a = xs[k];
         const f3 pk = xyz(a);
         const bool kf = phase_of(a) <= 0;
-        const float lk = lambda[k];
-        const float mk = m[k];
-        const uint32_t ck = ccell[k];
+        const float4 rk = lam_rec[k];  // (lambda, m, cell, -) from the lambda pass
+        const float lk = rk.x;
+        const float mk = rk.y;
+        const uint32_t ck = __float_as_uint(rk.z);
         const uint32_t bk = G.pack_addr(build_keys[k]);
         int g[3];
         unpack_cell(ck, g);
@@ -136,8 +141,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_position(const float4 *__restrict
         }
         struct Partner {
             float4 b;
-            uint32_t cell;
-            float lam, m;
+            float4 r;  // (lambda, m, cell, -)
         };
         ScreenRec rec;  // the lambda pass screened fluid particles at this x*
         rec.in = kf ? srec : nullptr;
@@ -148,16 +152,16 @@ __global__ __launch_bounds__(kMaxBlock) void k_position(const float4 *__restrict
             // (fluid particles replay the lambda pass's masks; a solid screens for
             // fluid partners only)
             [&](int, float4 b) { return (dist2(pk, xyz(b)) <= K.h2) & (kf | (phase_of(b) <= 0)); },
-            [&](int j) { return Partner{xs[j], ccell[j], lambda[j], m[j]}; },
+            [&](int j) { return Partner{xs[j], lam_rec[j]}; },
             [&](int j, const Partner &P) {
                 const float4 b = P.b;
                 // coincident partners are skipped; a solid partner's pair is k's only if j > k
                 if (!(dist2(pk, xyz(b)) > 0.0f) | ((phase_of(b) > 0) & !(kf & (j > k)))) return;
-                const bool partner_sees_k = G.in_block(P.cell, bk);
+                const bool partner_sees_k = G.in_block(__float_as_uint(P.r.z), bk);
                 if (j < k && !partner_sees_k) return;
                 const f3 diff = pk - xyz(b);
-                const float c = pair_coef(dist2(pk, xyz(b)), lk + P.lam, K, F);
-                const float cj = c * P.m;
+                const float c = pair_coef(dist2(pk, xyz(b)), lk + P.r.x, K, F);
+                const float cj = c * P.r.y;
                 sum.x = fmaf(cj, diff.x, sum.x);
                 sum.y = fmaf(cj, diff.y, sum.y);
                 sum.z = fmaf(cj, diff.z, sum.z);
@@ -381,17 +385,17 @@ __global__ __launch_bounds__(kMaxBlock) void k_xsph(const float4 *__restrict__ x
 // ------------------------------------------------------------ launchers
 template <class SP>
 void launch_lambda(const float4 *xs, const float *m, const int2 *ranges, float *lambda, float *density,
-                   uint32_t *ccell, uint32_t *srec, int n, const SP &sp, int block, hipStream_t st) {
+                   float4 *lam_rec, uint32_t *srec, int n, const SP &sp, int block, hipStream_t st) {
     hipLaunchKernelGGL(k_lambda<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, m, ranges, lambda, density,
-                       ccell, srec, n, sp);
+                       lam_rec, srec, n, sp);
 }
 template <class SP>
-void launch_position(const float4 *xs, const float *m, const float *lambda, const uint32_t *ccell,
-                     const int *build_keys, const int2 *ranges, const uint32_t *srec, const uint32_t *fluid_stamp,
-                     const uint32_t *stamp, float4 *sum_tmp, unsigned long long *corr, float4 *xs_out, float4 *x_io,
-                     float4 *v_out, int n, const SP &sp, const Epilogue &E, int block, hipStream_t st) {
-    hipLaunchKernelGGL(k_position<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, m, lambda, ccell,
-                       build_keys, ranges, srec, fluid_stamp, stamp, sum_tmp, corr, n, sp);
+void launch_position(const float4 *xs, const float4 *lam_rec, const int *build_keys, const int2 *ranges,
+                     const uint32_t *srec, const uint32_t *fluid_stamp, const uint32_t *stamp, float4 *sum_tmp,
+                     unsigned long long *corr, float4 *xs_out, float4 *x_io, float4 *v_out, int n, const SP &sp,
+                     const Epilogue &E, int block, hipStream_t st) {
+    hipLaunchKernelGGL(k_position<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, lam_rec, build_keys,
+                       ranges, srec, fluid_stamp, stamp, sum_tmp, corr, n, sp);
     hipLaunchKernelGGL(k_position_apply<SP>, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, sum_tmp, corr, xs_out,
                        x_io, v_out, n, sp, E);
 }
@@ -417,12 +421,12 @@ void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const ui
 }
 
 #define BP_INSTANTIATE_FLUID(SP)                                                                                \
-    template void launch_lambda<SP>(const float4 *, const float *, const int2 *, float *, float *, uint32_t *,  \
+    template void launch_lambda<SP>(const float4 *, const float *, const int2 *, float *, float *, float4 *,    \
                                     uint32_t *, int, const SP &, int, hipStream_t);                             \
-    template void launch_position<SP>(const float4 *, const float *, const float *, const uint32_t *,           \
-                                      const int *, const int2 *, const uint32_t *, const uint32_t *,            \
-                                      const uint32_t *, float4 *, unsigned long long *, float4 *, float4 *,     \
-                                      float4 *, int, const SP &, const Epilogue &, int, hipStream_t);           \
+    template void launch_position<SP>(const float4 *, const float4 *, const int *, const int2 *,                \
+                                      const uint32_t *, const uint32_t *, const uint32_t *, float4 *,           \
+                                      unsigned long long *, float4 *, float4 *, float4 *, int, const SP &,      \
+                                      const Epilogue &, int, hipStream_t);                                      \
     template void launch_post_a<SP>(const float4 *, const float4 *, const float *, const int2 *, float4 *,      \
                                     float *, float4 *, uint32_t *, uint32_t *, int, const SP &, int, int, int,  \
                                     hipStream_t);                                                               \

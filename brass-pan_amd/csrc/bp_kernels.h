@@ -29,12 +29,12 @@ void launch_reorder(const int *perm, const float4 *x_in, const float4 *v_in, con
 // and by the vorticity/tension and XSPH passes.
 template <class SP>
 void launch_lambda(const float4 *xs, const float *m, const int2 *ranges, float *lambda, float *density,
-                   uint32_t *ccell, uint32_t *srec, int n, const SP &sp, int block, hipStream_t st);
+                   float4 *lam_rec, uint32_t *srec, int n, const SP &sp, int block, hipStream_t st);
 template <class SP>
-void launch_position(const float4 *xs, const float *m, const float *lambda, const uint32_t *ccell,
-                     const int *build_keys, const int2 *ranges, const uint32_t *srec, const uint32_t *fluid_stamp,
-                     const uint32_t *stamp, float4 *sum_tmp, unsigned long long *corr, float4 *xs_out, float4 *x_io,
-                     float4 *v_out, int n, const SP &sp, const Epilogue &E, int block, hipStream_t st);
+void launch_position(const float4 *xs, const float4 *lam_rec, const int *build_keys, const int2 *ranges,
+                     const uint32_t *srec, const uint32_t *fluid_stamp, const uint32_t *stamp, float4 *sum_tmp,
+                     unsigned long long *corr, float4 *xs_out, float4 *x_io, float4 *v_out, int n, const SP &sp,
+                     const Epilogue &E, int block, hipStream_t st);
 template <class SP>
 void launch_post_a(const float4 *xs, const float4 *v, const float *density, const int2 *ranges, float4 *omega,
                    float *omag, float4 *normal, uint32_t *ccell, uint32_t *srec, int n, const SP &sp, int do_omega,

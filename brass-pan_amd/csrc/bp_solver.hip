@@ -211,7 +211,7 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
         }
         if (H.xl) HIP_TRY(snap(H.xl[it], s->xs.cur(), N, st));
         if (s->has_fluid) {
-            TIMED(F_LAMBDA, launch_lambda(s->xs.cur(), s->m.cur(), s->ranges, s->lambda, s->density, s->ccell,
+            TIMED(F_LAMBDA, launch_lambda(s->xs.cur(), s->m.cur(), s->ranges, s->lambda, s->density, s->lam_rec,
                                           H.rec ? H.rec[it] : s->srec, N, sp(), s->blk_lambda, st));
             if (H.lam) HIP_TRY(snap(H.lam[it], s->lambda, N, st));
             Epilogue E{};
@@ -219,10 +219,9 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
             E.finish = (!no_fuse && it + 1 == iters && !tail_groups) ? 1 : 0;
             E.inv_dt = inv_dt;
             E.sleep_eps2 = P.sleep_epsilon * P.sleep_epsilon;
-            TIMED(F_POSITION, launch_position(s->xs.cur(), s->m.cur(), s->lambda, s->ccell, sk, s->ranges,
-                                              H.rec ? H.rec[it] : s->srec, s->fluid_stamp, s->stamp, s->sum_tmp,
-                                              s->corr, s->xs.alt(), s->x.cur(), s->v.cur(), N, sp(), E,
-                                              s->blk_position, st));
+            TIMED(F_POSITION, launch_position(s->xs.cur(), s->lam_rec, sk, s->ranges, H.rec ? H.rec[it] : s->srec,
+                                              s->fluid_stamp, s->stamp, s->sum_tmp, s->corr, s->xs.alt(),
+                                              s->x.cur(), s->v.cur(), N, sp(), E, s->blk_position, st));
             s->xs.flip();
             planes_done = E.planes_next;
             finished = E.finish;
@@ -524,6 +523,7 @@ bp_status create_impl(const bp_scene_desc *scs, const bp_params *params, int ns,
     TRY(s->alloc(&s->ccell, N));
     TRY(s->alloc(&s->group, N));
     if (s->has_fluid) TRY(s->alloc(&s->srec, (size_t)kRecChunks * N));
+    if (s->has_fluid) TRY(s->alloc(&s->lam_rec, N));
     TRY(s->alloc(&s->sum_tmp, N));
     TRY(s->alloc(&s->corr, 3 * (size_t)N));
     TRY(s->alloc(&s->omega, N));
