@@ -370,13 +370,14 @@ __global__ void k_pick(float4 *__restrict__ pos, float4 *__restrict__ v, const i
 
 // updateVelocity + updatePositions (particlesolver.cuh:71-81,121-137)
 __global__ __launch_bounds__(kBlock) void k_finish(const float4 *__restrict__ xs, float4 *__restrict__ x,
-                                                   float4 *__restrict__ v, int n, float inv_dt, float eps2) {
+                                                   float4 *__restrict__ v, const float *__restrict__ vw, int n,
+                                                   float inv_dt, float eps2) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 a = xs[i];
     const float4 x4 = x[i];
     const f3 out = xyz(a), xp = xyz(x4);
-    v[i] = f4((out - xp) * inv_dt, 0.0f);
+    v[i] = f4((out - xp) * inv_dt, (vw && phase_of(a) < 0) ? vw[i] : 0.0f);  // v.w as in the position epilogue
     const float d2 = length2(out - xp);
     if (d2 >= eps2 || phase_of(a) < 0) x[i] = f4(out, x4.w);
 }
@@ -434,8 +435,9 @@ void launch_pick(float4 *pos, float4 *v, const int *map_to_new, int orig, f3 p, 
                  hipStream_t st) {
     hipLaunchKernelGGL(k_pick, dim3(1), dim3(64), 0, st, pos, v, map_to_new, orig, p, keep_pos, vel);
 }
-void launch_finish(const float4 *xs, float4 *x, float4 *v, int n, float inv_dt, float eps2, hipStream_t st) {
-    hipLaunchKernelGGL(k_finish, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, x, v, n, inv_dt, eps2);
+void launch_finish(const float4 *xs, float4 *x, float4 *v, const float *vw, int n, float inv_dt, float eps2,
+                   hipStream_t st) {
+    hipLaunchKernelGGL(k_finish, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, x, v, vw, n, inv_dt, eps2);
 }
 
 #define BP_INSTANTIATE_CONSTRAINTS(SP)                                                                          \

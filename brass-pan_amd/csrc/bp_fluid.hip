@@ -208,7 +208,7 @@ __global__ __launch_bounds__(kBlock) void k_position_apply(const float4 *__restr
         const f3 xp = xyz(x4);
         if (E.planes_next) out = plane_collide_one(out, xp, S.pl, S.radius, S.friction_static, S.friction_dynamic);
         if (E.finish) {  // updateVelocity + updatePositions (particlesolver.cuh:71-81,121-137)
-            v_out[k] = f4((out - xp) * E.inv_dt, 0.0f);
+            v_out[k] = f4((out - xp) * E.inv_dt, (E.vw && phase_of(a) < 0) ? E.vw[k] : 0.0f);
             const float d2 = length2(out - xp);
             if (d2 >= E.sleep_eps2 || phase_of(a) < 0) x_io[k] = f4(out, x4.w);
         }
@@ -222,7 +222,7 @@ template <class SP>
 __global__ __launch_bounds__(kMaxBlock) void k_post_a(const float4 *__restrict__ xs, const float4 *__restrict__ v,
                                                    const float *__restrict__ density,
                                                    const int2 *__restrict__ ranges, float4 *__restrict__ omega,
-                                                   float *__restrict__ omag,
+                                                   float4 *__restrict__ xom,
                                                    float4 *__restrict__ normal, uint32_t *__restrict__ ccell,
                                                    uint32_t *__restrict__ srec, int n, SP sp, int do_omega,
                                                    int do_normal) {
@@ -238,6 +238,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_post_a(const float4 *__restrict__
         int g[3];
         G.pos(pi, g);
         ccell[i] = G.pack(g);
+        // v.w holds the density (written with v for this pass), so a
+        // partner's velocity and density are one fetch
         const f3 vi = xyz(v[i]);
         f3 om = mk3(0.f, 0.f, 0.f), nn = mk3(0.f, 0.f, 0.f);
         ScreenRec rec;  // replayed by the vorticity/tension and XSPH passes
@@ -252,17 +254,16 @@ __global__ __launch_bounds__(kMaxBlock) void k_post_a(const float4 *__restrict__
                 const f3 diff = pi - xyz(b);
                 const float r = fsqrt(dist2(pi, xyz(b)));
                 const f3 gr = K.grad_coef(r, frcp(r)) * diff;
-                if (do_omega) om += cross(xyz(v[j]) - vi, gr);
-                if (do_normal) nn += frcp(density[j]) * gr;
+                const float4 vj = v[j];
+                if (do_omega) om += cross(xyz(vj) - vi, gr);
+                if (do_normal) nn += frcp(vj.w) * gr;
             },
             AllowNotSelf{i}, rec);
-        // |omega| in its own array and the density in normal.w: the
-        // vorticity/tension pass then fetches 36 B per partner, not 52
-        if (do_omega) {
-            const float mag = fsqrt(dot(om, om));
-            omega[i] = f4(om, mag);
-            omag[i] = mag;
-        }
+        // (x, |omega|) and (normal, density) are the two records the
+        // vorticity/tension pass fetches per partner
+        const float mag = do_omega ? fsqrt(dot(om, om)) : 0.0f;
+        if (do_omega) omega[i] = f4(om, mag);
+        xom[i] = f4(pi, mag);
         if (do_normal) normal[i] = f4(K.h * nn, density[i]);
     });
 }
@@ -273,7 +274,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_post_a(const float4 *__restrict__
 template <class SP>
 __global__ __launch_bounds__(kMaxBlock) void k_post_b(const float4 *__restrict__ xs, float4 *__restrict__ v,
                                                    const float4 *__restrict__ omega,
-                                                   const float *__restrict__ omag,
+                                                   const float4 *__restrict__ xom,
                                                    const float4 *__restrict__ normal,
                                                    const float *__restrict__ density,
                                                    const int2 *__restrict__ ranges,
@@ -301,23 +302,21 @@ __global__ __launch_bounds__(kMaxBlock) void k_post_b(const float4 *__restrict__
         rec.slot = i;
         rec.n = n;
         struct Partner {
-            float4 b, nrm;  // nrm.w: the partner's density
-            float om;       // |omega| of the partner
+            float4 xo;   // (x, |omega|) of the partner
+            float4 nrm;  // nrm.w: the partner's density
         };
         const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
         for_each_neighbor_nb<kBodyBatch>(
             G, ranges, g, xs, 0,
             [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
-            [&](int j) {
-                return Partner{xs[j], do_akinci ? normal[j] : zero4, do_vort ? omag[j] : 0.0f};
-            },
+            [&](int j) { return Partner{xom[j], do_akinci ? normal[j] : zero4}; },
             [&](int, const Partner &P) {
-                const float4 b = P.b;
+                const float4 b = P.xo;
                 if (!(dist2(pi, xyz(b)) > 0.0f)) return;  // coincident partners are skipped
                 const f3 diff = pi - xyz(b);
                 const float r = fsqrt(dist2(pi, xyz(b)));
                 const float ir = frcp(r);
-                if (do_vort) eta += (P.om * K.grad_coef(r, ir)) * diff;
+                if (do_vort) eta += (b.w * K.grad_coef(r, ir)) * diff;
                 if (do_akinci) {
                     // k_ij (-gamma C(r) diff / r - gamma (n_i - n_j)) (fluid.cuh:476-483)
                     const float kij = 2.0f * F.rest_density * frcp(rhoi + P.nrm.w);
@@ -401,16 +400,16 @@ void launch_position(const float4 *xs, const float4 *lam_rec, const int *build_k
 }
 template <class SP>
 void launch_post_a(const float4 *xs, const float4 *v, const float *density, const int2 *ranges, float4 *omega,
-                   float *omag, float4 *normal, uint32_t *ccell, uint32_t *srec, int n, const SP &sp, int do_omega,
+                   float4 *xom, float4 *normal, uint32_t *ccell, uint32_t *srec, int n, const SP &sp, int do_omega,
                    int do_normal, int block, hipStream_t st) {
     hipLaunchKernelGGL(k_post_a<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, v, density, ranges, omega,
-                       omag, normal, ccell, srec, n, sp, do_omega, do_normal);
+                       xom, normal, ccell, srec, n, sp, do_omega, do_normal);
 }
 template <class SP>
-void launch_post_b(const float4 *xs, float4 *v, const float4 *omega, const float *omag, const float4 *normal,
+void launch_post_b(const float4 *xs, float4 *v, const float4 *omega, const float4 *xom, const float4 *normal,
                    const float *density, const int2 *ranges, const uint32_t *ccell, const uint32_t *srec, int n,
                    const SP &sp, int do_vort, int do_akinci, float dt, int block, hipStream_t st) {
-    hipLaunchKernelGGL(k_post_b<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, v, omega, omag, normal,
+    hipLaunchKernelGGL(k_post_b<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, v, omega, xom, normal,
                        density, ranges, ccell, srec, n, sp, do_vort, do_akinci, dt);
 }
 template <class SP>
@@ -428,9 +427,9 @@ void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const ui
                                       unsigned long long *, float4 *, float4 *, float4 *, int, const SP &,      \
                                       const Epilogue &, int, hipStream_t);                                      \
     template void launch_post_a<SP>(const float4 *, const float4 *, const float *, const int2 *, float4 *,      \
-                                    float *, float4 *, uint32_t *, uint32_t *, int, const SP &, int, int, int,  \
+                                    float4 *, float4 *, uint32_t *, uint32_t *, int, const SP &, int, int, int, \
                                     hipStream_t);                                                               \
-    template void launch_post_b<SP>(const float4 *, float4 *, const float4 *, const float *, const float4 *,    \
+    template void launch_post_b<SP>(const float4 *, float4 *, const float4 *, const float4 *, const float4 *,   \
                                     const float *, const int2 *, const uint32_t *, const uint32_t *, int,       \
                                     const SP &, int, int, float, int, hipStream_t);                             \
     template void launch_xsph<SP>(const float4 *, const float4 *, const int2 *, const uint32_t *,               \

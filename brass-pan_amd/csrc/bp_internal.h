@@ -146,6 +146,9 @@ struct Epilogue {
     int finish;           // last iteration: updateVelocity + updatePositions
     float inv_dt;
     float sleep_eps2;
+    // with finish: v.w = vw[k] for fluid particles (the density, read by the
+    // omega/normal pass; the vorticity/tension pass sets it back to 0), else 0
+    const float *vw = nullptr;
 };
 
 // Everything a kernel needs to know about the scene a particle belongs to.

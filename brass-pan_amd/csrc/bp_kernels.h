@@ -37,10 +37,10 @@ void launch_position(const float4 *xs, const float4 *lam_rec, const int *build_k
                      const Epilogue &E, int block, hipStream_t st);
 template <class SP>
 void launch_post_a(const float4 *xs, const float4 *v, const float *density, const int2 *ranges, float4 *omega,
-                   float *omag, float4 *normal, uint32_t *ccell, uint32_t *srec, int n, const SP &sp, int do_omega,
+                   float4 *xom, float4 *normal, uint32_t *ccell, uint32_t *srec, int n, const SP &sp, int do_omega,
                    int do_normal, int block, hipStream_t st);
 template <class SP>
-void launch_post_b(const float4 *xs, float4 *v, const float4 *omega, const float *omag, const float4 *normal,
+void launch_post_b(const float4 *xs, float4 *v, const float4 *omega, const float4 *xom, const float4 *normal,
                    const float *density, const int2 *ranges, const uint32_t *ccell, const uint32_t *srec, int n,
                    const SP &sp, int do_vort, int do_akinci, float dt, int block, hipStream_t st);
 template <class SP>
@@ -72,7 +72,8 @@ void launch_shape_matching(float4 *xs, const int *map_to_new, const float4 *rest
 void launch_immovable(float4 *xs, const float4 *x, const int *ids, const int *map_to_new, int ni, hipStream_t st);
 void launch_pick(float4 *pos, float4 *v, const int *map_to_new, int orig, f3 p, int keep_pos, f3 vel,
                  hipStream_t st);
-void launch_finish(const float4 *xs, float4 *x, float4 *v, int n, float inv_dt, float eps2, hipStream_t st);
+void launch_finish(const float4 *xs, float4 *x, float4 *v, const float *vw, int n, float inv_dt, float eps2,
+                   hipStream_t st);
 
 // bp_adjoint.hip (adjoint passes; g* buffers hold adjoints in slot order)
 void launch_adj_inverse(const int *orig, int *map_to_new, int n, hipStream_t st);

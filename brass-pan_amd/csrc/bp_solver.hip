@@ -197,6 +197,8 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
         TIMED(F_WIND, launch_wind(s->xs.cur(), s->v.cur(), s->m.cur(), s->faces, s->map_to_new, s->fbuf, s->nf, sp(),
                                   subdt, s->fcsr.verts, s->fcsr.offs, s->fcsr.ents, s->fcsr.nv, st));
 
+    // the omega/normal pass reads a fluid partner's density from v.w
+    const float *vw = (s->has_fluid && (P.vorticity_eps != 0.0f || P.akinci)) ? s->density : nullptr;
     bool planes_done = fuse_first;
     bool finished = false;
     for (int it = 0; it < iters; ++it) {
@@ -219,6 +221,7 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
             E.finish = (!no_fuse && it + 1 == iters && !tail_groups) ? 1 : 0;
             E.inv_dt = inv_dt;
             E.sleep_eps2 = P.sleep_epsilon * P.sleep_epsilon;
+            E.vw = vw;
             TIMED(F_POSITION, launch_position(s->xs.cur(), s->lam_rec, sk, s->ranges, H.rec ? H.rec[it] : s->srec,
                                               s->fluid_stamp, s->stamp, s->sum_tmp, s->corr, s->xs.alt(),
                                               s->x.cur(), s->v.cur(), N, sp(), E, s->blk_position, st));
@@ -241,16 +244,16 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
     }
     HIP_TRY(snap(H.xf, s->xs.cur(), N, st));
     if (!finished)
-        TIMED(F_FINISH, launch_finish(s->xs.cur(), s->x.cur(), s->v.cur(), N, inv_dt,
+        TIMED(F_FINISH, launch_finish(s->xs.cur(), s->x.cur(), s->v.cur(), vw, N, inv_dt,
                                       P.sleep_epsilon * P.sleep_epsilon, st));
 
     if (s->has_fluid) {
         const int do_vort = P.vorticity_eps != 0.0f ? 1 : 0;
         const int do_ak = P.akinci ? 1 : 0;
         if (do_vort || do_ak) {
-            TIMED(F_POST_A, launch_post_a(s->xs.cur(), s->v.cur(), s->density, s->ranges, s->omega, s->omag, s->normal,
+            TIMED(F_POST_A, launch_post_a(s->xs.cur(), s->v.cur(), s->density, s->ranges, s->omega, s->xom, s->normal,
                                           s->ccell, s->srec, N, sp(), do_vort, do_ak, s->blk_post_a, st));
-            TIMED(F_POST_B, launch_post_b(s->xs.cur(), s->v.cur(), s->omega, s->omag, s->normal, s->density, s->ranges,
+            TIMED(F_POST_B, launch_post_b(s->xs.cur(), s->v.cur(), s->omega, s->xom, s->normal, s->density, s->ranges,
                                           s->ccell, s->srec, N, sp(), do_vort, do_ak, subdt, s->blk_post_b, st));
         }
         if (P.xsph_c != 0.0f) {
@@ -527,7 +530,7 @@ bp_status create_impl(const bp_scene_desc *scs, const bp_params *params, int ns,
     TRY(s->alloc(&s->sum_tmp, N));
     TRY(s->alloc(&s->corr, 3 * (size_t)N));
     TRY(s->alloc(&s->omega, N));
-    TRY(s->alloc(&s->omag, N));
+    TRY(s->alloc(&s->xom, N));
     TRY(s->alloc(&s->normal, N));
     if (s->has_solid && s->has_fluid) {
         TRY(s->alloc(&s->cell_stamp, s->num_cells));
@@ -550,7 +553,7 @@ bp_status create_impl(const bp_scene_desc *scs, const bp_params *params, int ns,
         HTRY(hipMemsetAsync(s->lambda, 0, N * sizeof(float), s->st));
         HTRY(hipMemsetAsync(s->density, 0, N * sizeof(float), s->st));
         HTRY(hipMemsetAsync(s->omega, 0, N * sizeof(float4), s->st));
-        HTRY(hipMemsetAsync(s->omag, 0, N * sizeof(float), s->st));
+        HTRY(hipMemsetAsync(s->xom, 0, N * sizeof(float4), s->st));
         HTRY(hipMemsetAsync(s->normal, 0, N * sizeof(float4), s->st));
         HTRY(hipMemsetAsync(s->corr, 0, 3 * (size_t)N * sizeof(unsigned long long), s->st));
     }
