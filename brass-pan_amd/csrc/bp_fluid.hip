@@ -9,6 +9,15 @@
 
 namespace bp {
 
+// x* as three planes x[], y[], z[] (`stride` floats apart) for the lambda
+// pass's distance screen.
+__global__ __launch_bounds__(kBlock) void k_split_planes(const float4 *__restrict__ xs, float *__restrict__ planes,
+                                                         int stride, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    store_planes(planes, stride, i, xyz(xs[i]));
+}
+
 // fluidLambda (fluid.cuh:78-159).  Also writes the record the position pass
 // reads for this particle as self and as partner, (lambda, m, current cell
 // packed, 0), so that pass fetches two words-of-four per partner (x* and this)
@@ -17,7 +26,8 @@ template <class SP>
 __global__ __launch_bounds__(kMaxBlock) void k_lambda(const float4 *__restrict__ xs, const float *__restrict__ m,
                                                    const int2 *__restrict__ ranges, float *__restrict__ lambda,
                                                    float *__restrict__ density, float4 *__restrict__ lam_rec,
-                                                   uint32_t *__restrict__ srec, int n, SP sp) {
+                                                   uint32_t *__restrict__ srec, const float *__restrict__ planes,
+                                                   int pstride, int n, SP sp) {
     const int i = xcd_index();
     if (i >= n) return;
     sp.with_scene(i, [&](const SceneConst &S) {
@@ -49,6 +59,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_lambda(const float4 *__restrict__
         rec.out = srec;
         rec.slot = i;
         rec.n = n;
+        rec.planes = planes;  // the screen tests distance only
+        rec.pstride = pstride;
         for_each_neighbor_nb<kBodyBatch>(
             G, ranges, g, xs, srec ? 0 : i, [&](int, float4 b) { return dist2(pi, xyz(b)) <= K.h2; },
             [&](int j) { return Partner{xs[j], m[j]}; },
@@ -214,6 +226,7 @@ __global__ __launch_bounds__(kBlock) void k_position_apply(const float4 *__restr
         }
     }
     xs_out[k] = f4(out, a.w);
+    store_planes(E.planes, E.pstride, k, out);
 }
 
 // fluidOmega (fluid.cuh:231-279) and fluidNormal (fluid.cuh:387-434) in one
@@ -384,9 +397,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_xsph(const float4 *__restrict__ x
 // ------------------------------------------------------------ launchers
 template <class SP>
 void launch_lambda(const float4 *xs, const float *m, const int2 *ranges, float *lambda, float *density,
-                   float4 *lam_rec, uint32_t *srec, int n, const SP &sp, int block, hipStream_t st) {
+                   float4 *lam_rec, uint32_t *srec, float *planes, int pstride, int planes_ready, int n, const SP &sp,
+                   int block, hipStream_t st) {
+    if (planes && !planes_ready)
+        hipLaunchKernelGGL(k_split_planes, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, planes, pstride, n);
     hipLaunchKernelGGL(k_lambda<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, m, ranges, lambda, density,
-                       lam_rec, srec, n, sp);
+                       lam_rec, srec, planes, pstride, n, sp);
 }
 template <class SP>
 void launch_position(const float4 *xs, const float4 *lam_rec, const int *build_keys, const int2 *ranges,
@@ -421,7 +437,7 @@ void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const ui
 
 #define BP_INSTANTIATE_FLUID(SP)                                                                                \
     template void launch_lambda<SP>(const float4 *, const float *, const int2 *, float *, float *, float4 *,    \
-                                    uint32_t *, int, const SP &, int, hipStream_t);                             \
+                                    uint32_t *, float *, int, int, int, const SP &, int, hipStream_t);          \
     template void launch_position<SP>(const float4 *, const float4 *, const int *, const int2 *,                \
                                       const uint32_t *, const uint32_t *, const uint32_t *, float4 *,           \
                                       unsigned long long *, float4 *, float4 *, float4 *, int, const SP &,      \

@@ -149,6 +149,10 @@ struct Epilogue {
     // with finish: v.w = vw[k] for fluid particles (the density, read by the
     // omega/normal pass; the vorticity/tension pass sets it back to 0), else 0
     const float *vw = nullptr;
+    // x* out also as position planes (ScreenRec::planes) when the next lambda
+    // pass screens this x* (nothing else moves it in between), else null
+    float *planes = nullptr;
+    int pstride = 0;
 };
 
 // Everything a kernel needs to know about the scene a particle belongs to.
@@ -315,6 +319,11 @@ struct ScreenRec {
     uint32_t *out = nullptr;       // record this walk's masks (every chunk is screened)
     const uint32_t *in = nullptr;  // replay recorded masks
     int slot = 0, n = 0;
+    // Optional copy of the positions as three planes x[], y[], z[], `pstride`
+    // floats apart, for a screen that tests positions only: 8 candidates are
+    // then 6 loads of 4 floats rather than 8 records (pick sees w = 0).
+    const float *planes = nullptr;
+    int pstride = 0;
 };
 
 // True iff one of the 27 cells around g carries stamp s in `table` (a cell
@@ -341,9 +350,18 @@ __device__ __forceinline__ bool stamp_near(const GridDesc &G, const int g[3], co
 //
 // Candidates are loaded in batches of 8 from one base address with immediate
 // offsets.  A batch may read up to 7 slots past the end of its run, so every
-// x* buffer a walk reads holds kWalkPad slots beyond the last particle; the
-// bits of those slots are cleared once per chunk.
+// x* buffer (and plane) a walk reads holds kWalkPad slots beyond the last
+// particle; the bits of those slots are cleared once per chunk.
 constexpr int kWalkPad = 8;
+// four floats at 4-byte alignment (one 16-byte load from any slot of a plane)
+typedef float float4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+// Stores slot i of the position planes (ScreenRec::planes), if there are any.
+__device__ __forceinline__ void store_planes(float *__restrict__ planes, int stride, int i, f3 p) {
+    if (!planes) return;
+    planes[i] = p.x;
+    planes[stride + i] = p.y;
+    planes[2 * (size_t)stride + i] = p.z;
+}
 template <class Pick, class Allow, class Consume>
 __device__ __forceinline__ void walk_chunks(const GridDesc &G, const int2 *__restrict__ ranges, const int g[3],
                                             const float4 *__restrict__ xs, int jmin, Pick &&pick, Allow &&allow,
@@ -365,10 +383,23 @@ __device__ __forceinline__ void walk_chunks(const GridDesc &G, const int2 *__res
             } else {
                 if (keep == 0u && !rec.out) continue;
                 for (int t0 = 0; t0 < cnt; t0 += 8) {
-                    const float4 *__restrict__ p = xs + (base + t0);
                     float4 bb[8];
+                    if (rec.planes) {
+                        const float *__restrict__ px = rec.planes + (base + t0);
+                        float4_a4 q[6];  // x, y, z of candidates 0-3 and 4-7
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) bb[u] = p[u];
+                        for (int a = 0; a < 3; ++a) {
+                            q[2 * a] = *reinterpret_cast<const float4_a4 *>(px + (size_t)a * rec.pstride);
+                            q[2 * a + 1] = *reinterpret_cast<const float4_a4 *>(px + (size_t)a * rec.pstride + 4);
+                        }
+#pragma unroll
+                        for (int u = 0; u < 8; ++u)
+                            bb[u] = make_float4(q[u >> 2][u & 3], q[2 + (u >> 2)][u & 3], q[4 + (u >> 2)][u & 3], 0.0f);
+                    } else {
+                        const float4 *__restrict__ p = xs + (base + t0);
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) bb[u] = p[u];
+                    }
                     uint32_t m8 = 0u;  // constant shifts within the batch, one variable shift per batch
 #pragma unroll
                     for (int u = 0; u < 8; ++u) m8 |= (uint32_t)pick(base + t0 + u, bb[u]) << u;
