@@ -9,13 +9,13 @@
 
 namespace bp {
 
-// x* as three planes x[], y[], z[] (`stride` floats apart) for the lambda
-// pass's distance screen.
-__global__ __launch_bounds__(kBlock) void k_split_planes(const float4 *__restrict__ xs, float *__restrict__ planes,
-                                                         int stride, int n) {
+// The lambda pass's inputs (LambdaIn) from x* and the masses.
+__global__ __launch_bounds__(kBlock) void k_lambda_in(const float4 *__restrict__ xs, const float *__restrict__ m,
+                                                      LambdaIn L, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    store_planes(planes, stride, i, xyz(xs[i]));
+    const float4 a = xs[i];
+    store_lambda_in(L, i, xyz(a), m[i], phase_of(a));
 }
 
 // fluidLambda (fluid.cuh:78-159).  Also writes the record the position pass
@@ -26,8 +26,7 @@ template <class SP>
 __global__ __launch_bounds__(kMaxBlock) void k_lambda(const float4 *__restrict__ xs, const float *__restrict__ m,
                                                    const int2 *__restrict__ ranges, float *__restrict__ lambda,
                                                    float *__restrict__ density, float4 *__restrict__ lam_rec,
-                                                   uint32_t *__restrict__ srec, const float *__restrict__ planes,
-                                                   int pstride, int n, SP sp) {
+                                                   uint32_t *__restrict__ srec, LambdaIn L, int n, SP sp) {
     const int i = xcd_index();
     if (i >= n) return;
     sp.with_scene(i, [&](const SceneConst &S) {
@@ -51,25 +50,20 @@ __global__ __launch_bounds__(kMaxBlock) void k_lambda(const float4 *__restrict__
         // Q1: only j >= i (fluid.cuh:117).  With a screen record the walk covers
         // every j (the position pass replays it) and j < i are masked off.
         const float inv_rho0 = F.inv_rest_density;
-        struct Partner {
-            float4 b;
-            float m;
-        };
         ScreenRec rec;
         rec.out = srec;
         rec.slot = i;
         rec.n = n;
-        rec.planes = planes;  // the screen tests distance only
-        rec.pstride = pstride;
+        rec.planes = L.planes;  // the screen tests distance only
+        rec.pstride = L.pstride;
         for_each_neighbor_nb<kBodyBatch>(
             G, ranges, g, xs, srec ? 0 : i, [&](int, float4 b) { return dist2(pi, xyz(b)) <= K.h2; },
-            [&](int j) { return Partner{xs[j], m[j]}; },
-            [&](int, const Partner &P) {
-                const float4 b = P.b;
+            [&](int j) { return L.xm[j]; },  // (x*, m), m negated for a non-fluid partner
+            [&](int, const float4 &b) {
                 const float d2 = dist2(pi, xyz(b));
-                const float mj = P.m;
+                const float mj = fabsf(b.w);
                 const float ld = mj * K.poly6(d2);
-                dens += (phase_of(b) < 0) ? ld : sds * ld;
+                dens += signbit(b.w) ? sds * ld : ld;
                 if (F.lambda_mode == 1 && d2 > 0.0f) {
                     // grad C_j = -m_j gradW / rho0 (fluid.cuh:132-137)
                     const float r = fsqrt(d2);
@@ -226,7 +220,7 @@ __global__ __launch_bounds__(kBlock) void k_position_apply(const float4 *__restr
         }
     }
     xs_out[k] = f4(out, a.w);
-    store_planes(E.planes, E.pstride, k, out);
+    if (E.lin.planes) store_lambda_in(E.lin, k, out, E.m[k], phase_of(a));
 }
 
 // fluidOmega (fluid.cuh:231-279) and fluidNormal (fluid.cuh:387-434) in one
@@ -397,12 +391,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_xsph(const float4 *__restrict__ x
 // ------------------------------------------------------------ launchers
 template <class SP>
 void launch_lambda(const float4 *xs, const float *m, const int2 *ranges, float *lambda, float *density,
-                   float4 *lam_rec, uint32_t *srec, float *planes, int pstride, int planes_ready, int n, const SP &sp,
+                   float4 *lam_rec, uint32_t *srec, const LambdaIn &L, int lin_ready, int n, const SP &sp,
                    int block, hipStream_t st) {
-    if (planes && !planes_ready)
-        hipLaunchKernelGGL(k_split_planes, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, planes, pstride, n);
+    if (!lin_ready) hipLaunchKernelGGL(k_lambda_in, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, m, L, n);
     hipLaunchKernelGGL(k_lambda<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, m, ranges, lambda, density,
-                       lam_rec, srec, planes, pstride, n, sp);
+                       lam_rec, srec, L, n, sp);
 }
 template <class SP>
 void launch_position(const float4 *xs, const float4 *lam_rec, const int *build_keys, const int2 *ranges,
@@ -437,7 +430,7 @@ void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const ui
 
 #define BP_INSTANTIATE_FLUID(SP)                                                                                \
     template void launch_lambda<SP>(const float4 *, const float *, const int2 *, float *, float *, float4 *,    \
-                                    uint32_t *, float *, int, int, int, const SP &, int, hipStream_t);          \
+                                    uint32_t *, const LambdaIn &, int, int, const SP &, int, hipStream_t);      \
     template void launch_position<SP>(const float4 *, const float4 *, const int *, const int2 *,                \
                                       const uint32_t *, const uint32_t *, const uint32_t *, float4 *,           \
                                       unsigned long long *, float4 *, float4 *, float4 *, int, const SP &,      \

@@ -217,7 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const int *__restrict__ perm
                                                     int *__restrict__ o_out, int *__restrict__ sid_out,
                                                     int *__restrict__ map_to_new, int n, float dt, f3 gstep,
                                                     float mass_k, int need_w, int stab_passes, int collide,
-                                                    float *__restrict__ planes, int pstride, SP sp) {
+                                                    LambdaIn L, SP sp) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int p = perm[i];
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const int *__restrict__ perm
     }
     if (collide) xs = plane_collide_one(xs, xp, pl, S.radius, S.friction_static, S.friction_dynamic);
     xs_out[i] = f4(xs, __int_as_float(ph));
-    store_planes(planes, pstride, i, xs);
+    store_lambda_in(L, i, xs, m, ph);
     x_out[i] = f4(xp, w);
     v_out[i] = v4;
     m_out[i] = m;
@@ -305,11 +305,10 @@ template <class SP>
 void launch_reorder(const int *perm, const float4 *x_in, const float4 *v_in, const float *m_in, const int *ph_in,
                     const int *o_in, const int *sid_in, float4 *xs_out, float4 *x_out, float4 *v_out, float *m_out,
                     int *ph_out, int *o_out, int *sid_out, int *map_to_new, int n, float dt, f3 gstep, float mass_k,
-                    int need_w, int stab_passes, int collide, float *planes, int pstride, const SP &sp,
-                    hipStream_t st) {
+                    int need_w, int stab_passes, int collide, const LambdaIn &L, const SP &sp, hipStream_t st) {
     hipLaunchKernelGGL(k_reorder<SP>, dim3(grid_blocks(n)), dim3(kBlock), 0, st, perm, x_in, v_in, m_in, ph_in, o_in,
                        sid_in, xs_out, x_out, v_out, m_out, ph_out, o_out, sid_out, map_to_new, n, dt, gstep, mass_k,
-                       need_w, stab_passes, collide, planes, pstride, sp);
+                       need_w, stab_passes, collide, L, sp);
 }
 
 // The renderer boundary without a renderer (mapParticleInfos, main.cu:273-282):
@@ -336,8 +335,8 @@ void launch_map_particle_infos(const float4 *x, const int *orig, const int *grou
                                           const int *, int2 *, hipStream_t);                                    \
     template void launch_reorder<SP>(const int *, const float4 *, const float4 *, const float *, const int *,    \
                                      const int *, const int *, float4 *, float4 *, float4 *, float *, int *,     \
-                                     int *, int *, int *, int, float, f3, float, int, int, int, float *, int,    \
-                                     const SP &, hipStream_t);
+                                     int *, int *, int *, int, float, f3, float, int, int, int,                  \
+                                     const LambdaIn &, const SP &, hipStream_t);
 BP_INSTANTIATE_GRID(OneScene)
 BP_INSTANTIATE_GRID(SceneTable)
 

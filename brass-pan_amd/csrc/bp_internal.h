@@ -141,6 +141,16 @@ struct FluidParams {
 };
 
 // Epilogue fused into the last full-particle Jacobi pass of an iteration.
+// What the lambda pass reads besides x*, rewritten whenever x* changes before
+// it: x* as three planes x[], y[], z[] (`pstride` floats apart, kWalkPad slots
+// of slack) for its distance screen, and (x*, m) records for its pair bodies,
+// with the sign of m set for a non-fluid particle (phase >= 0).
+struct LambdaIn {
+    float *planes = nullptr;
+    float4 *xm = nullptr;
+    int pstride = 0;
+};
+
 struct Epilogue {
     int planes_next;      // apply the next iteration's plane collisions
     int finish;           // last iteration: updateVelocity + updatePositions
@@ -149,10 +159,10 @@ struct Epilogue {
     // with finish: v.w = vw[k] for fluid particles (the density, read by the
     // omega/normal pass; the vorticity/tension pass sets it back to 0), else 0
     const float *vw = nullptr;
-    // x* out also as position planes (ScreenRec::planes) when the next lambda
-    // pass screens this x* (nothing else moves it in between), else null
-    float *planes = nullptr;
-    int pstride = 0;
+    // x* out also as the lambda pass's inputs (LambdaIn) when the next lambda
+    // pass reads this x* (nothing else moves it in between), else null
+    LambdaIn lin;
+    const float *m = nullptr;  // masses, with lin
 };
 
 // Everything a kernel needs to know about the scene a particle belongs to.
@@ -355,12 +365,13 @@ __device__ __forceinline__ bool stamp_near(const GridDesc &G, const int g[3], co
 constexpr int kWalkPad = 8;
 // four floats at 4-byte alignment (one 16-byte load from any slot of a plane)
 typedef float float4_a4 __attribute__((ext_vector_type(4), aligned(4)));
-// Stores slot i of the position planes (ScreenRec::planes), if there are any.
-__device__ __forceinline__ void store_planes(float *__restrict__ planes, int stride, int i, f3 p) {
-    if (!planes) return;
-    planes[i] = p.x;
-    planes[stride + i] = p.y;
-    planes[2 * (size_t)stride + i] = p.z;
+// Stores slot i of the lambda pass's inputs (x* at p, mass m, phase ph), if any.
+__device__ __forceinline__ void store_lambda_in(const LambdaIn &L, int i, f3 p, float m, int ph) {
+    if (!L.planes) return;
+    L.planes[i] = p.x;
+    L.planes[L.pstride + i] = p.y;
+    L.planes[2 * (size_t)L.pstride + i] = p.z;
+    L.xm[i] = make_float4(p.x, p.y, p.z, ph < 0 ? m : -m);
 }
 template <class Pick, class Allow, class Consume>
 __device__ __forceinline__ void walk_chunks(const GridDesc &G, const int2 *__restrict__ ranges, const int g[3],

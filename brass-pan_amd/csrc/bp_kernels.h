@@ -21,8 +21,7 @@ template <class SP>
 void launch_reorder(const int *perm, const float4 *x_in, const float4 *v_in, const float *m_in, const int *ph_in,
                     const int *o_in, const int *sid_in, float4 *xs_out, float4 *x_out, float4 *v_out, float *m_out,
                     int *ph_out, int *o_out, int *sid_out, int *map_to_new, int n, float dt, f3 gstep, float mass_k,
-                    int need_w, int stab_passes, int collide, float *planes, int pstride, const SP &sp,
-                    hipStream_t st);
+                    int need_w, int stab_passes, int collide, const LambdaIn &L, const SP &sp, hipStream_t st);
 
 // bp_fluid.hip (`block`: threads per workgroup of the neighbour pass, <= kMaxBlock).
 // `srec` (kRecChunks x n candidate masks) is written by the lambda and
@@ -30,7 +29,7 @@ void launch_reorder(const int *perm, const float4 *x_in, const float4 *v_in, con
 // and by the vorticity/tension and XSPH passes.
 template <class SP>
 void launch_lambda(const float4 *xs, const float *m, const int2 *ranges, float *lambda, float *density,
-                   float4 *lam_rec, uint32_t *srec, float *planes, int pstride, int planes_ready, int n, const SP &sp,
+                   float4 *lam_rec, uint32_t *srec, const LambdaIn &L, int lin_ready, int n, const SP &sp,
                    int block, hipStream_t st);
 template <class SP>
 void launch_position(const float4 *xs, const float4 *lam_rec, const int *build_keys, const int2 *ranges,

@@ -175,16 +175,16 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
     HIP_TRY(snap(H.perm, pm, N, st));
 
     const bool fuse_first = !no_fuse && s->any_planes && !wind_on && !pick && iters > 0;
-    // The lambda pass screens x* from position planes.  The pass that writes
-    // x* right before it also writes the planes when no other pass moves x* in
-    // between (fluid-only scenes); otherwise launch_lambda splits x* first.
-    const bool lone_fluid = s->planes && !tail_groups && !s->has_solid && !pick;
+    // The lambda pass reads LambdaIn besides x*.  The pass that writes x* right
+    // before it also writes LambdaIn when no other pass moves x* in between
+    // (fluid-only scenes); otherwise launch_lambda derives it from x* first.
+    const bool lone_fluid = s->lin.planes && !tail_groups && !s->has_solid && !pick;
     const bool planes_first = lone_fluid && !wind_on && (fuse_first || !s->any_planes);
     TIMED(F_REORDER, launch_reorder(pm, s->x.cur(), s->v.cur(), s->m.cur(), s->ph.cur(), s->orig.cur(), s->sid.cur(),
                                     s->xs.cur(), s->x.alt(), s->v.alt(), s->m.alt(), s->ph.alt(), s->orig.alt(),
                                     s->sid.alt(), s->map_to_new, N, subdt, gstep, P.mass_scaling_k, need_w ? 1 : 0,
-                                    P.stabilization_passes, fuse_first ? 1 : 0, planes_first ? s->planes : nullptr,
-                                    s->pstride, sp(), st));
+                                    P.stabilization_passes, fuse_first ? 1 : 0,
+                                    planes_first ? s->lin : LambdaIn{}, sp(), st));
     s->x.flip();
     s->v.flip();
     s->m.flip();
@@ -221,8 +221,8 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
         if (H.xl) HIP_TRY(snap(H.xl[it], s->xs.cur(), N, st));
         if (s->has_fluid) {
             TIMED(F_LAMBDA, launch_lambda(s->xs.cur(), s->m.cur(), s->ranges, s->lambda, s->density, s->lam_rec,
-                                          H.rec ? H.rec[it] : s->srec, s->planes, s->pstride, planes_ready ? 1 : 0,
-                                          N, sp(), s->blk_lambda, st));
+                                          H.rec ? H.rec[it] : s->srec, s->lin, planes_ready ? 1 : 0, N, sp(),
+                                          s->blk_lambda, st));
             if (H.lam) HIP_TRY(snap(H.lam[it], s->lambda, N, st));
             Epilogue E{};
             E.planes_next = (!no_fuse && s->any_planes && it + 1 < iters && !tail_groups && !pick) ? 1 : 0;
@@ -232,8 +232,8 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
             E.vw = vw;
             planes_ready = lone_fluid && it + 1 < iters && (E.planes_next || !s->any_planes);
             if (planes_ready) {
-                E.planes = s->planes;
-                E.pstride = s->pstride;
+                E.lin = s->lin;
+                E.m = s->m.cur();
             }
             TIMED(F_POSITION, launch_position(s->xs.cur(), s->lam_rec, sk, s->ranges, H.rec ? H.rec[it] : s->srec,
                                               s->fluid_stamp, s->stamp, s->sum_tmp, s->corr, s->xs.alt(),
@@ -540,8 +540,11 @@ bp_status create_impl(const bp_scene_desc *scs, const bp_params *params, int ns,
     TRY(s->alloc(&s->group, N));
     if (s->has_fluid) TRY(s->alloc(&s->srec, (size_t)kRecChunks * N));
     if (s->has_fluid) TRY(s->alloc(&s->lam_rec, N));
-    s->pstride = (N + kWalkPad + 63) / 64 * 64;  // planes hold kWalkPad slots of slack, 256-byte aligned
-    if (s->has_fluid) TRY(s->alloc(&s->planes, 3 * (size_t)s->pstride));
+    if (s->has_fluid) {
+        s->lin.pstride = (N + kWalkPad + 63) / 64 * 64;  // kWalkPad slots of slack, 256-byte aligned planes
+        TRY(s->alloc(&s->lin.planes, 3 * (size_t)s->lin.pstride));
+        TRY(s->alloc(&s->lin.xm, N));
+    }
     TRY(s->alloc(&s->sum_tmp, N));
     TRY(s->alloc(&s->corr, 3 * (size_t)N));
     TRY(s->alloc(&s->omega, N));

@@ -79,8 +79,7 @@ struct bp_solver {
 
     float *lambda = nullptr, *density = nullptr;
     float4 *lam_rec = nullptr;  // lambda pass -> position pass: (lambda, m, current cell, -) per slot
-    float *planes = nullptr;    // x* as planes x[], y[], z[] for the lambda screen (pstride floats apart)
-    int pstride = 0;
+    bp::LambdaIn lin;           // the lambda pass's inputs besides x* (fluid scenes)
     uint32_t *ccell = nullptr;
     uint32_t *srec = nullptr;  // screen record of the lambda and omega/normal passes (kRecChunks x N masks)
     int *group = nullptr;      // render group id by original id (bp_scene_desc::group_ids, else 0)
