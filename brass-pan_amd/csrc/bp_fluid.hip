@@ -23,7 +23,7 @@ __global__ __launch_bounds__(kBlock) void k_lambda_in(const float4 *__restrict__
 // packed, 0), so that pass fetches two words-of-four per partner (x* and this)
 // instead of four separate arrays; it searches from the same x* and cell.
 template <class SP>
-__global__ __launch_bounds__(kMaxBlock) void k_lambda(const float4 *__restrict__ xs, const float *__restrict__ m,
+__global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_lambda(const float4 *__restrict__ xs, const float *__restrict__ m,
                                                    const int2 *__restrict__ ranges, float *__restrict__ lambda,
                                                    float *__restrict__ density, float4 *__restrict__ lam_rec,
                                                    uint32_t *__restrict__ srec, LambdaIn L, int n, SP sp) {
@@ -114,7 +114,7 @@ __device__ __forceinline__ unsigned long long to_fixed(float v) {
 // A pair k owns whose partner j's search does not reach k's cell is delivered
 // to j through corr[j] (fixed point).  k_position_apply adds both parts.
 template <class SP>
-__global__ __launch_bounds__(kMaxBlock) void k_position(const float4 *__restrict__ xs,
+__global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_position(const float4 *__restrict__ xs,
                                                      const float4 *__restrict__ lam_rec,
                                                      const int *__restrict__ build_keys,
                                                      const int2 *__restrict__ ranges,
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(kBlock) void k_position_apply(const float4 *__restr
 // fluidOmega (fluid.cuh:231-279) and fluidNormal (fluid.cuh:387-434) in one
 // sweep (independent sums over the same neighbours).  omega.w = |omega|.
 template <class SP>
-__global__ __launch_bounds__(kMaxBlock) void k_post_a(const float4 *__restrict__ xs, const float4 *__restrict__ v,
+__global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_post_a(const float4 *__restrict__ xs, const float4 *__restrict__ v,
                                                    const float *__restrict__ density,
                                                    const int2 *__restrict__ ranges, float4 *__restrict__ omega,
                                                    float4 *__restrict__ xom,
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_post_a(const float4 *__restrict__
 // (fluid.cuh:436-489) for the same particle: both only change the particle's
 // own velocity, so the fused pass writes v in place.
 template <class SP>
-__global__ __launch_bounds__(kMaxBlock) void k_post_b(const float4 *__restrict__ xs, float4 *__restrict__ v,
+__global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 1) void k_post_b(const float4 *__restrict__ xs, float4 *__restrict__ v,
                                                    const float4 *__restrict__ omega,
                                                    const float4 *__restrict__ xom,
                                                    const float4 *__restrict__ normal,
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_post_b(const float4 *__restrict__
 
 // fluidXSph (fluid.cuh:338-385).  Non-fluid particles keep their velocity.
 template <class SP>
-__global__ __launch_bounds__(kMaxBlock) void k_xsph(const float4 *__restrict__ xs, const float4 *__restrict__ v,
+__global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_xsph(const float4 *__restrict__ xs, const float4 *__restrict__ v,
                                                  const int2 *__restrict__ ranges,
                                                  const uint32_t *__restrict__ ccell,
                                                  const uint32_t *__restrict__ srec, float4 *__restrict__ v_out,

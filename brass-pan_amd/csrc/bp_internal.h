@@ -11,6 +11,11 @@ constexpr int kMaxPlanes = 16;
 constexpr int kMaxRigidParticles = 64;  // NUM_MAX_PARTICLE_PER_RIGID_BODY (cubrasstower.cuh:22)
 constexpr int kBlock = 256;      // streaming kernels
 constexpr int kMaxBlock = 512;   // neighbour passes: block size chosen at run time (autotuner)
+// Minimum waves per SIMD a neighbour pass asks the compiler for (SP::kWaves;
+// 6 = at most 80 VGPRs).  The passes wait on memory, so occupancy pays where
+// the walk fits the budget without spilling: single-scene passes other than
+// vorticity/tension.  1 leaves the register budget open.
+#define kPassWaves(SP, W) __attribute__((amdgpu_waves_per_eu(SP::kWaves < W ? SP::kWaves : W)))
 
 // ------------------------------------------------------------ small vectors
 // Own 3-vector type: component-wise operators with the left-to-right order of
@@ -186,11 +191,13 @@ struct SceneConst {
 // that indexes the plane arrays with a loop variable keeps at(slot), since
 // the copy would then live in scratch memory.
 struct OneScene {
+    static constexpr int kWaves = 6;
     SceneConst c;
     __device__ __forceinline__ const SceneConst &at(int) const { return c; }
     template <class F> __device__ __forceinline__ void with_scene(int, F &&f) const { f(c); }
 };
 struct SceneTable {
+    static constexpr int kWaves = 1;
     const SceneConst *__restrict__ table;
     const int *__restrict__ sid;  // slot -> scene
     __device__ __forceinline__ const SceneConst &at(int slot) const { return table[sid[slot]]; }
@@ -254,6 +261,9 @@ __device__ __forceinline__ f3 plane_collide_one(f3 xs, f3 pos, const Planes &pl,
     return xs;
 }
 
+// four ints at 8-byte alignment (two adjacent cell ranges in one load)
+typedef int int4_a8 __attribute__((ext_vector_type(4), aligned(8)));
+
 // Iterates the reference's 27-cell candidate set (fluid.cuh:104-115) around
 // grid position g: the 9 (x,y) columns in order, each visiting the cells
 // z-1, z, z+1 (wrapped) in that order.  Their particles are consecutive slots
@@ -271,19 +281,24 @@ __device__ __forceinline__ void for_each_run(const GridDesc &G, const int2 *__re
         return G.base + ((uint32_t)(g[0] + dx) & G.mask[0]) * G.stride_x +
                ((uint32_t)(g[1] + dy) & G.mask[1]) * (uint32_t)G.dims[2];
     };
-    int2 nr[3];
-    {
-        const uint32_t col = column(0);
+    // a column's three ranges: two loads where its cells are adjacent
+    const bool adjacent = !brk01 && !brk12;
+    auto load_column = [&](uint32_t col, int2 r[3]) {
+        if (adjacent) {
+            const int4_a8 a = *reinterpret_cast<const int4_a8 *>(ranges + col + zc[0]);
+            r[0] = make_int2(a.x, a.y);
+            r[1] = make_int2(a.z, a.w);
+            r[2] = ranges[col + zc[2]];
+        } else {
 #pragma unroll
-        for (int q = 0; q < 3; ++q) nr[q] = ranges[col + zc[q]];
-    }
+            for (int q = 0; q < 3; ++q) r[q] = ranges[col + zc[q]];
+        }
+    };
+    int2 nr[3];
+    load_column(column(0), nr);
     for (int ci = 0; ci < 9; ++ci) {
         int2 r[3] = {nr[0], nr[1], nr[2]};
-        if (ci + 1 < 9) {
-            const uint32_t col = column(ci + 1);
-#pragma unroll
-            for (int q = 0; q < 3; ++q) nr[q] = ranges[col + zc[q]];
-        }
+        if (ci + 1 < 9) load_column(column(ci + 1), nr);
         int s = -1, e = -1;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
