@@ -9,13 +9,15 @@
 
 namespace bp {
 
-// The lambda pass's inputs (LambdaIn) from x* and the masses.
+// The lambda pass's inputs (LambdaIn) from x* and the masses; with m null,
+// only the position planes.
 __global__ __launch_bounds__(kBlock) void k_lambda_in(const float4 *__restrict__ xs, const float *__restrict__ m,
                                                       LambdaIn L, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 a = xs[i];
-    store_lambda_in(L, i, xyz(a), m[i], phase_of(a));
+    if (!m) L.xm = nullptr;
+    store_lambda_in(L, i, xyz(a), m ? m[i] : 0.0f, phase_of(a));
 }
 
 // fluidLambda (fluid.cuh:78-159).  Also writes the record the position pass
@@ -231,8 +233,8 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_post_a(const fl
                                                    const int2 *__restrict__ ranges, float4 *__restrict__ omega,
                                                    float4 *__restrict__ xom,
                                                    float4 *__restrict__ normal, uint32_t *__restrict__ ccell,
-                                                   uint32_t *__restrict__ srec, int n, SP sp, int do_omega,
-                                                   int do_normal) {
+                                                   uint32_t *__restrict__ srec, const float *__restrict__ planes,
+                                                   int pstride, int n, SP sp, int do_omega, int do_normal) {
     const int i = xcd_index();
     if (i >= n) return;
     sp.with_scene(i, [&](const SceneConst &S) {
@@ -253,9 +255,13 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_post_a(const fl
         rec.out = srec;
         rec.slot = i;
         rec.n = n;
+        // with position planes every particle has phase ph: the screen tests distance only
+        rec.planes = planes;
+        rec.pstride = pstride;
+        const bool any_phase = planes != nullptr;
         for_each_neighbor(
             G, ranges, g, xs, 0,
-            [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
+            [&](int, float4 b) { return (any_phase | (phase_of(b) == ph)) & (dist2(pi, xyz(b)) <= K.h2); },
             [&](int j, float4 b) {
                 if (!(dist2(pi, xyz(b)) > 0.0f)) return;  // coincident partners are skipped
                 const f3 diff = pi - xyz(b);
@@ -409,10 +415,13 @@ void launch_position(const float4 *xs, const float4 *lam_rec, const int *build_k
 }
 template <class SP>
 void launch_post_a(const float4 *xs, const float4 *v, const float *density, const int2 *ranges, float4 *omega,
-                   float4 *xom, float4 *normal, uint32_t *ccell, uint32_t *srec, int n, const SP &sp, int do_omega,
-                   int do_normal, int block, hipStream_t st) {
+                   float4 *xom, float4 *normal, uint32_t *ccell, uint32_t *srec, const LambdaIn *one_phase,
+                   int planes_ready, int n, const SP &sp, int do_omega, int do_normal, int block, hipStream_t st) {
+    if (one_phase && !planes_ready)
+        hipLaunchKernelGGL(k_lambda_in, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, nullptr, *one_phase, n);
     hipLaunchKernelGGL(k_post_a<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, v, density, ranges, omega,
-                       xom, normal, ccell, srec, n, sp, do_omega, do_normal);
+                       xom, normal, ccell, srec, one_phase ? one_phase->planes : nullptr,
+                       one_phase ? one_phase->pstride : 0, n, sp, do_omega, do_normal);
 }
 template <class SP>
 void launch_post_b(const float4 *xs, float4 *v, const float4 *omega, const float4 *xom, const float4 *normal,
@@ -436,8 +445,8 @@ void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const ui
                                       unsigned long long *, float4 *, float4 *, float4 *, int, const SP &,      \
                                       const Epilogue &, int, hipStream_t);                                      \
     template void launch_post_a<SP>(const float4 *, const float4 *, const float *, const int2 *, float4 *,      \
-                                    float4 *, float4 *, uint32_t *, uint32_t *, int, const SP &, int, int, int, \
-                                    hipStream_t);                                                               \
+                                    float4 *, float4 *, uint32_t *, uint32_t *, const LambdaIn *, int, int,     \
+                                    const SP &, int, int, int, hipStream_t);                                    \
     template void launch_post_b<SP>(const float4 *, float4 *, const float4 *, const float4 *, const float4 *,   \
                                     const float *, const int2 *, const uint32_t *, const uint32_t *, int,       \
                                     const SP &, int, int, float, int, hipStream_t);                             \

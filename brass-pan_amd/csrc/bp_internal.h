@@ -380,13 +380,15 @@ __device__ __forceinline__ bool stamp_near(const GridDesc &G, const int g[3], co
 constexpr int kWalkPad = 8;
 // four floats at 4-byte alignment (one 16-byte load from any slot of a plane)
 typedef float float4_a4 __attribute__((ext_vector_type(4), aligned(4)));
-// Stores slot i of the lambda pass's inputs (x* at p, mass m, phase ph), if any.
+// Stores slot i of the lambda pass's inputs (x* at p, mass m, phase ph): the
+// planes if L has them, the (x*, m) record if L has those.
 __device__ __forceinline__ void store_lambda_in(const LambdaIn &L, int i, f3 p, float m, int ph) {
-    if (!L.planes) return;
-    L.planes[i] = p.x;
-    L.planes[L.pstride + i] = p.y;
-    L.planes[2 * (size_t)L.pstride + i] = p.z;
-    L.xm[i] = make_float4(p.x, p.y, p.z, ph < 0 ? m : -m);
+    if (L.planes) {
+        L.planes[i] = p.x;
+        L.planes[L.pstride + i] = p.y;
+        L.planes[2 * (size_t)L.pstride + i] = p.z;
+    }
+    if (L.xm) L.xm[i] = make_float4(p.x, p.y, p.z, ph < 0 ? m : -m);
 }
 template <class Pick, class Allow, class Consume>
 __device__ __forceinline__ void walk_chunks(const GridDesc &G, const int2 *__restrict__ ranges, const int g[3],

@@ -230,7 +230,8 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
             E.inv_dt = inv_dt;
             E.sleep_eps2 = P.sleep_epsilon * P.sleep_epsilon;
             E.vw = vw;
-            planes_ready = lone_fluid && it + 1 < iters && (E.planes_next || !s->any_planes);
+            // (after the last iteration: the final x*, which the omega/normal pass screens)
+            planes_ready = lone_fluid && (it + 1 < iters ? (E.planes_next || !s->any_planes) : E.finish != 0);
             if (planes_ready) {
                 E.lin = s->lin;
                 E.m = s->m.cur();
@@ -265,7 +266,9 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
         const int do_ak = P.akinci ? 1 : 0;
         if (do_vort || do_ak) {
             TIMED(F_POST_A, launch_post_a(s->xs.cur(), s->v.cur(), s->density, s->ranges, s->omega, s->xom, s->normal,
-                                          s->ccell, s->srec, N, sp(), do_vort, do_ak, s->blk_post_a, st));
+                                          s->ccell, s->srec, s->one_phase ? &s->lin : nullptr,
+                                          finished && planes_ready ? 1 : 0, N, sp(), do_vort, do_ak, s->blk_post_a,
+                                          st));
             TIMED(F_POST_B, launch_post_b(s->xs.cur(), s->v.cur(), s->omega, s->xom, s->normal, s->density, s->ranges,
                                           s->ccell, s->srec, N, sp(), do_vort, do_ak, subdt, s->blk_post_b, st));
         }
@@ -478,6 +481,8 @@ bp_status create_impl(const bp_scene_desc *scs, const bp_params *params, int ns,
             hsid[g] = k;
             if (sc.phases[i] > 0) s->has_solid = true; else s->has_fluid = true;
         }
+        if (k == 0) s->one_phase = sc.num_particles > 0;
+        for (int i = 0; i < sc.num_particles; ++i) s->one_phase = s->one_phase && sc.phases[i] == hph[0] && hph[0] < 0;
         for (int i = 0; i < 2 * sc.num_distance; ++i) dp.push_back(sc.distance_pairs[i] + off);
         for (int i = 0; i < 2 * sc.num_distance; ++i) dpar.push_back(sc.distance_params[i]);
         for (int i = 0; i < 4 * sc.num_bendings; ++i) bd.push_back(sc.bendings[i] + off);

@@ -64,6 +64,7 @@ struct bp_solver {
     std::vector<int> poff;             // scene s owns original ids [poff[s], poff[s+1])
     std::vector<int> scene_of_orig;
     bool has_solid = false, has_fluid = false, any_planes = false;
+    bool one_phase = false;  // every particle has the same fluid phase (the same-phase tests always pass)
 
     bp_host::DBuf<float4> xs, x, v;
     bp_host::DBuf<float> m;

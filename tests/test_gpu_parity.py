@@ -215,6 +215,35 @@ def test_picking_and_queries():
     gpu.close()
 
 
+def test_picking_one_phase_post_passes():
+    """One fluid phase with vorticity and Akinci tension, stepped with a picked
+    particle.  Picking moves x* between passes, so the lambda pass's inputs and
+    the omega/normal pass's position planes come from their own split kernels
+    rather than from the pass before (the path a plain fluid block skips)."""
+    sc = scenes.dam_break(dims=(8, 8, 8), grid=(64, 64, 64))
+    prm = scenes.dam_break_params(vorticity_eps=1e-3, akinci=1)
+    arr = bp.SceneArrays(sc)
+    og = orc.Oracle(arr, prm, fp64=False, mode=orc.GATHER)
+    case("picking, one fluid phase, vorticity + Akinci")
+    gpu = bp.Solver(sc, prm)
+    for _ in range(2):
+        gpu.step(1, 1 / 60, picked=17, pos=(1.0, 1.0, 1.0), vel=(0.5, 0.0, 0.0))
+        og.step(1, 1 / 60, picked=17, pos=(1.0, 1.0, 1.0), vel=(0.5, 0.0, 0.0))
+    gp, gv = gpu.state_original_order()
+    rp, rv = og.state_original_order()
+    assert np.allclose(gp[17], (1.0, 1.0, 1.0)) and np.allclose(gv[17], (0.5, 0.0, 0.0))
+    assert np.abs(gp - rp).max() <= 4 * POS_TOL_GATHER
+    assert np.abs(gv - rv).max() <= 4 * POS_TOL_GATHER * 60
+    # without picking the same scene takes the fused path; both match the oracle
+    gpu2 = bp.Solver(sc, prm)
+    og2 = orc.Oracle(arr, prm, fp64=False, mode=orc.GATHER)
+    gpu2.step(2, 1 / 60)
+    og2.step(2, 1 / 60)
+    assert np.abs(gpu2.state_original_order()[0] - og2.state_original_order()[0]).max() <= 4 * POS_TOL_GATHER
+    gpu.close()
+    gpu2.close()
+
+
 def test_full_size_invariants_large_block():
     sc = scenes.large_block()
     prm = scenes.large_block_params()
