@@ -193,6 +193,22 @@ __global__ __launch_bounds__(kBlock) void ka_cells(const float4 *__restrict__ xs
     ccell[i] = G.pack(g);
 }
 
+// The projection adjoints' per-particle record, one fetch per partner: (lambda,
+// m, current cell packed, 0) for the position adjoint; the position adjoint's
+// apply step then replaces lambda by the density adjoint for the density
+// adjoint.
+template <class SP>
+__global__ __launch_bounds__(kBlock) void ka_proj_rec(const float4 *__restrict__ xs, const float *__restrict__ m,
+                                                      const float *__restrict__ lambda, float4 *__restrict__ rec,
+                                                      int n, SP sp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const GridDesc &G = sp.at(i).G;
+    int g[3];
+    G.pos(xyz(xs[i]), g);
+    rec[i] = make_float4(lambda[i], m[i], __uint_as_float(G.pack(g)), 0.0f);
+}
+
 // ------------------------------------------------------------ XSPH + update
 // Forward (fluid i): v3_i = v2_i + c sum_j (v2_j - v2_i) W(|x_i - x_j|^2),
 // v2 = (x* - x) / dt.  Own terms of slot i: its equation's derivative with
@@ -289,9 +305,8 @@ __global__ __launch_bounds__(kBlock) void ka_xsph_apply(const float4 *__restrict
 // u = m_j y_k - m_k y_j, each pair adds (u . diff) dc to the adjoints of
 // diff and lambda_k + lambda_j.
 template <class SP>
-__global__ __launch_bounds__(kMaxBlock) void ka_position(const float4 *__restrict__ xs, const float *__restrict__ m,
-                                                         const float *__restrict__ lambda,
-                                                         const uint32_t *__restrict__ ccell,
+__global__ __launch_bounds__(kMaxBlock) void ka_position(const float4 *__restrict__ xs,
+                                                         const float4 *__restrict__ prec,
                                                          const int *__restrict__ keys,
                                                          const int2 *__restrict__ ranges,
                                                          const float4 *__restrict__ gpost, float4 *__restrict__ gx_own,
@@ -308,19 +323,19 @@ __global__ __launch_bounds__(kMaxBlock) void ka_position(const float4 *__restric
         const float4 a = xs[k];
         const f3 pk = xyz(a);
         const bool kf = phase_of(a) <= 0;
-        const float lk = lambda[k];
-        const float mk = m[k];
+        const float4 rk = prec[k];  // (lambda, m, cell, -)
+        const float lk = rk.x;
+        const float mk = rk.y;
         const uint32_t bk = G.pack_addr(keys[k]);
         const float ir0 = F.inv_rest_density;
         const f3 yk = xyz(gpost[k]) * ir0;
         int g[3];
-        unpack_cell(ccell[k], g);
+        unpack_cell(__float_as_uint(rk.z), g);
         f3 gx = mk3(0.f, 0.f, 0.f);
         float gl = 0.0f;
         struct Partner {
             float4 b, gp;
-            uint32_t cell;
-            float lam, m;
+            float4 r;  // (lambda, m, cell, -)
         };
         // A fluid particle replays the forward lambda pass's screen of this x*
         // (taped); a solid screens and records for the density adjoint.
@@ -333,17 +348,17 @@ __global__ __launch_bounds__(kMaxBlock) void ka_position(const float4 *__restric
             G, ranges, g, xs, 0,
             // a solid screens for fluid partners only
             [&](int, float4 b) { return (dist2(pk, xyz(b)) <= K.h2) & (kf | (phase_of(b) <= 0)); },
-            [&](int j) { return Partner{xs[j], gpost[j], ccell[j], lambda[j], m[j]}; },
+            [&](int j) { return Partner{xs[j], gpost[j], prec[j]}; },
             [&](int j, const Partner &P) {
                 const float4 b = P.b;
                 // coincident partners are skipped; a solid partner's pair is k's only if j > k
                 if (!(dist2(pk, xyz(b)) > 0.0f) | ((phase_of(b) > 0) & !(kf & (j > k)))) return;
-                const bool partner_sees_k = G.in_block(P.cell, bk);
+                const bool partner_sees_k = G.in_block(__float_as_uint(P.r.z), bk);
                 if (j < k && !partner_sees_k) return;
                 const f3 diff = pk - xyz(b);
                 float c, dc_dd2, dc_dls;
-                pair_coef_grad(dist2(pk, xyz(b)), lk + P.lam, K, F, c, dc_dd2, dc_dls);
-                const f3 u = P.m * yk - mk * (xyz(P.gp) * ir0);
+                pair_coef_grad(dist2(pk, xyz(b)), lk + P.r.x, K, F, c, dc_dd2, dc_dls);
+                const f3 u = P.r.y * yk - mk * (xyz(P.gp) * ir0);
                 const float cbar = dot(u, diff);
                 const f3 gdiff = c * u + (2.0f * cbar * dc_dd2) * diff;
                 const float gls = cbar * dc_dls;
@@ -361,13 +376,14 @@ __global__ __launch_bounds__(kMaxBlock) void ka_position(const float4 *__restric
 }
 
 // x* adjoint before the correction = after + own + delivered terms; lambda
-// adjoint -> density adjoint (lambda = -(rho/rho0 - 1)/D for fluids).
+// adjoint -> density adjoint (lambda = -(rho/rho0 - 1)/D for fluids), which
+// replaces lambda in the projection record (prec.x).
 template <class SP>
 __global__ __launch_bounds__(kBlock) void ka_position_apply(const float4 *__restrict__ xs,
                                                             const float4 *__restrict__ gx_own,
                                                             const float *__restrict__ glam_own,
                                                             unsigned long long *__restrict__ corr,
-                                                            float4 *__restrict__ gxs, float *__restrict__ grho, int n,
+                                                            float4 *__restrict__ gxs, float4 *__restrict__ prec, int n,
                                                             SP sp) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -376,16 +392,15 @@ __global__ __launch_bounds__(kBlock) void ka_position_apply(const float4 *__rest
     const float gl = glam_own[k] + take1(corr, k, 3);
     gxs[k] = f4(g, 0.f);
     const bool fluid = phase_of(xs[k]) <= 0;
-    grho[k] = fluid ? -gl / (F.rest_density * F.lambda_denominator) : 0.0f;
+    reinterpret_cast<float *>(prec + k)[0] = fluid ? -gl / (F.rest_density * F.lambda_denominator) : 0.0f;
 }
 
 // ------------------------------------------------------------ density (λ)
 // Forward: rho_i = sum_{j >= i found by i's walk} s_j m_j W(|x_i - x_j|^2)
 // (s_j = 1 for fluids, the solid density scale for solids).
 template <class SP>
-__global__ __launch_bounds__(kMaxBlock) void ka_lambda(const float4 *__restrict__ xs, const float *__restrict__ m,
-                                                       const float *__restrict__ grho,
-                                                       const uint32_t *__restrict__ ccell,
+__global__ __launch_bounds__(kMaxBlock) void ka_lambda(const float4 *__restrict__ xs,
+                                                       const float4 *__restrict__ prec,
                                                        const int *__restrict__ keys, const int2 *__restrict__ ranges,
                                                        float4 *__restrict__ gx_own,
                                                        unsigned long long *__restrict__ corr,
@@ -400,17 +415,17 @@ __global__ __launch_bounds__(kMaxBlock) void ka_lambda(const float4 *__restrict_
         const float4 a = xs[k];
         const f3 pk = xyz(a);
         const bool kf = phase_of(a) <= 0;
-        const float rk = grho[k];
-        const float mk = m[k];
+        const float4 pr = prec[k];  // (density adjoint, m, cell, -)
+        const float rk = pr.x;
+        const float mk = pr.y;
         const float sk = (phase_of(a) < 0) ? 1.0f : sds;
         const uint32_t bk = G.pack_addr(keys[k]);
         int g[3];
-        unpack_cell(ccell[k], g);
+        unpack_cell(__float_as_uint(pr.z), g);
         f3 gx = mk3(0.f, 0.f, 0.f);
         struct Partner {
             float4 b;
-            uint32_t cell;
-            float m, grho;
+            float4 r;  // (density adjoint, m, cell, -)
         };
         ScreenRec rec;  // the records the position adjoint replayed or wrote for this x*
         rec.in = kf ? trec : srec;
@@ -419,21 +434,21 @@ __global__ __launch_bounds__(kMaxBlock) void ka_lambda(const float4 *__restrict_
         for_each_neighbor_nb<kBodyBatch>(
             G, ranges, g, xs, 0,
             [&](int, float4 b) { return (dist2(pk, xyz(b)) <= K.h2) & (kf | (phase_of(b) <= 0)); },
-            [&](int j) { return Partner{xs[j], ccell[j], m[j], grho[j]}; },
+            [&](int j) { return Partner{xs[j], prec[j]}; },
             [&](int j, const Partner &P) {
                 const float4 b = P.b;
                 // coincident partners are skipped; a solid partner's pair is k's only if j > k
                 if (!(dist2(pk, xyz(b)) > 0.0f) | ((phase_of(b) > 0) & !(kf & (j > k)))) return;
-                const bool partner_sees_k = G.in_block(P.cell, bk);
+                const bool partner_sees_k = G.in_block(__float_as_uint(P.r.z), bk);
                 const f3 diff = pk - xyz(b);
                 const float Wp2 = 2.0f * poly6_d(K, dist2(pk, xyz(b)));
                 if (j > k) {  // pair of rho_k
                     const float sj = (phase_of(b) < 0) ? 1.0f : sds;
-                    const f3 t = (rk * sj * P.m * Wp2) * diff;
+                    const f3 t = (rk * sj * P.r.y * Wp2) * diff;
                     gx += t;
                     if (!partner_sees_k) deliver3(corr, j, 0, -t);
                 } else if (partner_sees_k) {  // pair of rho_j
-                    gx += (P.grho * sk * mk * Wp2) * diff;
+                    gx += (P.r.x * sk * mk * Wp2) * diff;
                 }
             },
             [&](int base) { return kf ? ~chunk_bit(k, base) : chunk_below(k, base); }, rec);
@@ -742,16 +757,17 @@ void launch_adj_xsph(const float4 *xf, const float4 *x1, const float4 *gx2, cons
                        corr, gxs, gx1, gw, inv_dt, eps2, n, sp);
 }
 template <class SP>
-void launch_adj_projection(const float4 *xs, const float *m, const float *lambda, const uint32_t *ccell,
-                           const int *keys, const int2 *ranges, float4 *gxs, float4 *gx_own, float *glam_own,
-                           float *grho, unsigned long long *corr, const uint32_t *trec, uint32_t *srec, int n,
-                           const SP &sp, int block, hipStream_t st) {
-    hipLaunchKernelGGL(ka_position<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, m, lambda, ccell, keys,
-                       ranges, gxs, gx_own, glam_own, corr, trec, srec, n, sp);
+void launch_adj_projection(const float4 *xs, const float *m, const float *lambda, float4 *prec, const int *keys,
+                           const int2 *ranges, float4 *gxs, float4 *gx_own, float *glam_own,
+                           unsigned long long *corr, const uint32_t *trec, uint32_t *srec, int n, const SP &sp,
+                           int block, hipStream_t st) {
+    hipLaunchKernelGGL(ka_proj_rec<SP>, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, m, lambda, prec, n, sp);
+    hipLaunchKernelGGL(ka_position<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, prec, keys, ranges, gxs,
+                       gx_own, glam_own, corr, trec, srec, n, sp);
     hipLaunchKernelGGL(ka_position_apply<SP>, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, gx_own, glam_own, corr,
-                       gxs, grho, n, sp);
-    hipLaunchKernelGGL(ka_lambda<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, m, grho, ccell, keys,
-                       ranges, gx_own, corr, trec, srec, n, sp);
+                       gxs, prec, n, sp);
+    hipLaunchKernelGGL(ka_lambda<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, prec, keys, ranges,
+                       gx_own, corr, trec, srec, n, sp);
     hipLaunchKernelGGL(ka_add_own, dim3(grid_blocks(n)), dim3(kBlock), 0, st, gxs, gx_own, corr, n);
 }
 template <class SP>
@@ -797,10 +813,9 @@ void launch_adj_scene_reduce(const float4 *facc, const float *cacc, const int *p
                                       const uint32_t *, const int *, const int2 *, float4 *, float4 *, float *,  \
                                       unsigned long long *, float4 *, float4 *, float *, float, float, int,      \
                                       const SP &, int, hipStream_t);                                             \
-    template void launch_adj_projection<SP>(const float4 *, const float *, const float *, const uint32_t *,     \
-                                            const int *, const int2 *, float4 *, float4 *, float *, float *,     \
-                                            unsigned long long *, const uint32_t *, uint32_t *, int, const SP &, \
-                                            int, hipStream_t);                                                   \
+    template void launch_adj_projection<SP>(const float4 *, const float *, const float *, float4 *, const int *, \
+                                            const int2 *, float4 *, float4 *, float *, unsigned long long *,     \
+                                            const uint32_t *, uint32_t *, int, const SP &, int, hipStream_t);    \
     template void launch_adj_planes<SP>(const float4 *, const float4 *, float4 *, float4 *, int, const SP &,    \
                                         hipStream_t);                                                           \
     template void launch_adj_wind<SP>(const float4 *, const float4 *, const int *, const float *, const int4 *, \

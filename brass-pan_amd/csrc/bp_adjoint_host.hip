@@ -50,7 +50,8 @@ struct Adjoint {
     // adjoint buffers (slot order of the substep being reversed)
     float4 *gX[2] = {nullptr, nullptr}, *gV[2] = {nullptr, nullptr};
     float4 *gxs = nullptr, *gx1 = nullptr, *gv1 = nullptr, *gown = nullptr;
-    float *gw = nullptr, *glam = nullptr, *grho = nullptr, *cacc = nullptr;
+    float *gw = nullptr, *glam = nullptr, *cacc = nullptr;
+    float4 *prec = nullptr;  // projection adjoints' record: (lambda | density adjoint, m, cell, -)
     uint32_t *ccell = nullptr;
     int *mtn = nullptr, *contacts = nullptr;
     unsigned long long *corr = nullptr;
@@ -155,7 +156,7 @@ bp_status ensure(bp_solver *s, int total, int every) {
         if ((r = A.alloc(&A.gxs, N)) != BP_OK || (r = A.alloc(&A.gx1, N)) != BP_OK ||
             (r = A.alloc(&A.gv1, N)) != BP_OK || (r = A.alloc(&A.gown, N)) != BP_OK ||
             (r = A.alloc(&A.gw, N)) != BP_OK || (r = A.alloc(&A.glam, N)) != BP_OK ||
-            (r = A.alloc(&A.grho, N)) != BP_OK || (r = A.alloc(&A.cacc, N)) != BP_OK ||
+            (r = A.alloc(&A.prec, N)) != BP_OK || (r = A.alloc(&A.cacc, N)) != BP_OK ||
             (r = A.alloc(&A.ccell, N)) != BP_OK || (r = A.alloc(&A.mtn, N)) != BP_OK ||
             (r = A.alloc(&A.contacts, 1)) != BP_OK || (r = A.alloc(&A.corr, 6 * (size_t)N)) != BP_OK ||
             (r = A.alloc(&A.cb, s->nd)) != BP_OK || (r = A.alloc(&A.wb, s->nd)) != BP_OK ||
@@ -286,9 +287,8 @@ void reverse_substep(bp_solver *s, Adjoint &A, const Adjoint::Slot &T, int &cur,
             launch_adj_distance(T.xd[it], T.x1, s->dpairs, s->dparams, A.mtn, A.gxs, A.gw, A.cb, A.wb, s->nd,
                                 s->dcsr.verts, s->dcsr.offs, s->dcsr.ents, s->dcsr.nv, st);
         if (s->has_fluid) {
-            launch_adj_cells(T.xl[it], A.ccell, N, sp, st);
-            launch_adj_projection(T.xl[it], T.m, T.lam[it], A.ccell, T.keys, s->ranges, A.gxs, A.gown, A.glam,
-                                  A.grho, A.corr, T.rec[it], s->srec, N, sp, s->blk_position, st);
+            launch_adj_projection(T.xl[it], T.m, T.lam[it], A.prec, T.keys, s->ranges, A.gxs, A.gown, A.glam,
+                                  A.corr, T.rec[it], s->srec, N, sp, s->blk_position, st);
         }
         if (s->any_planes) launch_adj_planes(T.xp[it], T.x1, A.gxs, A.gx1, N, sp, st);
     }

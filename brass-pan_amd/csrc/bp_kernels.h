@@ -88,10 +88,10 @@ void launch_adj_xsph(const float4 *xf, const float4 *x1, const float4 *gx2, cons
                      unsigned long long *corr, float4 *gxs, float4 *gx1, float *gw, float inv_dt, float eps2, int n,
                      const SP &sp, int block, hipStream_t st);
 template <class SP>
-void launch_adj_projection(const float4 *xs, const float *m, const float *lambda, const uint32_t *ccell,
-                           const int *keys, const int2 *ranges, float4 *gxs, float4 *gx_own, float *glam_own,
-                           float *grho, unsigned long long *corr, const uint32_t *trec, uint32_t *srec, int n,
-                           const SP &sp, int block, hipStream_t st);
+void launch_adj_projection(const float4 *xs, const float *m, const float *lambda, float4 *prec, const int *keys,
+                           const int2 *ranges, float4 *gxs, float4 *gx_own, float *glam_own,
+                           unsigned long long *corr, const uint32_t *trec, uint32_t *srec, int n, const SP &sp,
+                           int block, hipStream_t st);
 template <class SP>
 void launch_adj_planes(const float4 *xp, const float4 *x1, float4 *gxs, float4 *gx1, int n, const SP &sp,
                        hipStream_t st);
