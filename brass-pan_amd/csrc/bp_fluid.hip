@@ -259,17 +259,21 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_post_a(const fl
         rec.planes = planes;
         rec.pstride = pstride;
         const bool any_phase = planes != nullptr;
-        for_each_neighbor(
+        struct Partner {
+            float4 b, v;  // x*, (v, density)
+        };
+        for_each_neighbor_nb<kBodyBatch>(
             G, ranges, g, xs, 0,
             [&](int, float4 b) { return (any_phase | (phase_of(b) == ph)) & (dist2(pi, xyz(b)) <= K.h2); },
-            [&](int j, float4 b) {
+            [&](int j) { return Partner{xs[j], v[j]}; },
+            [&](int, const Partner &P) {
+                const float4 b = P.b;
                 if (!(dist2(pi, xyz(b)) > 0.0f)) return;  // coincident partners are skipped
                 const f3 diff = pi - xyz(b);
                 const float r = fsqrt(dist2(pi, xyz(b)));
                 const f3 gr = K.grad_coef(r, frcp(r)) * diff;
-                const float4 vj = v[j];
-                if (do_omega) om += cross(xyz(vj) - vi, gr);
-                if (do_normal) nn += frcp(vj.w) * gr;
+                if (do_omega) om += cross(xyz(P.v) - vi, gr);
+                if (do_normal) nn += frcp(P.v.w) * gr;
             },
             AllowNotSelf{i}, rec);
         // (x, |omega|) and (normal, density) are the two records the
