@@ -322,11 +322,13 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 1) void k_post_b(const fl
             float4 xo;   // (x, |omega|) of the partner
             float4 nrm;  // nrm.w: the partner's density
         };
-        const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        // one 16-byte fetch of the second record whatever is used of it (a
+        // conditional fetch is split into per-component loads)
+        const float4 *__restrict__ second = do_akinci ? normal : xom;
         for_each_neighbor_nb<kBodyBatch>(
             G, ranges, g, xs, 0,
             [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
-            [&](int j) { return Partner{xom[j], do_akinci ? normal[j] : zero4}; },
+            [&](int j) { return Partner{xom[j], second[j]}; },
             [&](int, const Partner &P) {
                 const float4 b = P.xo;
                 if (!(dist2(pi, xyz(b)) > 0.0f)) return;  // coincident partners are skipped
