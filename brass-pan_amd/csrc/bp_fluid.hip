@@ -58,7 +58,8 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_lambda(const fl
         rec.n = n;
         rec.planes = L.planes;  // the screen tests distance only
         rec.pstride = L.pstride;
-        for_each_neighbor_nb<kBodyBatch>(
+        // one record per partner, so four partners per fetch fit the register budget
+        for_each_neighbor_nb<4>(
             G, ranges, g, xs, srec ? 0 : i, [&](int, float4 b) { return dist2(pi, xyz(b)) <= K.h2; },
             [&](int j) { return L.xm[j]; },  // (x*, m), m negated for a non-fluid partner
             [&](int, const float4 &b) {
