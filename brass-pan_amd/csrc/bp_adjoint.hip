@@ -222,7 +222,7 @@ __global__ __launch_bounds__(kMaxBlock) void ka_xsph(const float4 *__restrict__ 
                                                      const int2 *__restrict__ ranges, float4 *__restrict__ gv2_own,
                                                      float4 *__restrict__ gx_own, float *__restrict__ cacc,
                                                      unsigned long long *__restrict__ corr, int n, SP sp) {
-    const int i = xcd_index();
+    const int i = xcd_range_slot();
     if (i >= n) return;
     sp.with_scene(i, [&](const SceneConst &S) {
         const GridDesc &G = S.G;
@@ -314,7 +314,7 @@ __global__ __launch_bounds__(kMaxBlock) void ka_position(const float4 *__restric
                                                          unsigned long long *__restrict__ corr,
                                                          const uint32_t *__restrict__ trec,
                                                          uint32_t *__restrict__ srec, int n, SP sp) {
-    const int k = xcd_index();
+    const int k = xcd_range_slot();
     if (k >= n) return;
     sp.with_scene(k, [&](const SceneConst &S) {
         const GridDesc &G = S.G;
@@ -406,7 +406,7 @@ __global__ __launch_bounds__(kMaxBlock) void ka_lambda(const float4 *__restrict_
                                                        unsigned long long *__restrict__ corr,
                                                        const uint32_t *__restrict__ trec,
                                                        const uint32_t *__restrict__ srec, int n, SP sp) {
-    const int k = xcd_index();
+    const int k = xcd_range_slot();
     if (k >= n) return;
     sp.with_scene(k, [&](const SceneConst &S) {
         const GridDesc &G = S.G;

@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_particle_particle(const float4 *_
                                                                  int *__restrict__ contacts,
                                                                  const uint32_t *__restrict__ cell_stamp,
                                                                  const uint32_t *__restrict__ stamp) {
-    const int i = xcd_index();
+    const int i = pass_slot();
     if (i >= n) return;
     sp.with_scene(i, [&](const SceneConst &S) {
         const GridDesc &G = S.G;

@@ -29,7 +29,7 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_lambda(const fl
                                                    const int2 *__restrict__ ranges, float *__restrict__ lambda,
                                                    float *__restrict__ density, float4 *__restrict__ lam_rec,
                                                    uint32_t *__restrict__ srec, LambdaIn L, int n, SP sp) {
-    const int i = xcd_index();
+    const int i = pass_slot();
     if (i >= n) return;
     sp.with_scene(i, [&](const SceneConst &S) {
         const GridDesc &G = S.G;
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_position(const 
                                                      const uint32_t *__restrict__ fluid_stamp,
                                                      const uint32_t *__restrict__ stamp, float4 *__restrict__ sum_out,
                                                      unsigned long long *__restrict__ corr, int n, SP sp) {
-    const int k = xcd_index();
+    const int k = pass_slot();
     if (k >= n) return;
     sp.with_scene(k, [&](const SceneConst &S) {
         const GridDesc &G = S.G;
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_post_a(const fl
                                                    float4 *__restrict__ normal, uint32_t *__restrict__ ccell,
                                                    uint32_t *__restrict__ srec, const float *__restrict__ planes,
                                                    int pstride, int n, SP sp, int do_omega, int do_normal) {
-    const int i = xcd_index();
+    const int i = pass_slot();
     if (i >= n) return;
     sp.with_scene(i, [&](const SceneConst &S) {
         const GridDesc &G = S.G;
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 1) void k_post_b(const fl
                                                    const uint32_t *__restrict__ ccell,
                                                    const uint32_t *__restrict__ srec, int n, SP sp, int do_vort,
                                                    int do_akinci, float dt) {
-    const int i = xcd_index();
+    const int i = pass_slot();
     if (i >= n) return;
     sp.with_scene(i, [&](const SceneConst &S) {
         const GridDesc &G = S.G;
@@ -363,7 +363,7 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_xsph(const floa
                                                  const uint32_t *__restrict__ ccell,
                                                  const uint32_t *__restrict__ srec, float4 *__restrict__ v_out,
                                                  int n, SP sp, int use_ccell) {
-    const int i = xcd_index();
+    const int i = pass_slot();
     if (i >= n) return;
     sp.with_scene(i, [&](const SceneConst &S) {
         const GridDesc &G = S.G;

@@ -531,18 +531,24 @@ __device__ __forceinline__ void gather_entries(const int *__restrict__ ents, int
     }
 }
 
-// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
-// b % 8).  Neighbour passes renumber them so that each XCD walks one
-// contiguous range of sorted slots and its private L2 holds the neighbour
-// cells it needs.  Bijective for any grid size.
-constexpr int kNumXcd = 8;
-__device__ __forceinline__ int xcd_block() {
+// The slot a neighbour-pass thread owns.  Workgroups are dealt to the 8 XCDs
+// round-robin (workgroup b runs on XCD b % 8), so consecutive slot blocks,
+// which hold neighbouring cells, spread over all XCDs.  Renumbering blocks so
+// that each XCD walks one contiguous slot range (its L2 holding its cells)
+// measured slower once these passes became bound by vector-memory issue and
+// by the imbalance of long walks rather than by HBM misses.
+__device__ __forceinline__ int pass_slot() { return (int)blockIdx.x * (int)blockDim.x + (int)threadIdx.x; }
+// The reverse (adjoint) neighbour passes, which fetch three to four records
+// per partner, keep the renumbering: each XCD walks one contiguous range of
+// sorted slots.  Bijective for any grid size.
+__device__ __forceinline__ int xcd_range_slot() {
+    constexpr int kNumXcd = 8;
     const int nb = (int)gridDim.x, b = (int)blockIdx.x;
     const int q = nb / kNumXcd, r = nb % kNumXcd;
     const int xcd = b % kNumXcd, local = b / kNumXcd;
-    return (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+    const int blk = (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+    return blk * (int)blockDim.x + (int)threadIdx.x;
 }
-__device__ __forceinline__ int xcd_index() { return xcd_block() * (int)blockDim.x + (int)threadIdx.x; }
 
 inline int grid_blocks(long long n, int block = kBlock) { return (int)((n + block - 1) / block); }
 
