@@ -49,7 +49,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_particle_particle(const float4 *_
                                                                  float4 *__restrict__ xs_out, int n, SP sp,
                                                                  int *__restrict__ contacts,
                                                                  const uint32_t *__restrict__ cell_stamp,
-                                                                 const uint32_t *__restrict__ stamp) {
+                                                                 const uint32_t *__restrict__ stamp, LambdaIn L,
+                                                                 const float *__restrict__ m) {
     const int i = pass_slot();
     if (i >= n) return;
     sp.with_scene(i, [&](const SceneConst &S) {
@@ -66,6 +67,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_particle_particle(const float4 *_
         G.pos(xip, g);
         if (cell_stamp && pi <= 0 && !stamp_near(G, g, cell_stamp, *stamp)) {
             xs_out[i] = a;
+            if (L.planes) store_lambda_in(L, i, xip, m[i], pi);
             return;
         }
         const float r2x4 = r * r * 4.0f;
@@ -108,6 +110,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_particle_particle(const float4 *_
             });
         const f3 d = (cnt == 0) ? sumD : sumD + sumF / (float)cnt;
         xs_out[i] = f4(xip + d, a.w);
+        // the lambda pass, which follows, reads x* also as LambdaIn
+        if (L.planes) store_lambda_in(L, i, xip + d, m[i], pi);
         // the adjoint treats this pass as the identity: it counts the particles
         // whose contact set is not empty
         if (contacts && (sumD.x != 0.0f || sumD.y != 0.0f || sumD.z != 0.0f || cnt > 0)) atomicAdd(contacts, 1);
@@ -390,9 +394,9 @@ void launch_plane_collide(float4 *xs, const float4 *x, int n, const SP &sp, hipS
 template <class SP>
 void launch_particle_particle(const float4 *xs, const float4 *x, const int2 *ranges, float4 *xs_out, int n,
                               const SP &sp, int block, int *contacts, const uint32_t *cell_stamp,
-                              const uint32_t *stamp, hipStream_t st) {
+                              const uint32_t *stamp, const LambdaIn &L, const float *m, hipStream_t st) {
     hipLaunchKernelGGL(k_particle_particle<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, x, ranges,
-                       xs_out, n, sp, contacts, cell_stamp, stamp);
+                       xs_out, n, sp, contacts, cell_stamp, stamp, L, m);
 }
 void launch_mark_cells(const int *sorted_keys, const int *ph, int n, uint32_t *stamp, uint32_t *cell_stamp,
                        uint32_t *fluid_stamp, hipStream_t st) {
@@ -444,7 +448,7 @@ void launch_finish(const float4 *xs, float4 *x, float4 *v, const float *vw, int 
     template void launch_plane_collide<SP>(float4 *, const float4 *, int, const SP &, hipStream_t);              \
     template void launch_particle_particle<SP>(const float4 *, const float4 *, const int2 *, float4 *, int,      \
                                                const SP &, int, int *, const uint32_t *, const uint32_t *,       \
-                                               hipStream_t);                                                     \
+                                               const LambdaIn &, const float *, hipStream_t);                    \
     template void launch_wind<SP>(float4 *, const float4 *, const float *, const int4 *, const int *, float4 *,  \
                                   int, const SP &, float, const int *, const int *, const int *, int, hipStream_t);
 BP_INSTANTIATE_CONSTRAINTS(OneScene)

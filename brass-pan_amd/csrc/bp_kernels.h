@@ -54,7 +54,7 @@ void launch_plane_collide(float4 *xs, const float4 *x, int n, const SP &sp, hipS
 template <class SP>
 void launch_particle_particle(const float4 *xs, const float4 *x, const int2 *ranges, float4 *xs_out, int n,
                               const SP &sp, int block, int *contacts, const uint32_t *cell_stamp,
-                              const uint32_t *stamp, hipStream_t st);
+                              const uint32_t *stamp, const LambdaIn &L, const float *m, hipStream_t st);
 // advances the substep stamp and marks the cells holding a solid (cell_stamp)
 // and those holding a fluid particle (fluid_stamp)
 void launch_mark_cells(const int *sorted_keys, const int *ph, int n, uint32_t *stamp, uint32_t *cell_stamp,

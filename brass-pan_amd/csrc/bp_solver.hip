@@ -214,9 +214,11 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
         if (s->any_planes && !planes_done) TIMED(F_PLANES, launch_plane_collide(s->xs.cur(), s->x.cur(), N, sp(), st));
         planes_done = false;
         if (s->has_solid) {
+            // the last pass to move x* before the lambda pass: it also writes LambdaIn
             TIMED(F_PP, launch_particle_particle(s->xs.cur(), s->x.cur(), s->ranges, s->xs.alt(), N, sp(), s->blk_pp,
-                                                 H.contacts, s->cell_stamp, s->stamp, st));
+                                                 H.contacts, s->cell_stamp, s->stamp, s->lin, s->m.cur(), st));
             s->xs.flip();
+            planes_ready = s->lin.planes != nullptr;
         }
         if (H.xl) HIP_TRY(snap(H.xl[it], s->xs.cur(), N, st));
         if (s->has_fluid) {
