@@ -272,13 +272,33 @@ __global__ __launch_bounds__(kBlock) void k_wind_compute(const float4 *__restric
 // Sums are taken in particle order (every lane walks the body in LDS), so all
 // lanes hold the same centre of mass and Apq; the Müller 2016 rotation
 // extraction (shapematching.cuh:70-88) runs uniformly on every lane.
+// Blocks past the last body apply the pins (immovableConstraints) when the
+// caller has found no pinned particle inside a rigid body (Pins.n > 0 only
+// then): the two launches then touch disjoint particles.
+struct Pins {
+    const float4 *x;
+    const int *ids;
+    int n;
+};
+__device__ __forceinline__ void pin_one(float4 *xs, const float4 *x, const int *ids, const int *map_to_new, int t) {
+    const int slot = map_to_new[ids[t]];
+    const float4 a = xs[slot];
+    const float4 b = x[slot];
+    xs[slot] = make_float4(b.x, b.y, b.z, a.w);
+}
+
 __global__ __launch_bounds__(64) void k_shape_matching(float4 *__restrict__ xs, const int *__restrict__ map_to_new,
                                                        const float4 *__restrict__ rest,
                                                        const int2 *__restrict__ ranges, float4 *__restrict__ rot,
-                                                       float4 *__restrict__ cms, int iters) {
+                                                       float4 *__restrict__ cms, int iters, int nr, Pins pins) {
     __shared__ float sp[kMaxRigidParticles][3];
     __shared__ float sq[kMaxRigidParticles][3];
     const int b = blockIdx.x, t = threadIdx.x;
+    if (b >= nr) {
+        const int i = (b - nr) * 64 + t;
+        if (i < pins.n) pin_one(xs, pins.x, pins.ids, map_to_new, i);
+        return;
+    }
     const int2 rg = ranges[b];
     const int np = rg.y - rg.x;
     int slot = 0;
@@ -355,10 +375,7 @@ __global__ __launch_bounds__(kBlock) void k_immovable(float4 *__restrict__ xs, c
                                                       const int *__restrict__ map_to_new, int ni) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ni) return;
-    const int slot = map_to_new[ids[t]];
-    const float4 a = xs[slot];
-    const float4 b = x[slot];
-    xs[slot] = make_float4(b.x, b.y, b.z, a.w);
+    pin_one(xs, x, ids, map_to_new, t);
 }
 
 // setParticle (particlesolver.cuh:388-393) on the picked particle, found by
@@ -429,8 +446,11 @@ void launch_wind(float4 *xs, const float4 *v, const float *m, const int4 *faces,
                        map_to_new, m, 1, nv);
 }
 void launch_shape_matching(float4 *xs, const int *map_to_new, const float4 *rest, const int2 *ranges, float4 *rot,
-                           float4 *cms, int nr, int iters, hipStream_t st) {
-    hipLaunchKernelGGL(k_shape_matching, dim3(nr), dim3(64), 0, st, xs, map_to_new, rest, ranges, rot, cms, iters);
+                           float4 *cms, int nr, int iters, hipStream_t st, const float4 *pin_x, const int *pin_ids,
+                           int npins) {
+    const Pins pins{pin_x, pin_ids, npins};
+    hipLaunchKernelGGL(k_shape_matching, dim3(nr + (npins + 63) / 64), dim3(64), 0, st, xs, map_to_new, rest, ranges,
+                       rot, cms, iters, nr, pins);
 }
 void launch_immovable(float4 *xs, const float4 *x, const int *ids, const int *map_to_new, int ni, hipStream_t st) {
     hipLaunchKernelGGL(k_immovable, dim3(grid_blocks(ni)), dim3(kBlock), 0, st, xs, x, ids, map_to_new, ni);

@@ -48,10 +48,14 @@ __global__ __launch_bounds__(kBlock) void k_predict_hash(const float4 *__restric
 // Stable LSD radix sort, 8-bit digits, only over the bits the cell count
 // needs (keys < numCells, so the order equals a sort over all 32 bits,
 // cub::DeviceRadixSort::SortPairs, particlesolver.cuh:721-738).
+// A block sorts a tile of kSortWaves * 64 * R keys, R keys per lane.  Below
+// kSortBigN keys, R = 4: 4096-key tiles would leave most of the 256 CUs idle
+// (283k keys make 70 tiles) while each wave runs 16 dependent rounds.
 constexpr int kRadix = 256;
 constexpr int kSortWaves = 4;
-constexpr int kSortRounds = 16;                        // keys per lane
-constexpr int kSortTile = kSortWaves * 64 * kSortRounds;  // 4096 keys / block
+constexpr int kSortBigN = 1 << 20;
+__host__ __device__ constexpr int sort_tile(int rounds) { return kSortWaves * 64 * rounds; }
+static int sort_rounds(int n) { return n >= kSortBigN ? 16 : 4; }
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
     const int lane = __lane_id();
@@ -70,17 +74,18 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
 }
 
 // per-block digit histogram, written digit-major: counts[d * nb + block]
+template <int R>
 __global__ __launch_bounds__(kSortWaves * 64) void k_radix_hist(const int *__restrict__ keys, int n, int shift,
                                                                int *__restrict__ counts, int nb) {
     __shared__ int h[kRadix];
     const int tid = threadIdx.x;
     h[tid] = 0;
     __syncthreads();
-    const int base = blockIdx.x * kSortTile;
-    // each wave reads its own contiguous quarter of the tile in 16 rounds
+    const int base = blockIdx.x * sort_tile(R);
+    // each wave reads its own contiguous quarter of the tile in R rounds
     const int wave = tid >> 6, lane = tid & 63;
-    const int wbase = base + wave * 64 * kSortRounds;
-    for (int r = 0; r < kSortRounds; ++r) {
+    const int wbase = base + wave * 64 * R;
+    for (int r = 0; r < R; ++r) {
         const int idx = wbase + r * 64 + lane;
         const bool valid = idx < n;
         const uint32_t d = valid ? ((uint32_t)keys[idx] >> shift) & (kRadix - 1) : 0u;
@@ -116,6 +121,7 @@ __global__ __launch_bounds__(256) void k_radix_scan_rows(int *__restrict__ count
 }
 
 // stable scatter: rank = digit base + block prefix + earlier waves + earlier lanes
+template <int R>
 __global__ __launch_bounds__(kSortWaves * 64) void k_radix_scatter(const int *__restrict__ kin,
                                                                   const int *__restrict__ vin,
                                                                   int *__restrict__ kout, int *__restrict__ vout,
@@ -141,12 +147,12 @@ __global__ __launch_bounds__(kSortWaves * 64) void k_radix_scatter(const int *__
     __syncthreads();
     dbase[tid] = excl;
 
-    const int wbase = blockIdx.x * kSortTile + wave * 64 * kSortRounds;
-    int kreg[kSortRounds];
-    int rreg[kSortRounds];
+    const int wbase = blockIdx.x * sort_tile(R) + wave * 64 * R;
+    int kreg[R];
+    int rreg[R];
     const uint64_t lt = lanemask_lt();
 #pragma unroll
-    for (int r = 0; r < kSortRounds; ++r) {
+    for (int r = 0; r < R; ++r) {
         const int idx = wbase + r * 64 + lane;
         const bool valid = idx < n;
         const int k = valid ? kin[idx] : 0;
@@ -173,7 +179,7 @@ __global__ __launch_bounds__(kSortWaves * 64) void k_radix_scatter(const int *__
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < kSortRounds; ++r) {
+    for (int r = 0; r < R; ++r) {
         const int idx = wbase + r * 64 + lane;
         if (idx < n) {
             const uint32_t d = ((uint32_t)kreg[r] >> shift) & (kRadix - 1);
@@ -267,7 +273,8 @@ void launch_predict_hash(const float4 *x, const float4 *v, int *keys, int n, flo
 }
 
 size_t radix_counts_len(int n) {
-    const int nb = (n + kSortTile - 1) / kSortTile;
+    const int tile = sort_tile(sort_rounds(n));
+    const int nb = (n + tile - 1) / tile;
     return (size_t)nb * kRadix;
 }
 
@@ -276,16 +283,19 @@ size_t radix_counts_len(int n) {
 // Returns in *out_keys / *out_vals which buffers hold the result.
 void launch_radix_sort(int *keys_a, int *keys_b, int *vals_a, int *vals_b, int *counts, int *totals, int n, int bits,
                        int **out_keys, int **out_vals, hipStream_t st) {
-    const int nb = (n + kSortTile - 1) / kSortTile;
+    const bool big = sort_rounds(n) == 16;
+    const int tile = sort_tile(sort_rounds(n));
+    const int nb = (n + tile - 1) / tile;
     int *kin = keys_a, *kout = keys_b, *vin = nullptr, *vout = vals_b;
     int passes = (bits + 7) / 8;
     if (passes < 1) passes = 1;
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
-        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortWaves * 64), 0, st, kin, n, shift, counts, nb);
+        hipLaunchKernelGGL(big ? k_radix_hist<16> : k_radix_hist<4>, dim3(nb), dim3(kSortWaves * 64), 0, st, kin, n,
+                           shift, counts, nb);
         hipLaunchKernelGGL(k_radix_scan_rows, dim3(kRadix), dim3(256), 0, st, counts, nb, totals);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kSortWaves * 64), 0, st, kin, vin, kout, vout, n, shift,
-                           counts, totals, nb);
+        hipLaunchKernelGGL(big ? k_radix_scatter<16> : k_radix_scatter<4>, dim3(nb), dim3(kSortWaves * 64), 0, st, kin,
+                           vin, kout, vout, n, shift, counts, totals, nb);
         // next pass reads what this one wrote
         int *nk = kout, *nv = vout;
         kout = kin;

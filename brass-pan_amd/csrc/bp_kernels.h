@@ -68,8 +68,11 @@ template <class SP>
 void launch_wind(float4 *xs, const float4 *v, const float *m, const int4 *faces, const int *map_to_new,
                  float4 *fbuf, int nf, const SP &sp, float dt, const int *verts, const int *offs, const int *ents,
                  int nv, hipStream_t st);
+// With npins > 0 the same launch also applies the pins (launch_immovable); the
+// caller guarantees that no pinned particle belongs to a rigid body.
 void launch_shape_matching(float4 *xs, const int *map_to_new, const float4 *rest, const int2 *ranges, float4 *rot,
-                           float4 *cms, int nr, int iters, hipStream_t st);
+                           float4 *cms, int nr, int iters, hipStream_t st, const float4 *pin_x = nullptr,
+                           const int *pin_ids = nullptr, int npins = 0);
 void launch_immovable(float4 *xs, const float4 *x, const int *ids, const int *map_to_new, int ni, hipStream_t st);
 void launch_pick(float4 *pos, float4 *v, const int *map_to_new, int orig, f3 p, int keep_pos, f3 vel,
                  hipStream_t st);

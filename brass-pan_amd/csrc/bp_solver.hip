@@ -253,10 +253,14 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
             TIMED(F_BENDING, launch_bending(s->xs.cur(), s->x.cur(), s->bends, s->map_to_new, s->bbuf, s->nb,
                                             P.bend_stiffness, s->bcsr.verts, s->bcsr.offs, s->bcsr.ents, s->bcsr.nv,
                                             st));
+        // pins apply after shape matching; in the same launch when they touch no rigid body
+        const int fused_pins = s->pins_apart_from_rigid ? s->ni : 0;
         if (s->nr > 0)
             TIMED(F_SHAPE, launch_shape_matching(s->xs.cur(), s->map_to_new, s->rrest, s->rranges, s->rrot, s->rcm,
-                                                 s->nr, P.shape_matching_iterations, st));
-        if (s->ni > 0) TIMED(F_IMMOVABLE, launch_immovable(s->xs.cur(), s->x.cur(), s->immov, s->map_to_new, s->ni, st));
+                                                 s->nr, P.shape_matching_iterations, st, s->x.cur(), s->immov,
+                                                 fused_pins));
+        if (s->ni > 0 && !(s->nr > 0 && fused_pins > 0))
+            TIMED(F_IMMOVABLE, launch_immovable(s->xs.cur(), s->x.cur(), s->immov, s->map_to_new, s->ni, st));
     }
     HIP_TRY(snap(H.xf, s->xs.cur(), N, st));
     if (!finished)
@@ -616,6 +620,13 @@ bp_status create_impl(const bp_scene_desc *scs, const bp_params *params, int ns,
     }
     s->ni = (int)im.size();
     if (s->ni > 0) TRY(s->upload(&s->immov, im.data(), s->ni));
+    {  // may the shape-matching launch also apply the pins?
+        std::vector<char> in_body(N, 0);
+        for (int b = 0; b < s->nr; ++b)
+            for (int i = rr[2 * b]; i < rr[2 * b + 1]; ++i) in_body[i] = 1;
+        s->pins_apart_from_rigid = true;
+        for (int id : im) s->pins_apart_from_rigid = s->pins_apart_from_rigid && !in_body[id];
+    }
     HTRY(hipStreamSynchronize(s->st));
 #undef HTRY
 #undef TRY

@@ -110,6 +110,7 @@ struct bp_solver {
     float4 *rrest = nullptr, *rrot = nullptr, *rcm = nullptr;
     int ni = 0;
     int *immov = nullptr;
+    bool pins_apart_from_rigid = false;  // no pinned particle lies in a rigid body
 
     std::vector<float> mass_by_orig;
     std::vector<int> phase_by_orig;
