@@ -107,12 +107,7 @@ def kernel_roofline(s, subdt, substeps=5, workload=None):
     }, {k: [round(v[0], 3), v[1]] for k, v in sorted(times.items(), key=lambda kv: -kv[1][0])}
 
 
-def cpu_baseline(sc, prm, subdt, budget_s=20.0):
-    """Single-threaded FP32 oracle (reference order) on a bounded sample."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle_py as orc
-
-    o = orc.Oracle(bp.SceneArrays(sc), prm, fp64=False, mode=orc.SCATTER)
+def _oracle_rate(o, n, subdt, budget_s, max_substeps=50):
     o.step(1, subdt)  # warm-up substep (untimed)
     done = 0
     t0 = time.perf_counter()
@@ -120,11 +115,40 @@ def cpu_baseline(sc, prm, subdt, budget_s=20.0):
         o.step(1, subdt)
         done += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or done >= 50:
+        if el >= budget_s or done >= max_substeps:
             break
-    return {"value": round(sc.num_particles() * done / el, 1), "unit": "particle-substeps/s", "cores": 1,
-            "kind": "port", "sample": f"{done} substeps after 1 warm-up substep, {el:.1f} s, single-threaded "
-                                      f"FP32 oracle in reference order"}
+    return n * done / el, done, el
+
+
+def host_threads():
+    """Threads this process may use: the OpenMP setting, else the CPUs in its affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(sc, prm, subdt, budget_s=10.0):
+    """The FP32 oracle on a bounded sample of the same scene, in two forms:
+    gather form (the GPU's pair order) over all host threads with OpenMP, and
+    the reference's own order (scatter form) on one thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as orc
+
+    n = sc.num_particles()
+    threads = orc.set_threads(host_threads())
+    o = orc.Oracle(bp.SceneArrays(sc), prm, fp64=False, mode=orc.GATHER)
+    rate_mt, done_mt, el_mt = _oracle_rate(o, n, subdt, budget_s)
+    del o
+    orc.set_threads(1)
+    o = orc.Oracle(bp.SceneArrays(sc), prm, fp64=False, mode=orc.SCATTER)
+    rate_1, done_1, el_1 = _oracle_rate(o, n, subdt, budget_s)
+    return ({"value": round(rate_mt, 1), "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+             "sample": f"{done_mt} substeps after 1 warm-up substep, {el_mt:.1f} s, FP32 oracle in gather form "
+                       f"(the GPU's pair order) over {threads} OpenMP threads"},
+            {"value": round(rate_1, 1), "unit": "particle-substeps/s", "cores": 1, "kind": "port",
+             "sample": f"{done_1} substeps after 1 warm-up substep, {el_1:.1f} s, single-threaded FP32 oracle "
+                       f"in reference order"})
 
 
 def hip_stream():
@@ -379,7 +403,7 @@ def main():
     if not args.no_adjoint:
         out["adjoint_64"] = time_adjoint(graphs=not args.no_graph)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(sc, prm, subdt)
+        out["cpu_baseline"], out["cpu_baseline_reference_order"] = cpu_baseline(sc, prm, subdt)
     print(json.dumps(out), flush=True)
 
 

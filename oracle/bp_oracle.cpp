@@ -9,6 +9,8 @@
 // changes the FP32 bits.
 #include "bp_oracle.h"
 
+#include <omp.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -17,6 +19,11 @@
 #include <numeric>
 #include <type_traits>
 #include <vector>
+
+// Per-particle loops whose iteration i writes only slot i's outputs (every
+// neighbour pass in gather form, and the density and contact passes in both
+// forms) run over OpenMP threads; results do not depend on the thread count.
+#define OR_PARALLEL_FOR _Pragma("omp parallel for schedule(dynamic, 256)")
 
 namespace {
 
@@ -492,6 +499,7 @@ template <class R> struct Solver : SolverBase {
     }
     // planeStabilize (particleplane.cuh:3-19)
     void plane_stabilize(V3<R> o, V3<R> nrm) {
+        OR_PARALLEL_FOR
         for (int i = 0; i < N; ++i) {
             V3<R> o2p = o - x[i];
             R d = dot(o2p, nrm) + radius;
@@ -503,6 +511,7 @@ template <class R> struct Solver : SolverBase {
     // particlePlaneCollisionConstraint (particleplane.cuh:21-58)
     void plane_collide(V3<R> o, V3<R> nrm) {
         const R fs = R(P.friction_static), fd = R(P.friction_dynamic);
+        OR_PARALLEL_FOR
         for (int i = 0; i < N; ++i) {
             V3<R> o2p = o - xs[i];
             R d = dot(o2p, nrm) + radius;
@@ -533,6 +542,7 @@ template <class R> struct Solver : SolverBase {
         const R fs = R(P.friction_static), fd = R(P.friction_dynamic);
         const R r = radius;
         for (int i = 0; i < N; ++i) dx[i] = V3<R>{0, 0, 0};
+        OR_PARALLEL_FOR
         for (int i = 0; i < N; ++i) {
             const V3<R> xi = x[i], xip = xs[i];
             V3<R> sumD{0, 0, 0}, sumF{0, 0, 0};
@@ -579,6 +589,7 @@ template <class R> struct Solver : SolverBase {
     // fluidLambda (fluid.cuh:78-159)
     void fluid_lambda() {
         const R sds = R(P.solid_density_scale);
+        OR_PARALLEL_FOR
         for (int i = 0; i < N; ++i) {
             if (phase[i] > 0) {  // solids carry lambda 0 (the reference leaves stale values)
                 lambda[i] = R(0);
@@ -657,6 +668,7 @@ template <class R> struct Solver : SolverBase {
             // (order-independent integer sums), as the GPU kernel does.
             std::vector<long long> corr((size_t)3 * N, 0);
             std::vector<V3<R>> sums(N);
+            OR_PARALLEL_FOR
             for (int k = 0; k < N; ++k) {
                 const V3<R> pk = xs[k];
                 const int bk = sorted_keys[k];
@@ -679,9 +691,15 @@ template <class R> struct Solver : SolverBase {
                     sum += mass[j] * delta;
                     if (j > k && !partner_sees_k) {
                         V3<R> give = -mass[k] * delta;
-                        corr[3 * (size_t)j + 0] += std::llrint((double)give.x * 1099511627776.0);
-                        corr[3 * (size_t)j + 1] += std::llrint((double)give.y * 1099511627776.0);
-                        corr[3 * (size_t)j + 2] += std::llrint((double)give.z * 1099511627776.0);
+                        const long long gx = std::llrint((double)give.x * 1099511627776.0);
+                        const long long gy = std::llrint((double)give.y * 1099511627776.0);
+                        const long long gz = std::llrint((double)give.z * 1099511627776.0);
+#pragma omp atomic
+                        corr[3 * (size_t)j + 0] += gx;
+#pragma omp atomic
+                        corr[3 * (size_t)j + 1] += gy;
+#pragma omp atomic
+                        corr[3 * (size_t)j + 2] += gz;
                     }
                 });
                 sums[k] = sum;
@@ -815,6 +833,7 @@ template <class R> struct Solver : SolverBase {
     }
     // fluidOmega (fluid.cuh:231-279)
     void fluid_omega() {
+        OR_PARALLEL_FOR
         for (int i = 0; i < N; ++i) {
             const int pi = phase[i];
             if (pi >= 0) continue;
@@ -833,6 +852,7 @@ template <class R> struct Solver : SolverBase {
     // fluidVorticity (fluid.cuh:281-336), in place on v
     void fluid_vorticity(R dt) {
         const R eps = R(P.vorticity_eps);
+        OR_PARALLEL_FOR
         for (int i = 0; i < N; ++i) {
             const int pi = phase[i];
             if (pi >= 0) continue;
@@ -853,6 +873,7 @@ template <class R> struct Solver : SolverBase {
     }
     // fluidNormal (fluid.cuh:387-434)
     void fluid_normal() {
+        OR_PARALLEL_FOR
         for (int i = 0; i < N; ++i) {
             const int pi = phase[i];
             if (pi >= 0) continue;
@@ -873,6 +894,7 @@ template <class R> struct Solver : SolverBase {
     // reference leaves whatever the swap buffer held in those slots.
     void fluid_akinci(R dt) {
         const R g = R(P.tension_gamma);
+        OR_PARALLEL_FOR
         for (int i = 0; i < N; ++i) {
             const int pi = phase[i];
             vtmp[i] = v[i];
@@ -900,6 +922,7 @@ template <class R> struct Solver : SolverBase {
     // fluidXSph (fluid.cuh:338-385) -> vtmp, then swap (particlesolver.cuh:1060)
     void fluid_xsph() {
         const R c = th_xsph;
+        OR_PARALLEL_FOR
         for (int i = 0; i < N; ++i) {
             const int pi = phase[i];
             vtmp[i] = v[i];
@@ -1135,6 +1158,11 @@ double or_rollout_tangent(const bp_scene_desc *scene, const bp_params *params, i
                           const double *target, double norm, const double *vdirs, int ndirs_v, double *dloss) {
     if (ndirs_v > kDirs - 6) ndirs_v = kDirs - 6;
     return rollout_loss<Dual>(scene, params, substeps, subdt, target, norm, nullptr, nullptr, vdirs, ndirs_v, dloss);
+}
+
+int or_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
 }
 
 void or_neighbor_grid(const float *xs, int n, const float cs[3], const int dims[3], int *keys, int *perm,

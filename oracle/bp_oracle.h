@@ -68,6 +68,10 @@ double or_rollout_tangent(const bp_scene_desc *scene, const bp_params *params, i
                           const double *target, double norm, const double *vdirs, int ndirs_v, double *dloss);
 
 /* updateGrid stand-alone (grid.cuh:30-99, particlesolver.cuh:712-739). */
+/* OpenMP threads for the per-particle loops (results do not depend on it);
+ * returns the count now in effect. n <= 0 only queries. */
+int or_set_threads(int n);
+
 void or_neighbor_grid(const float *xs, int n, const float cell_size[3],
                       const int grid_dims[3], int *keys, int *perm,
                       int *cell_start, int *cell_end);

@@ -36,6 +36,7 @@ def load():
             "or_debug_fetch": (None, [C.c_void_p, C.c_int, _D]),
             "or_debug_fetch_int": (None, [C.c_void_p, C.c_int, _I]),
             "or_neighbor_grid": (None, [_F, C.c_int, _F, _I, _I, _I, _I, _I]),
+            "or_set_threads": (C.c_int, [C.c_int]),
             "or_cell_address": (C.c_int, [_F, _F, _I]),
             "or_poly6": (C.c_double, [C.c_double, C.c_double, C.c_int]),
             "or_grad_spiky": (None, [_D, C.c_double, C.c_double, C.c_int, _D]),
@@ -134,6 +135,11 @@ class Oracle:
         out = np.empty(self.n if which in (0, 1) else (self.n, 3))
         self.lib.or_debug_fetch(self.h, int(which), _d(out))
         return out
+
+
+def set_threads(n=0):
+    """OpenMP threads of the oracle's per-particle loops; n <= 0 only queries."""
+    return int(load().or_set_threads(int(n)))
 
 
 def neighbor_grid(xs, cell_size, grid_dims):
