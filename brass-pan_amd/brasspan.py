@@ -93,6 +93,7 @@ SIGNATURES = {
     "bp_batch_scene_range": (_S, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "bp_solver_set_graph": (_S, [_P, C.c_int]),
     "bp_solver_autotune": (_S, [_P, C.c_float, C.c_int, C.c_char_p, C.c_int]),
+    "bp_solver_set_fusion": (_S, [_P, C.c_int]),
     "bp_forward_checkpointed": (_S, [_P, C.c_int, C.c_float, C.c_int]),
     "bp_backward": (_S, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
                          C.POINTER(C.c_float), C.POINTER(C.c_float)]),
@@ -238,6 +239,10 @@ class Solver:
 
     def set_graph(self, on: bool = True):
         _check(self.lib.bp_solver_set_graph(self.h, 1 if on else 0))
+
+    def set_fusion(self, on: bool = True):
+        """Iteration fusion on or off (bp_solver_set_fusion; results are bit-identical)."""
+        _check(self.lib.bp_solver_set_fusion(self.h, 1 if on else 0))
 
     def autotune(self, dt: float, substeps_per_trial: int = 2) -> dict:
         import json

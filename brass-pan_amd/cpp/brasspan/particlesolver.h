@@ -115,7 +115,9 @@ struct SolverHandle {
     void synchronize() { bp_check(bp_solver_sync(handle)); }
     // Replays pairs of substeps from a captured hipGraph.
     void setGraph(bool enable) { bp_check(bp_solver_set_graph(handle, enable ? 1 : 0)); }
-    // Chooses the neighbour-pass workgroup sizes; returns the JSON report.
+    // Iteration fusion (epilogue passes inside their producing kernels); bit-identical either way.
+    void setFusion(bool enable) { bp_check(bp_solver_set_fusion(handle, enable ? 1 : 0)); }
+    // Chooses the neighbour-pass workgroup sizes and iteration fusion; returns the JSON report.
     std::string autotune(float subDeltaTime, int substepsPerTrial = 2) {
         char buf[512] = {0};
         bp_check(bp_solver_autotune(handle, subDeltaTime, substepsPerTrial, buf, (int)sizeof(buf)));

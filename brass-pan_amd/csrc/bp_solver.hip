@@ -315,7 +315,7 @@ bp_status run_substeps(bp_solver *s, int count, float subdt) {
     int k = 0;
     if (s->use_graph && !s->timing) {
         while (k < count && s->sk_idx < 0) {  // the first substep has no ranges to clear
-            bp_status r = substep(s, subdt, inv_dt, gstep, false, -1, mk3(0.f, 0.f, 0.f));
+            bp_status r = substep(s, subdt, inv_dt, gstep, false, -1, mk3(0.f, 0.f, 0.f), !s->fuse);
             if (r != BP_OK) return r;
             ++k;
         }
@@ -324,8 +324,8 @@ bp_status run_substeps(bp_solver *s, int count, float subdt) {
                 s->drop_graph();
                 const int sig0 = s->buffer_sig();
                 HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
-                bp_status r1 = substep(s, subdt, inv_dt, gstep, false, -1, mk3(0.f, 0.f, 0.f));
-                bp_status r2 = (r1 == BP_OK) ? substep(s, subdt, inv_dt, gstep, false, -1, mk3(0.f, 0.f, 0.f)) : r1;
+                bp_status r1 = substep(s, subdt, inv_dt, gstep, false, -1, mk3(0.f, 0.f, 0.f), !s->fuse);
+                bp_status r2 = (r1 == BP_OK) ? substep(s, subdt, inv_dt, gstep, false, -1, mk3(0.f, 0.f, 0.f), !s->fuse) : r1;
                 hipGraph_t g = nullptr;
                 hipError_t e = hipStreamEndCapture(s->st, &g);
                 if (r2 != BP_OK) return r2;
@@ -342,7 +342,7 @@ bp_status run_substeps(bp_solver *s, int count, float subdt) {
         }
     }
     for (; k < count; ++k) {
-        bp_status r = substep(s, subdt, inv_dt, gstep, false, -1, mk3(0.f, 0.f, 0.f));
+        bp_status r = substep(s, subdt, inv_dt, gstep, false, -1, mk3(0.f, 0.f, 0.f), !s->fuse);
         if (r != BP_OK) return r;
     }
     return BP_OK;
@@ -735,7 +735,7 @@ bp_status bp_solver_step(bp_solver *s, int num_substeps, float dt, int picked, c
         const f3 gstep = gravity_step(s->P, subdt);
         const f3 pp = mk3(ppos[0], ppos[1], ppos[2]);
         for (int k = 0; k < num_substeps; ++k) {
-            bp_status st = substep(s, subdt, inv_dt, gstep, true, picked, pp);
+            bp_status st = substep(s, subdt, inv_dt, gstep, true, picked, pp, !s->fuse);
             if (st != BP_OK) return st;
         }
         // :1066-1071: keep the solved position, set the picked velocity
@@ -760,6 +760,30 @@ bp_status bp_solver_set_graph(bp_solver *s, int enable) {
     if (!s->use_graph) s->drop_graph();
     return BP_OK;
 }
+
+bp_status bp_solver_set_fusion(bp_solver *s, int enable) {
+    if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
+    s->fuse = enable != 0;
+    s->drop_graph();
+    return BP_OK;
+}
+
+namespace {
+// GPU time of `count` substeps (HIP events on the handle's stream)
+bp_status time_substeps(bp_solver *s, int count, float dt, float *ms) {
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    HIP_TRY(hipEventRecord(a, s->st));
+    bp_status st = run_substeps(s, count, dt);
+    HIP_TRY(hipEventRecord(b, s->st));
+    HIP_TRY(hipEventSynchronize(b));
+    HIP_TRY(hipEventElapsedTime(ms, a, b));
+    HIP_TRY(hipEventDestroy(a));
+    HIP_TRY(hipEventDestroy(b));
+    return st;
+}
+}  // namespace
 
 bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, char *report, int report_len) {
     if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
@@ -801,6 +825,18 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
     s->blk_xsph = best_blk[4];
     s->blk_pp = best_blk[5];
     (void)bp_solver_enable_kernel_timing(s, 0);
+    // iteration fusion: the epilogue passes (next iteration's plane
+    // collisions, the velocity/position update, the lambda pass's inputs) in
+    // the producing kernels, or every pass its own launch; best of two trials
+    float fused_ms[2] = {1e30f, 1e30f};
+    for (int trial = 0; trial < 2 && st == BP_OK; ++trial)
+        for (int f = 0; f < 2 && st == BP_OK; ++f) {
+            s->fuse = f == 1;
+            float ms = 0.f;
+            st = time_substeps(s, substeps_per_trial, dt, &ms);
+            if (ms < fused_ms[f]) fused_ms[f] = ms;
+        }
+    s->fuse = fused_ms[1] <= fused_ms[0];
     s->use_graph = g;
     s->drop_graph();
     if (st != BP_OK) return st;
@@ -810,8 +846,9 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
     if (report && report_len > 0)
         std::snprintf(report, (size_t)report_len,
                       "{\"fluid_lambda\": %d, \"fluid_position\": %d, \"omega_normal\": %d, "
-                      "\"vorticity_tension\": %d, \"xsph\": %d, \"particle_particle\": %d}",
-                      best_blk[0], best_blk[1], best_blk[2], best_blk[3], best_blk[4], best_blk[5]);
+                      "\"vorticity_tension\": %d, \"xsph\": %d, \"particle_particle\": %d, "
+                      "\"iteration_fusion\": %d}",
+                      best_blk[0], best_blk[1], best_blk[2], best_blk[3], best_blk[4], best_blk[5], s->fuse ? 1 : 0);
     return BP_OK;
 }
 

@@ -121,6 +121,7 @@ struct bp_solver {
 
     // hipGraph of two substeps (the ping-pong buffers return to their start)
     bool use_graph = false;
+    bool fuse = true;  // iteration fusion: epilogue passes inside the producing kernels
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     float graph_dt = 0.f;

@@ -245,11 +245,18 @@ bp_status bp_forward_checkpointed(bp_solver *s, int substeps, float subdt, int c
 bp_status bp_backward(bp_solver *s, const float *grad_x, const float *grad_v, float *grad_x0, float *grad_v0,
                       float *grad_theta);
 
+/* Iteration fusion (default on): the next iteration's plane collisions, the
+ * velocity/position update and the lambda pass's inputs are computed inside
+ * the kernels that produce their operands; off, every pass is its own launch.
+ * Results are bit-identical either way.  No counterpart in the reference. */
+bp_status bp_solver_set_fusion(bp_solver *s, int enable);
+
 /* Chooses the workgroup size of each neighbour pass among 64..512 by timing
- * `substeps_per_trial` substeps of size dt per candidate, then restores the
- * particle and rigid state it started from (results do not depend on the
- * workgroup sizes).  Writes a JSON object {pass name: size} into `report`
- * (may be NULL).  No counterpart in the reference. */
+ * `substeps_per_trial` substeps of size dt per candidate, and iteration fusion
+ * on or off (best of two trials each), then restores the particle and rigid
+ * state it started from (results do not depend on either choice).  Writes a
+ * JSON object {pass name: size, "iteration_fusion": 0|1} into `report` (may
+ * be NULL).  No counterpart in the reference. */
 bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, char *report, int report_len);
 
 /* Timing helper for benchmarks: runs `substeps` substeps of size dt and

@@ -108,10 +108,15 @@ def test_autotune_keeps_results():
     tuned = bp.BatchSolver(scs, prms)
     report = tuned.autotune(DT / 2, 2)
     print("autotune report", report)
-    assert set(report) == {"fluid_lambda", "fluid_position", "omega_normal", "vorticity_tension", "xsph",
-                           "particle_particle"}
-    assert all(v in (64, 128, 256, 512) for v in report.values())
+    passes = {"fluid_lambda", "fluid_position", "omega_normal", "vorticity_tension", "xsph", "particle_particle"}
+    assert set(report) == passes | {"iteration_fusion"}
+    assert all(report[k] in (64, 128, 256, 512) for k in passes)
+    assert report["iteration_fusion"] in (0, 1)
     # the tuner advances its own copy of the state and restores it
+    assert np.array_equal(ref.state_original_order()[0], tuned.state_original_order()[0])
+    # and its choices do not change results
+    ref.step(2, DT)
+    tuned.step(2, DT)
     assert np.array_equal(ref.state_original_order()[0], tuned.state_original_order()[0])
     ref.step(2, DT)
     tuned.step(2, DT)

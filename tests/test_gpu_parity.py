@@ -331,3 +331,27 @@ def test_edge_cases_empty_single_and_one_scene_batch():
     assert bs.ranges[1] == (scs[0].num_particles(), 0)
     bs.step(2, DT)
     assert np.all(np.isfinite(bs.state_original_order()[0]))
+
+
+@pytest.mark.parametrize("label", ["dam break 2k", "mixed phase small"])
+def test_iteration_fusion_is_bit_identical(label):
+    """bp_solver_set_fusion(0) launches every pass on its own; the fused
+    epilogues must give the same bits, and the autotuner's choice with them."""
+    if label == "dam break 2k":
+        sc, prm = scenes.dam_break(dims=(16, 16, 8), grid=(64, 64, 64)), scenes.dam_break_params()
+    else:
+        sc = scenes.mixed_phase(fluid_dims=(16, 16, 16), num_boxes=6, cloth_n=16, grid=(128, 128, 128))
+        prm = scenes.mixed_phase_params()
+    case(f"iteration fusion: {label}")
+    a, b = bp.Solver(sc, prm), bp.Solver(sc, prm)
+    b.set_fusion(False)
+    a.step(4, 4 * DT)
+    b.step(4, 4 * DT)
+    pa, va, ma = a.get_state()
+    pb, vb, mb = b.get_state()
+    assert np.array_equal(ma, mb) and np.array_equal(pa, pb) and np.array_equal(va, vb)
+    tune = a.autotune(DT, 1)
+    assert tune["iteration_fusion"] in (0, 1)
+    assert np.array_equal(a.get_state()[0], pa)  # the autotuner restores the state
+    a.close()
+    b.close()
