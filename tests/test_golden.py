@@ -134,3 +134,24 @@ def test_oracle_reproduces_mixed_vectors(have_oracle):
     q, _ = o.rigid_state()
     np.testing.assert_allclose(q, f["quats_fp32_gather"][-1], rtol=0, atol=1e-6)
     assert np.abs(f["traj_fp32_gather"] - f["traj_fp64"]).max() < 5e-5
+
+
+def test_oracle_thread_count_does_not_change_results(have_oracle):
+    """The oracle's per-particle loops run over OpenMP threads (each iteration
+    writes only its own slot; pair deliveries are atomic integer adds), so one
+    thread and several give the same bits, in both summation orders."""
+    sc, prm = mg.mixed_small()
+    arr = bp.SceneArrays(sc)
+    before = orc.set_threads()
+    try:
+        for mode in (orc.GATHER, orc.SCATTER):
+            out = []
+            for threads in (1, 4):
+                orc.set_threads(threads)
+                o = orc.Oracle(arr, prm, fp64=False, mode=mode)
+                o.step(3, 3 * mg.DT)
+                out.append(o.get_state())
+            for a, b in zip(*out):
+                assert np.array_equal(a, b), mode
+    finally:
+        orc.set_threads(before)
