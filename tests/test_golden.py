@@ -9,8 +9,10 @@ What keeps them honest:
     (grid.cuh:9-47: (x + 50000) / cellSize in FP32, truncation, & (D - 1)
     wrap, x-major address) and numpy's stable argsort, independently of the
     oracle's C++;
-  - the physics vectors obey invariants checked here (mass-weighted momentum
-    of one Δp projection, the FP32-vs-FP64 gap of the rollout);
+  - the physics vectors are checked here against what they must satisfy: the
+    first substep's keys and permutation recomputed by the numpy hash from
+    the stored inputs, positive densities, the FP32-vs-FP64 gap of the
+    rollout, and agreement of the GPU and reference summation orders;
   - the oracle must still reproduce all of them, so a change to the oracle or
     to the scene generator fails here first.
 """
