@@ -11,7 +11,10 @@
 //  * rigidbodyInitialPositions is indexed by original particle id for any
 //    insertion order (the reference needs rigid bodies first).
 //  * cellSize / gridDims make the hash grid explicit (the reference hard-codes
-//    cellSize = 2.3 r and 512^3 cells, particlesolver.cuh:150-151).
+//    512^3 cells, particlesolver.cuh:151).  cellSize left at zero follows the
+//    reference: particleRadius * 2.3f when the solver is built (:150).
+//  * addCloth takes a trailing dihedralBending flag (default off) that adds
+//    the reference's commented-out hinge generator (scene.cpp:108-123).
 #pragma once
 #include <hip/hip_vector_types.h>
 
@@ -33,7 +36,7 @@ struct Scene {
     // ---- builders (all return the [begin, end) particle range) -------------
     int2 addCloth(const float3 &startPosition, const float3 &stepX, const float3 &stepY, const int numJointX,
                   const int numJointY, const float massPerParticle, const float kStiffness, const float kBending,
-                  const bool isSelfCollidable = true) {
+                  const bool isSelfCollidable = true, const bool dihedralBending = false) {
         const float lx = len(stepX), ly = len(stepY);
         const float ld = len(make_float3(stepX.x + stepY.x, stepX.y + stepY.y, stepX.z + stepY.z));
         const int base = (int)positions.size();
@@ -66,6 +69,8 @@ struct Scene {
                     for (int j = -1; j <= 1; j++) distance(P(x - 1, y + j), P(x + 1, y + j), lx * 2, kBending);
                     for (int i = -1; i <= 1; i++) distance(P(x + i, y - 1), P(x + i, y + 1), ly * 2, kBending);
                 }
+                if (dihedralBending && x < numJointX - 1 && y < numJointY - 1)  // hinge across each quad
+                    bendingConstraints.push_back((x + y) % 2 ? make_int4(p3, p2, p1, p4) : make_int4(p1, p4, p3, p2));
             }
         return make_int2(base, (int)positions.size());
     }
@@ -154,9 +159,19 @@ struct Scene {
     size_t numParticles() { return positions.size(); }
 
     float particleRadius = 0.05f;
-    float fluidKernelRadius = 2.3f * particleRadius;
-    float3 cellSize = make_float3(0.05f * 2.3f, 0.05f * 2.3f, 0.05f * 2.3f);
+    // scene.h:501: the product is taken in double, then stored as float
+    float fluidKernelRadius = 2.3 * particleRadius;
+    // Hash-cell edge.  Left at zero it follows the radius when the solver is
+    // built, particleRadius * 2.3f in float as particlesolver.cuh:150; any
+    // positive value set by the caller is used as it is.
+    float3 cellSize = make_float3(0.f, 0.f, 0.f);
     int3 gridDims = make_int3(512, 512, 512);
+
+    float3 effectiveCellSize() const {
+        if (cellSize.x > 0.f && cellSize.y > 0.f && cellSize.z > 0.f) return cellSize;
+        const float c = particleRadius * 2.3f;
+        return make_float3(c, c, c);
+    }
 
     std::vector<float3> positions;
     std::vector<float> masses;

@@ -169,7 +169,29 @@ __global__ __launch_bounds__(kBlock) void k_distance_apply(float4 *__restrict__ 
     xs[slot] = f4(xyz(a) + dx, a.w);
 }
 
+// Double-precision 3-vectors for the dihedral hinge below.
+struct d3 {
+    double x, y, z;
+};
+__device__ __forceinline__ d3 operator+(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ d3 operator-(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ d3 operator-(d3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ d3 operator*(d3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ d3 operator/(d3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ double ddot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ d3 dcross(d3 a, d3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ d3 dwide(float4 a) { return {(double)a.x, (double)a.y, (double)a.z}; }
+__device__ __forceinline__ float4 dnarrow(d3 a) { return make_float4((float)a.x, (float)a.y, (float)a.z, 1.0f); }
+
 // bendingConstraints (bending.cuh:3-76): the four corrections of each dihedral
+// hinge, evaluated in FP64.  Near a flat hinge (d -> -1) both
+// sqrt(1 - d^2)(acos d - pi) and sum w|Q|^2 vanish, and in FP32 their ratio
+// is set by rounding (FP32 and FP64 evaluations of a rippled flag differ by
+// 2e-3 per substep), so the reference's FP32 arithmetic does not define the
+// result there; the host oracle evaluates the hinge in FP64 as well.  At most
+// a few thousand hinges per scene: the FP64 rate is not a concern.
 __global__ __launch_bounds__(kBlock) void k_bending_compute(const float4 *__restrict__ xs,
                                                             const float4 *__restrict__ x,
                                                             const int4 *__restrict__ bends,
@@ -180,33 +202,33 @@ __global__ __launch_bounds__(kBlock) void k_bending_compute(const float4 *__rest
     const int4 ids = bends[c];
     const int id1 = map_to_new[ids.x], id2 = map_to_new[ids.y], id3 = map_to_new[ids.z], id4 = map_to_new[ids.w];
     for (int a = 0; a < 4; ++a) bbuf[4 * c + a] = make_float4(0.f, 0.f, 0.f, 0.f);
-    const f3 P1 = xyz(xs[id1]);
-    const f3 P2 = xyz(xs[id2]) - P1, P3 = xyz(xs[id3]) - P1, P4 = xyz(xs[id4]) - P1;
-    const f3 c23 = cross(P2, P3), c24 = cross(P2, P4);
-    const float l23 = length(c23);
-    if (l23 <= 1e-5f) return;
-    const float l24 = length(c24);
-    if (l24 <= 1e-5f) return;
-    const f3 N1 = c23 * (1.0f / l23);
-    const f3 N2 = c24 * (1.0f / l24);
-    float d = dot(N1, N2);
-    d = fminf(fmaxf(d, -1.0f), 1.0f);
-    const f3 c2n1 = cross(P2, N1), c2n2 = cross(P2, N2);
-    const f3 c3n1 = cross(P3, N1), c3n2 = cross(P3, N2);
-    const f3 c4n1 = cross(P4, N1), c4n2 = cross(P4, N2);
-    const f3 Q3 = (c2n2 - c2n1 * d) / l23;
-    const f3 Q4 = (c2n1 - c2n2 * d) / l24;
-    const f3 Q2 = (-(c3n2 - c3n1 * d)) / l23 - (c4n1 - c4n2 * d) / l24;
-    const f3 Q1 = -Q2 - Q3 - Q4;
-    const float w1 = x[id1].w, w2 = x[id2].w, w3 = x[id3].w, w4 = x[id4].w;
-    const float den = w1 * length2(Q1) + w2 * length2(Q2) + w3 * length2(Q3) + w4 * length2(Q4);
-    const float num = sqrtf(1.0f - d * d) * (acosf(d) - 3.141592f);
-    if (den <= 1e-5f) return;
-    const float sc = num / den;
-    bbuf[4 * c + 0] = f4(-w1 * Q1 * sc * kb, 1.0f);
-    bbuf[4 * c + 1] = f4(-w2 * Q2 * sc * kb, 1.0f);
-    bbuf[4 * c + 2] = f4(-w3 * Q3 * sc * kb, 1.0f);
-    bbuf[4 * c + 3] = f4(-w4 * Q4 * sc * kb, 1.0f);
+    const d3 P1 = dwide(xs[id1]);
+    const d3 P2 = dwide(xs[id2]) - P1, P3 = dwide(xs[id3]) - P1, P4 = dwide(xs[id4]) - P1;
+    const d3 c23 = dcross(P2, P3), c24 = dcross(P2, P4);
+    const double l23 = sqrt(ddot(c23, c23));
+    if (l23 <= (double)1e-5f) return;
+    const double l24 = sqrt(ddot(c24, c24));
+    if (l24 <= (double)1e-5f) return;
+    const d3 N1 = c23 * (1.0 / l23);
+    const d3 N2 = c24 * (1.0 / l24);
+    double d = ddot(N1, N2);
+    d = fmin(fmax(d, -1.0), 1.0);
+    const d3 c2n1 = dcross(P2, N1), c2n2 = dcross(P2, N2);
+    const d3 c3n1 = dcross(P3, N1), c3n2 = dcross(P3, N2);
+    const d3 c4n1 = dcross(P4, N1), c4n2 = dcross(P4, N2);
+    const d3 Q3 = (c2n2 - c2n1 * d) / l23;
+    const d3 Q4 = (c2n1 - c2n2 * d) / l24;
+    const d3 Q2 = (-(c3n2 - c3n1 * d)) / l23 - (c4n1 - c4n2 * d) / l24;
+    const d3 Q1 = -Q2 - Q3 - Q4;
+    const double w1 = x[id1].w, w2 = x[id2].w, w3 = x[id3].w, w4 = x[id4].w;
+    const double den = w1 * ddot(Q1, Q1) + w2 * ddot(Q2, Q2) + w3 * ddot(Q3, Q3) + w4 * ddot(Q4, Q4);
+    const double num = sqrt(1.0 - d * d) * (acos(d) - (double)3.141592f);
+    if (den <= (double)1e-5f) return;
+    const double sc = num / den, kbd = (double)kb;
+    bbuf[4 * c + 0] = dnarrow(Q1 * (-w1) * sc * kbd);
+    bbuf[4 * c + 1] = dnarrow(Q2 * (-w2) * sc * kbd);
+    bbuf[4 * c + 2] = dnarrow(Q3 * (-w3) * sc * kbd);
+    bbuf[4 * c + 3] = dnarrow(Q4 * (-w4) * sc * kbd);
 }
 // sum of per-constraint corrections listed in a CSR, applied in place
 __global__ __launch_bounds__(kBlock) void k_gather_apply(float4 *__restrict__ xs, const int *__restrict__ verts,

@@ -230,8 +230,11 @@ class Scene:
         return s, e
 
     def add_cloth(self, start_position, step_x, step_y, num_joint_x, num_joint_y, mass_per_particle,
-                  k_stiffness, k_bending, is_self_collidable=True):
-        """scene.cpp:4-128 with storage-order indices and global offsets."""
+                  k_stiffness, k_bending, is_self_collidable=True, dihedral_bending=False):
+        """scene.cpp:4-128 with storage-order indices and global offsets.
+        dihedral_bending adds one hinge per quad, (p3, p2, p1, p4) or
+        (p1, p4, p3, p2) by the parity of x + y, as the reference's
+        commented-out generator (scene.cpp:108-123)."""
         sx = np.asarray(step_x, np.float32)
         sy = np.asarray(step_y, np.float32)
         nx, ny = int(num_joint_x), int(num_joint_y)
@@ -277,6 +280,8 @@ class Scene:
                     for i in (-1, 0, 1):
                         self.distance_pairs.append((P(x + i, y - 1), P(x + i, y + 1)))
                         self.distance_params.append((ly * 2, k_bending))
+                if dihedral_bending and x < nx - 1 and y < ny - 1:
+                    self.bending_constraints.append((p3, p2, p1, p4) if (x + y) % 2 else (p1, p4, p3, p2))
         return base, end
 
     def make_immovable(self, particle_id: int):

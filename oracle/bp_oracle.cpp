@@ -737,41 +737,50 @@ template <class R> struct Solver : SolverBase {
         }
         for (int i = 0; i < N; ++i) xs[i] += dx[i];
     }
-    // bendingConstraints (bending.cuh:3-76) + Δx apply
+    // bendingConstraints (bending.cuh:3-76) + Δx apply.  The hinge is
+    // evaluated in the type W (double for the FP32 instance): near a flat
+    // hinge (d -> -1) sqrt(1 - d^2)(acos d - pi) and sum w|Q|^2 both vanish
+    // and in FP32 their ratio is dominated by rounding (FP32 against FP64
+    // per-substep differences of 2e-3 on a rippled flag), so the reference's
+    // FP32 arithmetic does not define the result there.  The GPU kernel
+    // computes the same hinge in FP64; the corrections are then rounded to R.
     void bending() {
+        using W = std::conditional_t<std::is_same<R, float>::value, double, R>;
+        auto wide = [](V3<R> a) { return V3<W>{W(a.x), W(a.y), W(a.z)}; };
         for (int i = 0; i < N; ++i) dx[i] = V3<R>{0, 0, 0};
         const int nb = (int)bends.size() / 4;
-        const R kb = R(P.bend_stiffness);
+        const W kb = W(P.bend_stiffness);
         for (int c = 0; c < nb; ++c) {
             int id1 = to_new[bends[4 * c]], id2 = to_new[bends[4 * c + 1]];
             int id3 = to_new[bends[4 * c + 2]], id4 = to_new[bends[4 * c + 3]];
-            V3<R> P1 = xs[id1];
-            V3<R> P2 = xs[id2] - P1, P3 = xs[id3] - P1, P4 = xs[id4] - P1;
-            V3<R> c23 = cross(P2, P3), c24 = cross(P2, P4);
-            R l23 = length(c23);
-            if (l23 <= R(1e-5f)) continue;
-            R l24 = length(c24);
-            if (l24 <= R(1e-5f)) continue;
-            V3<R> N1 = c23 * (R(1) / l23);
-            V3<R> N2 = c24 * (R(1) / l24);
-            R d = dot(N1, N2);
-            d = fmin_(fmax_(d, R(-1)), R(1));
-            V3<R> c2n1 = cross(P2, N1), c2n2 = cross(P2, N2);
-            V3<R> c3n1 = cross(P3, N1), c3n2 = cross(P3, N2);
-            V3<R> c4n1 = cross(P4, N1), c4n2 = cross(P4, N2);
-            V3<R> Q3 = (c2n2 - c2n1 * d) / l23;
-            V3<R> Q4v = (c2n1 - c2n2 * d) / l24;
-            V3<R> Q2 = (-(c3n2 - c3n1 * d)) / l23 - (c4n1 - c4n2 * d) / l24;
-            V3<R> Q1 = -Q2 - Q3 - Q4v;
-            R w1 = invw[id1], w2 = invw[id2], w3 = invw[id3], w4 = invw[id4];
-            R den = w1 * length2(Q1) + w2 * length2(Q2) + w3 * length2(Q3) + w4 * length2(Q4v);
-            R num = sqrt_(R(1) - d * d) * (acos_(d) - R(3.141592f));
-            if (den <= R(1e-5f)) continue;
-            R sc = num / den;
-            dx[id1] += -w1 * Q1 * sc * kb;
-            dx[id2] += -w2 * Q2 * sc * kb;
-            dx[id3] += -w3 * Q3 * sc * kb;
-            dx[id4] += -w4 * Q4v * sc * kb;
+            V3<W> P1 = wide(xs[id1]);
+            V3<W> P2 = wide(xs[id2]) - P1, P3 = wide(xs[id3]) - P1, P4 = wide(xs[id4]) - P1;
+            V3<W> c23 = cross(P2, P3), c24 = cross(P2, P4);
+            W l23 = length(c23);
+            if (l23 <= W(1e-5f)) continue;
+            W l24 = length(c24);
+            if (l24 <= W(1e-5f)) continue;
+            V3<W> N1 = c23 * (W(1) / l23);
+            V3<W> N2 = c24 * (W(1) / l24);
+            W d = dot(N1, N2);
+            d = fmin_(fmax_(d, W(-1)), W(1));
+            V3<W> c2n1 = cross(P2, N1), c2n2 = cross(P2, N2);
+            V3<W> c3n1 = cross(P3, N1), c3n2 = cross(P3, N2);
+            V3<W> c4n1 = cross(P4, N1), c4n2 = cross(P4, N2);
+            V3<W> Q3 = (c2n2 - c2n1 * d) / l23;
+            V3<W> Q4v = (c2n1 - c2n2 * d) / l24;
+            V3<W> Q2 = (-(c3n2 - c3n1 * d)) / l23 - (c4n1 - c4n2 * d) / l24;
+            V3<W> Q1 = -Q2 - Q3 - Q4v;
+            W w1 = W(invw[id1]), w2 = W(invw[id2]), w3 = W(invw[id3]), w4 = W(invw[id4]);
+            W den = w1 * length2(Q1) + w2 * length2(Q2) + w3 * length2(Q3) + w4 * length2(Q4v);
+            W num = sqrt_(W(1) - d * d) * (acos_(d) - W(3.141592f));
+            if (den <= W(1e-5f)) continue;
+            W sc = num / den;
+            auto narrow = [](V3<W> a) { return V3<R>{R(a.x), R(a.y), R(a.z)}; };
+            dx[id1] += narrow(-w1 * Q1 * sc * kb);
+            dx[id2] += narrow(-w2 * Q2 * sc * kb);
+            dx[id3] += narrow(-w3 * Q3 * sc * kb);
+            dx[id4] += narrow(-w4 * Q4v * sc * kb);
         }
         for (int i = 0; i < N; ++i) xs[i] += dx[i];
     }

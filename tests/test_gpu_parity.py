@@ -18,7 +18,12 @@ pytestmark = pytest.mark.gpu
 
 # per-substep position tolerances (units: scene length; h = 0.1)
 POS_TOL_GATHER = 2e-5    # same summation order; rcp/sqrt/sin/cos ulps and FMA differ
-POS_TOL_SCATTER = 2e-3   # reference summation order; borderline branches may flip
+# Reference summation order (the ownership rule of fluid.cuh:199-227 reproduced
+# exactly, so the pair sets are the reference's): the worst case measured over
+# every per-substep case is 3.3e-6 (mixed phase, full size) and the worst
+# 99.9th percentile 1.5e-6 (dihedral bending), both in units of length (h = 0.1).
+POS_TOL_SCATTER = 1e-5
+POS_Q999_SCATTER = POS_TOL_SCATTER / 4
 DT = 1.0 / 120.0
 
 
@@ -37,8 +42,8 @@ def set_same_state(gpu, ora):
 def per_substep_parity(sc, prm, substeps, label, tol_g=POS_TOL_GATHER, tol_s=POS_TOL_SCATTER, warm=3):
     """GPU vs oracle after one substep from the same state, `substeps` times.
     Against the reference-order oracle (same pair set, reference summation
-    order) the 99.9th percentile must stay below tol_s / 20 and the maximum
-    below tol_s."""
+    order) the 99.9th percentile must stay below POS_Q999_SCATTER (scaled
+    with tol_s) and the maximum below tol_s."""
     arr = bp.SceneArrays(sc)
     og = orc.Oracle(arr, prm, fp64=False, mode=orc.GATHER)
     os_ = orc.Oracle(arr, prm, fp64=False, mode=orc.SCATTER)
@@ -72,7 +77,7 @@ def per_substep_parity(sc, prm, substeps, label, tol_g=POS_TOL_GATHER, tol_s=POS
         worst_g, worst_s = max(worst_g, float(err_g.max())), max(worst_s, float(err_s.max()))
         q_s = max(q_s, float(np.quantile(err_s, 0.999)))
         assert err_g.max() <= tol_g, f"substep {k}: |x_gpu - x_oracle(gather)| = {err_g.max()}"
-        assert np.quantile(err_s, 0.999) <= tol_s / 20 and err_s.max() <= tol_s, \
+        assert np.quantile(err_s, 0.999) <= POS_Q999_SCATTER * tol_s / POS_TOL_SCATTER and err_s.max() <= tol_s, \
             f"substep {k}: |x_gpu - x_oracle(scatter)| = {err_s.max()}"
         # v = (x* - x)/dt, then XSPH (gain c * sum W, about 80 at c = 0.01) amplifies rounding
         assert float(np.abs(gv - rv).max()) <= tol_g / DT * 2 + 2e-4 * float(np.abs(rv).max())
@@ -355,3 +360,76 @@ def test_iteration_fusion_is_bit_identical(label):
     assert np.array_equal(a.get_state()[0], pa)  # the autotuner restores the state
     a.close()
     b.close()
+
+
+def test_substep_parity_dihedral_bending():
+    """bendingConstraints (bending.cuh:3-76) on the GPU against the oracle.
+    A 24 x 24 flag pinned along one edge with one hinge per quad, generated
+    as the reference's commented-out code does (scene.cpp:108-123, hinges
+    (p3, p2, p1, p4) and (p1, p4, p3, p2)), started with a ripple so the
+    hinges are bent from the first substep, in wind; beside it four free
+    particles form a hinge at each of three dihedral angles, with unequal
+    masses."""
+    import math
+    sc = scenes.Scene(particle_radius=0.025, fluid_kernel_radius=0.1, cell_size=0.1, grid_dims=(64, 64, 64))
+    for k, ang in enumerate((1.2, 2.0, 2.8)):
+        base = np.array([1.0 + 0.5 * k, 0.6, 1.0])
+        pts = [base, base + (0.1, 0, 0), base + (0.05, 0.1, 0),
+               base + (0.06, 0.1 * math.cos(ang), 0.1 * math.sin(ang))]
+        first = sc.num_particles()
+        for i, p in enumerate(pts):
+            sc.add_granulars_block((1, 1, 1), p, (0.05, 0.05, 0.05), 1.0 + i)
+        sc.bending_constraints.append((first, first + 1, first + 2, first + 3))
+    n = 24
+    cs, _ = sc.add_cloth((1.0, 1.5, 2.0), (0.05, 0, 0), (0, 0.05, 0), n, n, 0.5, 0.2, 0.1, True,
+                         dihedral_bending=True)
+    ripple = np.zeros((sc.num_particles(), 3), np.float32)
+    ripple[cs:, 2] = 0.01 * np.sin(np.arange(n * n) * 0.7)
+    sc._pos = [sc.positions + ripple]
+    for y in range(n):
+        sc.make_immovable(cs + y)
+    sc.add_box_planes((0, 0, 0), (6.4, 6.4, 6.4))
+    assert len(sc.bending_constraints) == 3 + (n - 1) ** 2
+    prm = bp.default_params(iterations=4, wind=(3.0, 0.0, 1.0), friction_static=0.3, friction_dynamic=0.3)
+    per_substep_parity(sc, prm, 4, "dihedral bending: flag 24x24 + three hinges")
+
+
+def test_substep_parity_mixed_phase_full_283k():
+    """The mixed-phase workload at its full size (282,624 particles: 64
+    shape-matched boxes, a 64^3 fluid block, a pinned 128 x 128 flag in wind)
+    against the oracle, two substeps after one warm-up substep."""
+    sc = scenes.mixed_phase()
+    assert sc.num_particles() == 282_624
+    per_substep_parity(sc, scenes.mixed_phase_params(), 2, "mixed phase 283k", warm=1)
+
+
+def test_nan_guard_names_the_particle():
+    """bp_params.nan_guard (the reference's opt-in CHECK_NAN printIfNan,
+    kernel/cubrasstower.cuh:15): after a step whose result holds a
+    non-finite position, bp_solver_step fails with BP_ERR_NOT_FINITE and
+    names the first such slot and its original id (a device-side scan, four
+    bytes read back).  Finite steps pass; with the guard off the same state
+    steps without an error.  The planted particle is isolated (granular, no
+    neighbour within 2r or h), so the NaN stays in it: every cell index the
+    kernels form is masked to the grid."""
+    sc = scenes.Scene(particle_radius=0.025, fluid_kernel_radius=0.1, cell_size=0.1, grid_dims=(32, 32, 32))
+    sc.add_granulars_block((2, 2, 2), (1.0, 1.0, 1.0), (0.3, 0.3, 0.3), 1.0)
+    sc.add_box_planes((0, 0, 0), (3.2, 3.2, 3.2))
+    case("nan guard")
+    g = bp.Solver(sc, bp.default_params(nan_guard=1))
+    g.step(1, DT)
+    p, v, m = g.get_state()
+    k = 5
+    p[k, 1] = np.nan
+    g.set_state(p, v, m)
+    with pytest.raises(bp.BrassPanError) as e:
+        g.step(1, DT)
+    assert e.value.status == 4 and f"original id {m[k]})" in str(e.value), str(e.value)
+    pos, _, mp = g.get_state()
+    bad = np.where(~np.isfinite(pos).all(1))[0]
+    assert len(bad) == 1 and mp[bad[0]] == m[k]
+    g.close()
+    h = bp.Solver(sc, bp.default_params(nan_guard=0))
+    h.set_state(p, v, m)
+    h.step(1, DT)
+    h.close()
