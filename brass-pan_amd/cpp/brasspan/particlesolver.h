@@ -31,7 +31,8 @@ struct SceneDesc {
         d.group_ids = s.groupIds.empty() ? nullptr : s.groupIds.data();
         d.particle_radius = s.particleRadius;
         d.kernel_radius = s.fluidKernelRadius;
-        d.cell_size[0] = s.cellSize.x; d.cell_size[1] = s.cellSize.y; d.cell_size[2] = s.cellSize.z;
+        const float3 cs = s.effectiveCellSize();
+        d.cell_size[0] = cs.x; d.cell_size[1] = cs.y; d.cell_size[2] = cs.z;
         d.grid_dims[0] = s.gridDims.x; d.grid_dims[1] = s.gridDims.y; d.grid_dims[2] = s.gridDims.z;
         d.num_planes = (int)s.planes.size();
         d.plane_origins = planeO.data();

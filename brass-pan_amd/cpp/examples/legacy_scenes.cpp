@@ -2,8 +2,9 @@
 // scenes shaped like main.cu's initRigidBodiesScene, initFluidScene and
 // initRopesScene (main.cu:140-255), built as OldSceneFormat and handed to
 // ParticleSolver(const std::shared_ptr<OldSceneFormat> &).  Usage:
-// legacy_scenes [frames] [frame file prefix]; with a prefix, each scene's last
-// frame is written to <prefix><name>.bpfr.
+// legacy_scenes [frames] [frame file prefix] [dump]; with a prefix, each
+// scene's last frame is written to <prefix><name>.bpfr, and with "dump" also
+// the assembled scene to <prefix><name>.bpsc (scenefile.h).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -12,6 +13,7 @@
 #include <vector>
 
 #include "brasspan/particlesolver.h"
+#include "brasspan/scenefile.h"
 
 static std::shared_ptr<OldSceneFormat> base(float radius, size_t maxParticles) {
     auto s = std::make_shared<OldSceneFormat>();
@@ -29,6 +31,7 @@ static std::shared_ptr<OldSceneFormat> base(float radius, size_t maxParticles) {
 int main(int argc, char **argv) {
     const int frames = argc > 1 ? std::atoi(argv[1]) : 20;
     const std::string prefix = argc > 2 ? argv[2] : "";
+    const bool dump = argc > 3 && std::string(argv[3]) == "dump";
     const float r = 0.05f;
     const glm::vec3 d(2 * r);
     std::vector<std::pair<std::string, std::shared_ptr<OldSceneFormat>>> scenes;
@@ -59,6 +62,11 @@ int main(int argc, char **argv) {
     }
     try {
         for (auto &[name, s] : scenes) {
+            if (dump && !prefix.empty()) {  // the assembly on a copy: it advances the scene's counters
+                OldSceneFormat copy = *s;
+                const bool fluid = s->fluidRestDensity > 0.0f || !s->fluids.empty();
+                writeSceneFile(prefix + name + ".bpsc", assembleOldScene(copy), fluid ? s->fluidRestDensity : -1.0f);
+            }
             bp_params p;
             bp_default_params(&p);
             ParticleSolver solver(s, &p);

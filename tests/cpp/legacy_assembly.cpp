@@ -1,6 +1,9 @@
 // Host-only check of the legacy scene assembly (brass-pan_amd/cpp/brasspan/
 // oldscene.h) against the rules of the reference's legacy constructor
 // (particlesolver.cuh:218-342, 395-710).  Prints "legacy assembly ok".
+// Also prints the bits of the hash-cell edge and kernel radius a Scene hands
+// the solver (default radius, and radius 0.025 set after construction), and
+// with a path argument writes the assembled scene there as a scene file.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -8,6 +11,9 @@
 #include <string>
 
 #include "brasspan/oldscene.h"
+#include "brasspan/scenefile.h"
+
+#include <cstring>
 
 static int failures = 0;
 #define CHECK(c)                                                          \
@@ -40,7 +46,23 @@ static std::shared_ptr<OldSceneFormat> sample(float r) {
     return o;
 }
 
-int main() {
+static unsigned bits(float f) {
+    unsigned u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+
+int main(int argc, char **argv) {
+    {  // Scene defaults: h = 2.3 * radius in double at construction, cell = radius * 2.3f at solver build
+        Scene a;
+        Scene b;
+        b.particleRadius = 0.025f;
+        Scene c;
+        c.cellSize = make_float3(0.1f, 0.2f, 0.3f);
+        std::printf("scene defaults %08x %08x %08x %08x %08x %08x\n", bits(a.effectiveCellSize().x),
+                    bits(a.fluidKernelRadius), bits(b.effectiveCellSize().x), bits(b.fluidKernelRadius),
+                    bits(c.effectiveCellSize().y), bits(b.effectiveCellSize().z));
+    }
     const float r = 0.05f;
     auto o = sample(r);
     const Scene s = assembleOldScene(*o);
@@ -121,6 +143,7 @@ int main() {
     q = sample(r);
     for (glm::vec3 &p : q->rigidBodies[0]->positions_CM_Origin) p.x += 0.01f;
     CHECK(throws(q, "expected Center of Mass at the origin"));
+    if (argc > 1) writeSceneFile(argv[1], s, o->fluidRestDensity);
     if (failures) return 1;
     std::printf("legacy assembly ok\n");
     return 0;

@@ -1,7 +1,8 @@
 """GPU: the legacy scene format through ParticleSolver(OldSceneFormat)
 (particlesolver.cuh:218-342): brass-pan_amd/bin/legacy_scenes builds scenes
 shaped like main.cu's legacy scenes (rigid-box stack, fluid block, ropes and
-cloths), steps them and writes each last frame."""
+cloths), steps them and writes each last frame; with "dump" it also writes
+each assembled scene, which the oracle then runs from the same input."""
 import os
 import subprocess
 import tempfile
@@ -9,19 +10,23 @@ import tempfile
 import numpy as np
 import pytest
 
+import brasspan as bp
 import frames
+import oracle_py as orc
+import scenefile
+from test_gpu_parity import POS_TOL_GATHER, per_substep_parity
 
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "brass-pan_amd", "bin", "legacy_scenes")
 
 
 def test_legacy_scenes_run():
-    exe = os.path.join(ROOT, "brass-pan_amd", "bin", "legacy_scenes")
     print("[gpu case] legacy scenes, 60 frames", flush=True)
     with tempfile.TemporaryDirectory() as d:
         prefix = os.path.join(d, "f_")
-        r = subprocess.run([exe, "60", prefix], capture_output=True, text=True, timeout=240)
+        r = subprocess.run([EXE, "60", prefix], capture_output=True, text=True, timeout=240)
         assert r.returncode == 0, r.stderr
         print(r.stdout)
         rows = {ln.split()[0]: dict(zip(ln.split()[1::2], ln.split()[2::2])) for ln in r.stdout.splitlines()}
@@ -47,3 +52,43 @@ def test_legacy_scenes_run():
         m = fr["rigid"]["matrices"][b].astype(np.float64)
         assert np.abs(m[:3, :3] @ m[:3, :3].T - np.eye(3)).max() < 1e-5
         assert np.abs(m[:3, 3] - pos[s:e].mean(0)).max() < 1e-5
+
+
+@pytest.fixture(scope="module")
+def legacy_dump():
+    d = tempfile.mkdtemp()
+    prefix = os.path.join(d, "f_")
+    print("[gpu case] legacy scenes, one frame with scene dumps", flush=True)
+    r = subprocess.run([EXE, "1", prefix, "dump"], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr
+    return prefix
+
+
+def _legacy_params(rho):
+    # ParticleSolver(OldSceneFormat) with the reference literals and the scene's rest density
+    return bp.default_params(**({} if rho is None else {"fluid_rest_density": rho}))
+
+
+@pytest.mark.parametrize("name", ["rigid", "fluid", "ropes"])
+def test_legacy_scene_frame_matches_oracle(legacy_dump, name):
+    """The C++ program's frame after update(2, 1/60) from the assembled
+    legacy scene, against the oracle run on the scene file the program wrote
+    (the same assembly, read back in Python)."""
+    sc, rho = scenefile.read_scene_file(legacy_dump + name + ".bpsc")
+    fr = frames.read_frame(legacy_dump + name + ".bpfr")
+    o = orc.Oracle(bp.SceneArrays(sc), _legacy_params(rho), fp64=False, mode=orc.GATHER)
+    o.step(2, 1.0 / 60.0)
+    op, _ = o.state_original_order()
+    assert np.array_equal(fr["groups"], sc.group_ids)
+    err = float(np.abs(fr["positions"].astype(np.float64) - op).max())
+    print(f"legacy {name}: n={sc.num_particles()} frame vs oracle after 2 substeps: {err:.3e}")
+    assert err <= 4 * POS_TOL_GATHER
+
+
+@pytest.mark.parametrize("name", ["rigid", "ropes"])
+def test_legacy_scene_per_substep_parity(legacy_dump, name):
+    """Per-substep GPU vs oracle on the assembled legacy scene (rigid stack:
+    shape matching on a plane; ropes: rope and cloth distance constraints,
+    pins and wind faces)."""
+    sc, rho = scenefile.read_scene_file(legacy_dump + name + ".bpsc")
+    per_substep_parity(sc, _legacy_params(rho), 3, f"legacy {name}")
