@@ -340,6 +340,8 @@ class Solver:
             out = np.empty(self.n, np.float32)
         elif which in (2, 3):
             out = np.empty(self.n, np.int32)
+        elif which == 6:  # largest |term| the last backward delivered in fixed point
+            out = np.zeros(1, np.float32)
         else:
             out = np.empty((self.n, 3), np.float32)
         _check(self.lib.bp_debug_fetch(self.h, int(which), out.ctypes.data_as(C.c_void_p)))

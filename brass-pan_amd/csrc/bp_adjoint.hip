@@ -27,14 +27,27 @@ namespace {
 constexpr double kAdjScale = 1099511627776.0;  // 2^40
 constexpr int kCorrStride = 6;                  // delivered components per slot
 
-__device__ __forceinline__ unsigned long long afix(float v) {
+// Range of a delivered term.  |v| <= 2^17 keeps one term below 2^57 in
+// 2^-40 fixed point, so a slot's sum of up to 64 terms of one sign cannot
+// wrap the int64.  A larger or non-finite term is clamped and counted in the
+// guard words after the last slot (corr[kCorrStride * n]: count,
+// [kCorrStride * n + 1]: the largest |term| delivered, as float bits);
+// bp_backward fails with BP_ERR_NOT_FINITE when the count is not zero.
+constexpr float kAdjMaxTerm = 131072.0f;  // 2^17
+__device__ __forceinline__ unsigned long long afix(float v, unsigned long long *guard) {
+    const float a = fabsf(v);
+    if (!(a <= kAdjMaxTerm)) {
+        atomicAdd(&guard[0], 1ull);
+        v = (v != v) ? 0.0f : copysignf(kAdjMaxTerm, v);
+    }
+    atomicMax(&guard[1], (unsigned long long)__float_as_uint(a == a ? a : INFINITY));
     return (unsigned long long)(long long)rint((double)v * kAdjScale);
 }
 __device__ __forceinline__ float aunfix(unsigned long long u) { return (float)((double)(long long)u / kAdjScale); }
-__device__ __forceinline__ void deliver3(unsigned long long *corr, int j, int c0, f3 v) {
-    atomicAdd(&corr[kCorrStride * j + c0 + 0], afix(v.x));
-    atomicAdd(&corr[kCorrStride * j + c0 + 1], afix(v.y));
-    atomicAdd(&corr[kCorrStride * j + c0 + 2], afix(v.z));
+__device__ __forceinline__ void deliver3(unsigned long long *corr, int j, int c0, f3 v, unsigned long long *guard) {
+    atomicAdd(&corr[kCorrStride * j + c0 + 0], afix(v.x, guard));
+    atomicAdd(&corr[kCorrStride * j + c0 + 1], afix(v.y, guard));
+    atomicAdd(&corr[kCorrStride * j + c0 + 2], afix(v.z, guard));
 }
 // adds and clears delivered components [c0, c0 + 3) of slot k
 __device__ __forceinline__ f3 take3(unsigned long long *corr, int k, int c0) {
@@ -262,8 +275,8 @@ __global__ __launch_bounds__(kMaxBlock) void ka_xsph(const float4 *__restrict__ 
                     gv += (c * W) * yj;
                     gx += (c * dot(yj, -dv) * Wp2) * diff;
                 } else {  // j never sees this pair: deliver i's terms for j
-                    deliver3(corr, j, 0, (c * W) * yi);
-                    deliver3(corr, j, 3, -sa * diff);
+                    deliver3(corr, j, 0, (c * W) * yi, corr + kCorrStride * n);
+                    deliver3(corr, j, 3, -sa * diff, corr + kCorrStride * n);
                 }
             },
             AllowNotSelf{i});
@@ -365,8 +378,8 @@ __global__ __launch_bounds__(kMaxBlock) void ka_position(const float4 *__restric
                 gx += gdiff;
                 gl += gls;
                 if (j > k && !partner_sees_k) {
-                    deliver3(corr, j, 0, -gdiff);
-                    atomicAdd(&corr[kCorrStride * j + 3], afix(gls));
+                    deliver3(corr, j, 0, -gdiff, corr + kCorrStride * n);
+                    atomicAdd(&corr[kCorrStride * j + 3], afix(gls, corr + kCorrStride * n));
                 }
             },
             [&](int base) { return kf ? ~chunk_bit(k, base) : chunk_below(k, base); }, rec);
@@ -446,7 +459,7 @@ __global__ __launch_bounds__(kMaxBlock) void ka_lambda(const float4 *__restrict_
                     const float sj = (phase_of(b) < 0) ? 1.0f : sds;
                     const f3 t = (rk * sj * P.r.y * Wp2) * diff;
                     gx += t;
-                    if (!partner_sees_k) deliver3(corr, j, 0, -t);
+                    if (!partner_sees_k) deliver3(corr, j, 0, -t, corr + kCorrStride * n);
                 } else if (partner_sees_k) {  // pair of rho_j
                     gx += (P.r.x * sk * mk * Wp2) * diff;
                 }

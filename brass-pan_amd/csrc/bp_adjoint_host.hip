@@ -54,7 +54,8 @@ struct Adjoint {
     float4 *prec = nullptr;  // projection adjoints' record: (lambda | density adjoint, m, cell, -)
     uint32_t *ccell = nullptr;
     int *mtn = nullptr, *contacts = nullptr;
-    unsigned long long *corr = nullptr;
+    unsigned long long *corr = nullptr;  // 6 delivered components per slot, then 2 guard words
+    float max_delivered = 0.f;            // largest |delivered term| of the last backward, seed units
     float4 *cb = nullptr;
     float2 *wb = nullptr;
     float4 *fbar = nullptr, *facc = nullptr;
@@ -84,6 +85,8 @@ struct Adjoint {
 void adjoint_invalidate(bp_solver *s) {
     if (s && s->adj) s->adj->recorded = false;
 }
+
+float adjoint_max_delivered(const bp_solver *s) { return (s && s->adj) ? s->adj->max_delivered : 0.0f; }
 
 void adjoint_release(bp_solver *s) {
     if (!s || !s->adj) return;
@@ -158,13 +161,13 @@ bp_status ensure(bp_solver *s, int total, int every) {
             (r = A.alloc(&A.gw, N)) != BP_OK || (r = A.alloc(&A.glam, N)) != BP_OK ||
             (r = A.alloc(&A.prec, N)) != BP_OK || (r = A.alloc(&A.cacc, N)) != BP_OK ||
             (r = A.alloc(&A.ccell, N)) != BP_OK || (r = A.alloc(&A.mtn, N)) != BP_OK ||
-            (r = A.alloc(&A.contacts, 1)) != BP_OK || (r = A.alloc(&A.corr, 6 * (size_t)N)) != BP_OK ||
+            (r = A.alloc(&A.contacts, 1)) != BP_OK || (r = A.alloc(&A.corr, 6 * (size_t)N + 2)) != BP_OK ||
             (r = A.alloc(&A.cb, s->nd)) != BP_OK || (r = A.alloc(&A.wb, s->nd)) != BP_OK ||
             (r = A.alloc(&A.fbar, 4 * (size_t)s->nf)) != BP_OK || (r = A.alloc(&A.facc, 2 * (size_t)s->nf)) != BP_OK ||
             (r = A.alloc(&A.seed, 3 * (size_t)N)) != BP_OK || (r = A.alloc(&A.outbuf, 3 * (size_t)N)) != BP_OK ||
             (r = A.alloc(&A.theta, 6 * (size_t)s->nscenes)) != BP_OK)
             return r;
-        HIP_TRY(hipMemsetAsync(A.corr, 0, 6 * (size_t)N * sizeof(unsigned long long), s->st));
+        HIP_TRY(hipMemsetAsync(A.corr, 0, (6 * (size_t)N + 2) * sizeof(unsigned long long), s->st));
         if ((r = A.alloc_state(A.last, N, batched)) != BP_OK) return r;
         // per-scene slot and face ranges
         std::vector<int> foff(s->nscenes + 1, 0);
@@ -369,6 +372,8 @@ bp_status bp_backward(bp_solver *s, const float *grad_x, const float *grad_v, fl
     launch_adj_seed(grad_v ? A.seed : nullptr, orig_T, A.gV[0], scale, N, st);
     HIP_TRY(hipMemsetAsync(A.cacc, 0, (size_t)N * sizeof(float), st));
     if (s->nf > 0) HIP_TRY(hipMemsetAsync(A.facc, 0, 2 * (size_t)s->nf * sizeof(float4), st));
+    // guard words of the fixed-point deliveries: overflow count, largest |term|
+    HIP_TRY(hipMemsetAsync(A.corr + 6 * (size_t)N, 0, 2 * sizeof(unsigned long long), st));
 
     const float inv_dt = 1.0f / A.subdt;
     const f3 gstep = gravity_step(s->P, A.subdt);
@@ -401,8 +406,19 @@ bp_status bp_backward(bp_solver *s, const float *grad_x, const float *grad_v, fl
     if (r2 != BP_OK) return r2;
     HIP_TRY(hipGetLastError());
     int contacts = 0;
+    unsigned long long guard[2] = {0ull, 0ull};
     HIP_TRY(hipMemcpyAsync(&contacts, A.contacts, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(guard, A.corr + 6 * (size_t)N, sizeof(guard), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    {
+        const uint32_t bits = (uint32_t)guard[1];
+        float mx_term;
+        std::memcpy(&mx_term, &bits, sizeof(float));
+        A.max_delivered = mx_term / scale;
+    }
+    if (guard[0] > 0)
+        return fail(BP_ERR_NOT_FINITE, "adjoint term out of the fixed-point delivery range (" +
+                                           std::to_string(guard[0]) + " terms above 2^17 after seed scaling)");
     if (contacts > 0)
         return fail(BP_ERR_UNSUPPORTED, "particle-particle contact occurred in the rollout; the adjoint does not "
                                         "differentiate contact (" + std::to_string(contacts) + " particle passes)");

@@ -340,6 +340,19 @@ void launch_map_particle_infos(const float4 *x, const int *orig, const int *grou
     hipLaunchKernelGGL(k_map_particle_infos, dim3(grid_blocks(n)), dim3(kBlock), 0, st, x, orig, group, out, n);
 }
 
+// The opt-in non-finite guard (the reference's CHECK_NAN printIfNan,
+// kernel/cubrasstower.cuh:15): the lowest slot whose position has a NaN or
+// infinite component, by atomicMin into *first, which the host sets to n.
+__global__ __launch_bounds__(kBlock) void k_first_nonfinite(const float4 *__restrict__ x, int n, int *first) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = x[i];
+    if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z))) atomicMin(first, i);
+}
+void launch_first_nonfinite(const float4 *x, int n, int *first, hipStream_t st) {
+    hipLaunchKernelGGL(k_first_nonfinite, dim3(grid_blocks(n)), dim3(kBlock), 0, st, x, n, first);
+}
+
 #define BP_INSTANTIATE_GRID(SP)                                                                                 \
     template void launch_predict_hash<SP>(const float4 *, const float4 *, int *, int, float, f3, const SP &,    \
                                           const int *, int2 *, hipStream_t);                                    \

@@ -393,6 +393,8 @@ bp_status validate_scene(const bp_scene_desc *sc) {
     if (sc->num_immovables < 0 || !ids_ok(sc->immovables, sc->num_immovables))
         return fail(BP_ERR_INVALID_ARG, "invalid immovable ids");
     if (sc->num_rigid < 0) return fail(BP_ERR_INVALID_ARG, "num_rigid < 0");
+    if (sc->num_rigid > 0 && !sc->rigid_ranges) return fail(BP_ERR_INVALID_ARG, "rigid_ranges is NULL");
+    std::vector<std::pair<int, int>> bodies;
     for (int b = 0; b < sc->num_rigid; ++b) {
         const int lo = sc->rigid_ranges[2 * b], hi = sc->rigid_ranges[2 * b + 1];
         if (lo < 0 || hi <= lo || hi > N || hi > sc->rigid_rest_len || !sc->rigid_rest_positions)
@@ -400,7 +402,14 @@ bp_status validate_scene(const bp_scene_desc *sc) {
         if (hi - lo > kMaxRigidParticles)
             return fail(BP_ERR_CAPACITY, "rigid body has more than 64 particles "
                                          "(NUM_MAX_PARTICLE_PER_RIGID_BODY)");
+        bodies.emplace_back(lo, hi);
     }
+    // one shape-matching workgroup per body writes its particles: two bodies
+    // sharing a particle would race on it
+    std::sort(bodies.begin(), bodies.end());
+    for (size_t b = 1; b < bodies.size(); ++b)
+        if (bodies[b].first < bodies[b - 1].second)
+            return fail(BP_ERR_INVALID_ARG, "rigid ranges overlap");
     return BP_OK;
 }
 
@@ -542,6 +551,7 @@ bp_status create_impl(const bp_scene_desc *scs, const bp_params *params, int ns,
         TRY(s->alloc(&s->vals[k], N));
     }
     TRY(s->alloc(&s->map_to_new, N));
+    TRY(s->alloc(&s->guard_slot, 1));
     TRY(s->alloc(&s->counts, radix_counts_len(N)));
     TRY(s->alloc(&s->totals, 256));
     TRY(s->alloc(&s->ranges, s->num_cells));
@@ -743,13 +753,18 @@ bp_status bp_solver_step(bp_solver *s, int num_substeps, float dt, int picked, c
         TIMED(F_PICK, launch_pick(s->x.cur(), s->v.cur(), s->map_to_new, picked, pp, 1, pv, s->st));
     }
     HIP_TRY(hipGetLastError());
-    if (s->P.nan_guard) {
-        std::vector<float4> hx(s->N);
-        HIP_TRY(hipMemcpyAsync(hx.data(), s->x.cur(), s->N * sizeof(float4), hipMemcpyDeviceToHost, s->st));
+    if (s->P.nan_guard) {  // scanned on the device; 4 bytes come back
+        int first = s->N;
+        HIP_TRY(hipMemcpyAsync(s->guard_slot, &first, sizeof(int), hipMemcpyHostToDevice, s->st));
+        launch_first_nonfinite(s->x.cur(), s->N, s->guard_slot, s->st);
+        HIP_TRY(hipMemcpyAsync(&first, s->guard_slot, sizeof(int), hipMemcpyDeviceToHost, s->st));
         HIP_TRY(hipStreamSynchronize(s->st));
-        for (int i = 0; i < s->N; ++i)
-            if (!std::isfinite(hx[i].x) || !std::isfinite(hx[i].y) || !std::isfinite(hx[i].z))
-                return fail(BP_ERR_NOT_FINITE, "non-finite position at slot " + std::to_string(i));
+        if (first < s->N) {
+            int orig_id = -1;
+            HIP_TRY(hipMemcpy(&orig_id, s->orig.cur() + first, sizeof(int), hipMemcpyDeviceToHost));
+            return fail(BP_ERR_NOT_FINITE, "non-finite position at slot " + std::to_string(first) +
+                                               " (original id " + std::to_string(orig_id) + ")");
+        }
     }
     return BP_OK;
 }
@@ -1101,6 +1116,7 @@ bp_status bp_debug_fetch(bp_solver *s, int which, void *out) {
         for (int i = 0; i < N; ++i) { o[3 * i] = t[i].x; o[3 * i + 1] = t[i].y; o[3 * i + 2] = t[i].z; }
         break;
     }
+    case 6: *(float *)out = adjoint_max_delivered(s); return BP_OK;
     default: return fail(BP_ERR_INVALID_ARG, "unknown debug buffer");
     }
     HIP_TRY(hipStreamSynchronize(s->st));

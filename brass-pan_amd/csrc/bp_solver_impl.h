@@ -70,6 +70,7 @@ struct bp_solver {
     bp_host::DBuf<float> m;
     bp_host::DBuf<int> ph, orig, sid;
     int *map_to_new = nullptr;
+    int *guard_slot = nullptr;  // the non-finite guard's answer (one int)
 
     int *keys[2] = {nullptr, nullptr};
     int *vals[2] = {nullptr, nullptr};
@@ -218,5 +219,7 @@ bp::f3 gravity_step(const bp_params &P, float subdt);
 void adjoint_release(bp_solver *s);
 // a recorded rollout no longer matches the solver state (bp_backward refuses it)
 void adjoint_invalidate(bp_solver *s);
+// largest |term| the last bp_backward delivered in fixed point (seed units; 0 if none ran)
+float adjoint_max_delivered(const bp_solver *s);
 
 }  // namespace bp_host

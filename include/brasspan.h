@@ -187,7 +187,10 @@ bp_status bp_set_rigid_state(bp_solver *s, const float *quats);
 /* Intermediate buffers of the last substep, slot order, for parity checks.
  * which: 0 = densities [N], 1 = lambdas [N], 2 = sorted cell keys [N],
  *        3 = sort permutation (previous slot of each new slot) [N],
- *        4 = omegas [3N], 5 = normals [3N]. */
+ *        4 = omegas [3N], 5 = normals [3N],
+ *        6 = the largest |term| the last bp_backward delivered between slots
+ *            in 2^-40 fixed point, in seed units [1] (bp_backward fails with
+ *            BP_ERR_NOT_FINITE when a term leaves that format's range). */
 bp_status bp_debug_fetch(bp_solver *s, int which, void *host_out);
 
 /* Stand-alone neighbour grid (updateGrid, particlesolver.cuh:712-765):

@@ -202,3 +202,94 @@ def test_full_size_adjoint_against_gpu_differences():
         e = rel(float(gth[0, k]), fd)
         print(f"  scene 0 theta {k}: adjoint {gth[0, k]:.6e} fd {fd:.6e} rel {e:.2e}")
         assert e <= FD_TOL
+
+
+def _scene_at(sc, pos, vel):
+    """A copy of scene `sc` that starts from the state (pos, vel), original order."""
+    import copy
+    out = copy.deepcopy(sc)
+    out._pos = [np.ascontiguousarray(pos, np.float32)]
+    out._vel = [np.ascontiguousarray(vel, np.float32)]
+    out._has_vel = True
+    return out
+
+
+def _compare(label, L, gv0, gth, Lh, dL, dirs, which=range(6)):
+    names = ["wind_x", "wind_y", "wind_z", "drag", "lift", "xsph_c"]
+    worst = 0.0
+    print(f"{label}: loss gpu {L:.9e} fp64 {Lh:.9e}")
+    for k in range(6):
+        e = rel(float(gth[0, k]), dL[k])
+        if k in which:
+            worst = max(worst, e)
+        print(f"  d/d{names[k]}: gpu {gth[0, k]:.6e} fp64 {dL[k]:.6e} rel {e:.2e}{'' if k in which else ' (reported)'}")
+    for j, d in enumerate(dirs):
+        g = float(np.sum(gv0.astype(np.float64) * d))
+        e = rel(g, dL[6 + j])
+        worst = max(worst, e)
+        print(f"  dL/dv0 . u{j}: gpu {g:.6e} fp64 {dL[6 + j]:.6e} rel {e:.2e}")
+    return worst
+
+
+def test_full_size_scene_adjoint_matches_fp64_derivative():
+    """One scene of the differentiable workload at its full size (16,384
+    fluid particles + a 64 x 64 cloth in wind) against the FP64 host
+    derivative.
+
+    Over the whole rollout (32 substeps, checkpoints every 8) the FP32 and
+    FP64 trajectories of the fluid drift apart (the half-sum density re-sorts
+    when a particle changes cell, so the fluid's rollout is only piecewise
+    smooth and the two take different branches), so the derivatives along the
+    fluid's initial velocities differ by several percent while the loss agrees
+    to 1e-4.  The cloth's parameters (wind, drag, lift) move only the cloth,
+    which never meets the fluid: their gradients over the whole rollout are
+    checked at ADJ_TOL.  Every component -- the six parameters and the initial
+    velocities along two random directions -- is then checked at ADJ_TOL on
+    each of the four 8-substep segments, GPU and host starting from the same
+    FP32 state (the GPU rollout's state at the segment's first substep).  By
+    the chain rule the segments' derivatives compose the rollout's.
+
+    Also reports the largest term the backward delivered between slots in
+    fixed point (bp_debug_fetch 6), which must stay inside the format's range
+    (|term| <= 2^17 with the seeds scaled to a unit maximum)."""
+    scs, prms = scenes.adjoint_scenes(1)
+    sc, prm = scs[0], prms[0]
+    n = sc.num_particles()
+    assert n == 16_384 + 64 * 64
+    K, every = scenes.ADJOINT_SUBSTEPS, scenes.ADJOINT_EVERY
+    target = seeded_target(sc, 4)
+    rng = np.random.default_rng(8)
+    dirs = [rng.normal(0, 1, (n, 3)) for _ in range(2)]
+    case(f"full-size scene adjoint n={n} substeps={K} every={every}")
+    s = bp.Solver(sc, prm)
+    L, gx0, gv0, gth = gpu_gradient(s, target, n, K, every)
+    mx = float(s.debug(6)[0])
+    s.close()
+    print(f"largest delivered term {mx:.3e} (seed units)")
+    assert mx <= 2.0 ** 17
+    case("FP64 host derivative of the full-size rollout")
+    Lh, dL = orc.rollout_tangent(bp.SceneArrays(sc), prm, K, DT, target, n, dirs)
+    _compare("32 substeps", L, gv0, gth, Lh, dL, [], which=range(5))
+    assert rel(L, Lh) < 1e-4
+    worst = max(rel(float(gth[0, k]), dL[k]) for k in range(5))
+    assert worst <= ADJ_TOL, f"cloth parameters over the rollout: worst relative error {worst:.3e}"
+
+    # the segment starts: states entering substeps 0, 8, 16, 24 of the GPU rollout
+    g = bp.Solver(sc, prm)
+    starts = []
+    for k in range(K):
+        if k % every == 0:
+            starts.append(g.state_original_order())
+        g.step(1, DT)
+    g.close()
+    worst = 0.0
+    for seg, (pos, vel) in enumerate(starts):
+        sseg = _scene_at(sc, pos, vel)
+        case(f"segment {seg}: substeps {seg * every}..{seg * every + every - 1}")
+        t = bp.Solver(sseg, prm)
+        Ls, _, gv0s, gths = gpu_gradient(t, target, n, every, every)
+        t.close()
+        Lhs, dLs = orc.rollout_tangent(bp.SceneArrays(sseg), prm, every, DT, target, n, dirs)
+        worst = max(worst, _compare(f"segment {seg}", Ls, gv0s, gths, Lhs, dLs, dirs))
+        assert rel(Ls, Lhs) < 1e-5
+    assert worst <= ADJ_TOL, f"worst relative gradient error over the segments {worst:.3e}"

@@ -104,3 +104,31 @@ def test_cpp_api_example_builds_and_fails_loudly_without_device(exe):
     assert os.path.exists(exe), f"{exe} not built"
     r = subprocess.run([exe, "1"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "no HIP device" in r.stderr
+
+
+def _create_desc_status(arr, params=None):
+    lib = bp.load_library()
+    h = C.c_void_p()
+    st = lib.bp_solver_create(C.byref(arr.desc), C.byref(params or bp.default_params()), None, C.byref(h))
+    return st, lib.bp_last_error_string().decode()
+
+
+def test_create_rejects_null_and_overlapping_rigid_ranges():
+    """One shape-matching workgroup per body writes that body's particles, so
+    ranges must exist and be disjoint; both are checked before any device
+    call (BP_ERR_INVALID_ARG)."""
+    sc = small_scene(grid_dims=(64, 64, 64))
+    sc.add_rigid_box((2, 2, 2), (1.0, 1.0, 1.0), 0.05, 1.0)
+    sc.add_rigid_box((2, 2, 2), (2.0, 1.0, 1.0), 0.05, 1.0)
+    arr = bp.SceneArrays(sc)
+    arr.desc.rigid_ranges = None
+    st, msg = _create_desc_status(arr)
+    assert st == 1 and "rigid_ranges" in msg
+    arr = bp.SceneArrays(sc)
+    arr.rranges[1, 0] = arr.rranges[0, 1] - 1  # second body starts inside the first
+    st, msg = _create_desc_status(arr)
+    assert st == 1 and "overlap" in msg
+    arr = bp.SceneArrays(sc)
+    arr.rranges[:] = arr.rranges[::-1].copy()  # disjoint but listed out of order: accepted
+    st, msg = _create_desc_status(arr)
+    assert st != 1, msg
