@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "liboracle.so")
+# BP_ORACLE_LIB selects another build of the same source, such as the
+# sanitized one of `make -C oracle asan`.
+LIB_PATH = os.environ.get("BP_ORACLE_LIB") or os.path.join(_HERE, "lib", "liboracle.so")
 _lib = None
 
 _D = C.POINTER(C.c_double)
