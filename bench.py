@@ -1,15 +1,19 @@
 """Benchmark: particle-substeps/s of the gfx950 particle solver substep.
 
 A step is one substep (grid build + sort + 4 solver iterations + post passes)
-of the plain solver over a synthetic scene resident in HBM.  Prints one JSON
-line.  The default workload is the 282,624-particle mixed-phase scene (fluid +
-64 shape-matched boxes + 128x128 cloth under wind); the same line also carries
-the 4,194,304-particle large block, the HBM-roofline case, and the 512-scene
-batch (2k..64k particles per scene) against the plain solver looped over the
-same scenes, and the adjoint/forward time ratio of the 64-scene
-differentiable rollout (32 substeps, checkpoints every 8).
+over synthetic scenes resident in HBM.  Prints one JSON line.  The headline
+workload is the metric's own configuration: 512 independent scenes of 2k..64k
+particles (fluid block, 1-8 shape-matched boxes and a pinned cloth in wind
+each; 9.5M particles in all) advanced by the batched solver, one launch per
+pass for all scenes, replayed from a hipGraph.  The same line carries the
+plain solver looped over the same scenes (the comparison the metric names),
+the roofline of the batch's dominant kernel, a CPU baseline on a bounded
+sample of the same scenes, the per-substep error against the FP32 and FP64
+oracles, and sub-objects for the 16,384-particle dam break, the
+282,624-particle mixed scene, the 4,194,304-particle block and the adjoint /
+forward ratio of the 64-scene differentiable rollout.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 from __future__ import annotations
 
@@ -107,48 +111,12 @@ def kernel_roofline(s, subdt, substeps=5, workload=None):
     }, {k: [round(v[0], 3), v[1]] for k, v in sorted(times.items(), key=lambda kv: -kv[1][0])}
 
 
-def _oracle_rate(o, n, subdt, budget_s, max_substeps=50):
-    o.step(1, subdt)  # warm-up substep (untimed)
-    done = 0
-    t0 = time.perf_counter()
-    while True:
-        o.step(1, subdt)
-        done += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or done >= max_substeps:
-            break
-    return n * done / el, done, el
-
-
 def host_threads():
     """Threads this process may use: the OpenMP setting, else the CPUs in its affinity mask."""
     env = os.environ.get("OMP_NUM_THREADS", "")
     if env.isdigit() and int(env) > 0:
         return int(env)
     return len(os.sched_getaffinity(0))
-
-
-def cpu_baseline(sc, prm, subdt, budget_s=10.0):
-    """The FP32 oracle on a bounded sample of the same scene, in two forms:
-    gather form (the GPU's pair order) over all host threads with OpenMP, and
-    the reference's own order (scatter form) on one thread."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle_py as orc
-
-    n = sc.num_particles()
-    threads = orc.set_threads(host_threads())
-    o = orc.Oracle(bp.SceneArrays(sc), prm, fp64=False, mode=orc.GATHER)
-    rate_mt, done_mt, el_mt = _oracle_rate(o, n, subdt, budget_s)
-    del o
-    orc.set_threads(1)
-    o = orc.Oracle(bp.SceneArrays(sc), prm, fp64=False, mode=orc.SCATTER)
-    rate_1, done_1, el_1 = _oracle_rate(o, n, subdt, budget_s)
-    return ({"value": round(rate_mt, 1), "unit": "particle-substeps/s", "cores": threads, "kind": "port",
-             "sample": f"{done_mt} substeps after 1 warm-up substep, {el_mt:.1f} s, FP32 oracle in gather form "
-                       f"(the GPU's pair order) over {threads} OpenMP threads"},
-            {"value": round(rate_1, 1), "unit": "particle-substeps/s", "cores": 1, "kind": "port",
-             "sample": f"{done_1} substeps after 1 warm-up substep, {el_1:.1f} s, single-threaded FP32 oracle "
-                       f"in reference order"})
 
 
 def hip_stream():
@@ -161,29 +129,44 @@ def hip_stream():
     return h.value
 
 
-def time_batched(frames=3, warmup=1, num=512, graphs=True):
-    """The 512-scene batched workload (2 substeps per frame of 1/60 s) in one
-    bp_batch_create handle with graphs and tuned block sizes, against the
-    plain solver looped over the same scenes on one stream, eager and with
-    graphs.  The batch must equal the eager loop bit for bit."""
-    scs, prms = scenes.batched_scenes(num)
-    dt, sub = 1.0 / 60.0, 2
-    n = sum(sc.num_particles() for sc in scs)
+BATCH_DT, BATCH_SUB = 1.0 / 60.0, 2  # frames of update(2, 1/60): substeps of 1/120 s
+
+
+def run_substeps(s, count, subdt):
+    """`count` substeps of `subdt`, two per call so that graph replay engages
+    (a graph holds two substeps; 2 * subdt / 2 is exact in FP32)."""
+    while count >= 2:
+        s.step(2, 2 * subdt)
+        count -= 2
+    if count:
+        s.step(1, subdt)
+
+
+def batched_headline(scs, prms, steps, warmup, graphs=True):
+    """The 512-scene batch in one bp_batch_create handle: workgroup sizes
+    and iteration fusion from bp_solver_autotune (which restores the state it
+    started from), then `warmup` untimed and `steps` timed substeps replayed
+    from the handle's hipGraph."""
+    subdt = BATCH_DT / BATCH_SUB
     bs = bp.BatchSolver(scs, prms)
-    tune = bs.autotune(dt / sub, 2)
+    tune = bs.autotune(subdt, 2)
     bs.set_graph(graphs)
-    for _ in range(warmup):
-        bs.step(sub, dt)
+    run_substeps(bs, warmup, subdt)
     bs.sync()
     t0 = time.perf_counter()
-    for _ in range(frames):
-        bs.step(sub, dt)
+    run_substeps(bs, steps, subdt)
     bs.sync()
-    wall_b = time.perf_counter() - t0
-    bpos, _ = bs.state_original_order()
-    ranges = bs.ranges
-    bs.close()
+    return bs, tune, time.perf_counter() - t0
 
+
+def plain_loop(scs, prms, frames=3, graphs=True):
+    """The plain solver looped over the same scenes on one stream (the
+    per-scene loop the batched mode is measured against), eager and with
+    per-solver graphs, over `frames` frames of update(2, 1/60) after one
+    warm-up frame.  Its eager state must equal a batch stepped as far, bit
+    for bit."""
+    sub, dt = BATCH_SUB, BATCH_DT
+    n = sum(sc.num_particles() for sc in scs)
     stream = hip_stream()
     loop = [bp.Solver(sc, prm, stream=stream) for sc, prm in zip(scs, prms)]
 
@@ -193,31 +176,141 @@ def time_batched(frames=3, warmup=1, num=512, graphs=True):
                 s.step(sub, dt)
         loop[-1].sync()
 
-    run_loop(warmup)
+    run_loop(1)
     t0 = time.perf_counter()
     run_loop(frames)
     wall_e = time.perf_counter() - t0
-    same = all(np.array_equal(bpos[f:f + c], s.state_original_order()[0]) for (f, c), s in zip(ranges, loop))
+    ref = bp.BatchSolver(scs, prms)
+    ref.set_graph(graphs)
+    for _ in range(1 + frames):
+        ref.step(sub, dt)
+    bpos, _ = ref.state_original_order()
+    same = all(np.array_equal(bpos[f:f + c], s.state_original_order()[0]) for (f, c), s in zip(ref.ranges, loop))
+    ref.close()
     for s in loop:
         s.set_graph(graphs)
-    run_loop(warmup)
+    run_loop(1)
     t0 = time.perf_counter()
     run_loop(frames)
     wall_g = time.perf_counter() - t0
     for s in loop:
         s.close()
     ps = n * sub * frames
-    best_loop = min(wall_e, wall_g)
-    return {
-        "value": round(ps / wall_b, 1), "unit": "particle-substeps/s", "ms_per_substep": round(wall_b / (frames * sub) * 1e3, 3),
-        "plain_loop_eager": round(ps / wall_e, 1), "plain_loop_graph": round(ps / wall_g, 1),
-        "speedup_vs_plain_loop": round(best_loop / wall_b, 2),
-        "bitexact_vs_plain_loop": bool(same),
-        "config": {"workload": f"batched_{num}_scenes", "scenes": num, "particles": n,
-                   "particles_min": min(c for _, c in ranges), "particles_max": max(c for _, c in ranges),
-                   "iterations": prms[0].iterations, "substeps_per_frame": sub, "frames": frames,
-                   "warmup_frames": warmup, "block_sizes": tune},
-    }
+    return {"eager": round(ps / wall_e, 1), "graph": round(ps / wall_g, 1), "unit": "particle-substeps/s",
+            "substeps": sub * frames, "bitexact_vs_batch": bool(same)}
+
+
+def time_batched(frames=3, warmup=1, num=512, graphs=True):
+    """The batched leg alone (for profiling): the batch over `frames` frames
+    and the plain loop over the same scenes."""
+    scs, prms = scenes.batched_scenes(num)
+    bs, tune, wall = batched_headline(scs, prms, BATCH_SUB * frames, BATCH_SUB * warmup, graphs)
+    n = bs.n
+    bs.close()
+    ps = n * BATCH_SUB * frames
+    loop = plain_loop(scs, prms, frames, graphs)
+    return {"value": round(ps / wall, 1), "unit": "particle-substeps/s",
+            "ms_per_substep": round(wall / (frames * BATCH_SUB) * 1e3, 3), "plain_loop": loop,
+            "speedup_vs_plain_loop": round(ps / wall / max(loop["eager"], loop["graph"]), 2),
+            "config": {"workload": f"batched_{num}_scenes", "scenes": num, "particles": n, "block_sizes": tune}}
+
+
+def cpu_baseline_batch(scs, prms, subdt, budget_s=15.0, ref_budget_s=5.0):
+    """The FP32 oracle on a bounded sample of the batched workload: scenes in
+    batch order, each one warm-up substep and two timed substeps, until the
+    timed work reaches the budget.  Gather form (the GPU's pair order) over
+    all host threads with OpenMP, and the reference's own order (scatter
+    form) on one thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as orc
+
+    def sample(mode, budget):
+        done = el = 0.0
+        used = 0
+        for sc, prm in zip(scs, prms):
+            o = orc.Oracle(bp.SceneArrays(sc), prm, fp64=False, mode=mode)
+            o.step(1, subdt)
+            t0 = time.perf_counter()
+            o.step(1, subdt)
+            o.step(1, subdt)
+            el += time.perf_counter() - t0
+            done += 2 * sc.num_particles()
+            used += 1
+            del o
+            if el >= budget:
+                break
+        return done / el, used, int(done), el
+
+    threads = orc.set_threads(host_threads())
+    r_mt, u_mt, d_mt, e_mt = sample(orc.GATHER, budget_s)
+    orc.set_threads(1)
+    r_1, u_1, d_1, e_1 = sample(orc.SCATTER, ref_budget_s)
+    orc.set_threads(host_threads())
+    return ({"value": round(r_mt, 1), "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+             "sample": f"the first {u_mt} of the 512 scenes, 2 timed substeps each after 1 warm-up substep "
+                       f"({d_mt} particle-substeps, {e_mt:.1f} s), FP32 oracle in gather form (the GPU's pair "
+                       f"order) over {threads} OpenMP threads"},
+            {"value": round(r_1, 1), "unit": "particle-substeps/s", "cores": 1, "kind": "port",
+             "sample": f"the first {u_1} scenes, 2 timed substeps each ({d_1} particle-substeps, {e_1:.1f} s), "
+                       f"single-threaded FP32 oracle in reference order"})
+
+
+def substep_errors(sc, prm, subdt, substeps=3, h=0.1):
+    """Per-substep error of the GPU: from the same state as the FP32 oracle in
+    reference order and its FP64 twin, one substep each, `substeps` times;
+    max and RMS of |x_gpu - x_oracle| over particles, in units of h."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as orc
+
+    arr = bp.SceneArrays(sc)
+    o32 = orc.Oracle(arr, prm, fp64=False, mode=orc.SCATTER)
+    o64 = orc.Oracle(arr, prm, fp64=True, mode=orc.SCATTER)
+    g = bp.Solver(sc, prm)
+    rows = []
+    for _ in range(substeps):
+        p, v, m = o32.get_state()
+        o64.set_state(p, v, m)
+        g.set_state(p.astype(np.float32), v.astype(np.float32), m)
+        if o32.num_rigid:
+            q, _ = o32.rigid_state()
+            o64.set_rigid_state(q)
+            g.set_rigid_state(q.astype(np.float32))
+        g.step(1, subdt)
+        o32.step(1, subdt)
+        o64.step(1, subdt)
+        x = g.state_original_order()[0].astype(np.float64)
+        e32 = np.linalg.norm(x - o32.state_original_order()[0], axis=1) / h
+        e64 = np.linalg.norm(x - o64.state_original_order()[0], axis=1) / h
+        rows.append((e32.max(), np.sqrt((e32 ** 2).mean()), e64.max(), np.sqrt((e64 ** 2).mean())))
+    g.close()
+    r = np.array(rows)
+    return {"max_vs_fp32": float(r[:, 0].max()), "rms_vs_fp32": float(r[:, 1].max()),
+            "max_vs_fp64": float(r[:, 2].max()), "rms_vs_fp64": float(r[:, 3].max()), "unit": "h",
+            "substeps": substeps, "particles": sc.num_particles()}
+
+
+def horizon_errors(sc, prm, subdt, substeps=10, h=0.1):
+    """Free-running error over `substeps` substeps from the scene's initial
+    state: max |x_gpu - x_oracle| / h after each substep, against the FP32
+    oracle (reference order) and the FP64 oracle."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as orc
+
+    arr = bp.SceneArrays(sc)
+    o32 = orc.Oracle(arr, prm, fp64=False, mode=orc.SCATTER)
+    o64 = orc.Oracle(arr, prm, fp64=True, mode=orc.SCATTER)
+    g = bp.Solver(sc, prm)
+    e32, e64 = [], []
+    for _ in range(substeps):
+        g.step(1, subdt)
+        o32.step(1, subdt)
+        o64.step(1, subdt)
+        x = g.state_original_order()[0].astype(np.float64)
+        e32.append(float(np.abs(x - o32.state_original_order()[0]).max() / h))
+        e64.append(float(np.abs(x - o64.state_original_order()[0]).max() / h))
+    g.close()
+    return {"max_vs_fp32": [float("%.3g" % e) for e in e32], "max_vs_fp64": [float("%.3g" % e) for e in e64],
+            "unit": "h", "particles": sc.num_particles()}
 
 
 def time_adjoint(num=64, graphs=True):
@@ -336,20 +429,61 @@ def copy_calibration(mib):
     hip.hipFree(b)
 
 
+def elementwise_copy_bandwidth(mib=1024, reps=5):
+    """A second STREAM-style check of the HBM peak with a compute-kernel
+    copy (torch's vectorized elementwise copy) rather than the runtime's DMA
+    path: 1 GiB of float32, HIP events on torch's stream, best of `reps`,
+    bytes read plus bytes written.  None if torch is not importable."""
+    try:
+        import torch
+    except ImportError:
+        return None
+    if not torch.cuda.is_available():
+        return None
+    n = (mib << 20) // 4
+    a = torch.ones(n, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    best = float("inf")
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    del a, b
+    torch.cuda.empty_cache()
+    return {"bytes": 2 * (mib << 20), "ms": round(best, 4), "GBs": round(2 * (mib << 20) / (best * 1e-3) / 1e9, 1)}
+
+
+def plain_leg(name, steps, warmup, roof_substeps=5):
+    """A plain-solver workload: timing, roofline of its dominant kernel."""
+    sc, prm, subdt = build_workload(name)
+    s, wall = time_plain(sc, prm, subdt, steps, warmup)
+    roof, times = kernel_roofline(s, subdt, substeps=roof_substeps, workload=name)
+    out = {"value": round(s.n * steps / wall, 1), "unit": "particle-substeps/s",
+           "ms_per_step": round(wall / steps * 1e3, 4), "steps": steps, "warmup": warmup, "particles": s.n,
+           "block_sizes": s.block_sizes, "roofline": roof, "kernel_ms": times}
+    return out, s, sc, prm, subdt
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="mixed_phase_283k", choices=sorted(scenes.WORKLOADS))
+    ap.add_argument("--steps", type=int, default=40, help="timed substeps of the 512-scene batch")
+    ap.add_argument("--warmup", type=int, default=10, help="untimed substeps before them")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-large", action="store_true", help="skip the 4M large-block measurement")
-    ap.add_argument("--no-batched", action="store_true", help="skip the 512-scene batched measurement")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the per-substep errors against the oracle")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-solver sub-objects (c1, c2, c3)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every kernel (no hipGraph replay; for kernel-trace profiling)")
-    ap.add_argument("--leg", choices=("all", "batched", "adjoint"), default="all",
-                    help="batched / adjoint: run only that measurement and print its object (for profiling)")
+    ap.add_argument("--leg", choices=("all", "headline", "batched", "adjoint", "plain"), default="all",
+                    help="headline (the batch's timing and roofline) / batched (with the plain loop) / adjoint / "
+                         "plain (--workload): run only that measurement and print its object (for profiling)")
+    ap.add_argument("--workload", default="mixed_phase_283k", choices=sorted(scenes.WORKLOADS),
+                    help="the plain workload of --leg plain")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device (rocprof counter calibration)")
     args = ap.parse_args()
@@ -366,44 +500,73 @@ def main():
     if args.leg == "adjoint":
         print(json.dumps({"adjoint_64": time_adjoint(graphs=not args.no_graph)}), flush=True)
         return
-    sc, prm, subdt = build_workload(args.workload)
-    s, wall = time_plain(sc, prm, subdt, args.steps, args.warmup)
-    n = s.n
+    if args.leg == "plain":
+        steps = 10 if args.workload == "large_block_4m" else 50
+        out, s, *_ = plain_leg(args.workload, steps, 2 if args.workload == "large_block_4m" else 10)
+        s.close()
+        print(json.dumps({args.workload: out}), flush=True)
+        return
+
+    # headline: the 512-scene batch
+    scs, prms = scenes.batched_scenes(512)
+    subdt = BATCH_DT / BATCH_SUB
+    graphs = not args.no_graph
+    bs, tune, wall = batched_headline(scs, prms, args.steps, args.warmup, graphs)
+    n = bs.n
     value = n * args.steps / wall
-    roof, fam_times = kernel_roofline(s, subdt, workload=args.workload)
-    s.close()
+    roof, fam_times = kernel_roofline(bs, subdt, substeps=2, workload="batched_512_scenes")
+    bs.close()
+    if args.leg == "headline":
+        print(json.dumps({"batched_512_scenes": {"value": round(value, 1), "ms_per_step": round(wall / args.steps * 1e3, 4),
+                                                 "roofline": roof, "kernel_ms": fam_times, "block_sizes": tune}}),
+              flush=True)
+        return
+    loop = plain_loop(scs, prms, frames=3, graphs=graphs)
+    best_loop = max(loop["eager"], loop["graph"])
+    ranges_n = [sc.num_particles() for sc in scs]
     out = {
-        "metric": "particle-substeps/s (4 iters) plain vs 512-scene batched; adjoint/forward time ratio", "value": round(value, 1), "unit": "particle-substeps/s",
+        "metric": "particle-substeps/s (4 iters) plain vs 512-scene batched; adjoint/forward time ratio",
+        "value": round(value, 1), "unit": "particle-substeps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (seeded splitmix64 scene, no checkpoint)",
-        "config": {"workload": args.workload, "particles": n, "iterations": prm.iterations,
-                   "substep_dt": subdt, "distance_constraints": len(sc.distance_pairs),
-                   "wind_faces": len(sc.faces), "rigid_bodies": len(sc.rigidbody_particle_id_ranges),
-                   "block_sizes": s.block_sizes, "parallelism": "single GPU"},
+        "data": "synthetic (seeded splitmix64 scenes, no checkpoint)",
+        "config": {"workload": "batched_512_scenes", "scenes": len(scs), "particles": n,
+                   "particles_min": min(ranges_n), "particles_max": max(ranges_n),
+                   "iterations": prms[0].iterations, "substep_dt": subdt,
+                   "distance_constraints": sum(len(sc.distance_pairs) for sc in scs),
+                   "wind_faces": sum(len(sc.faces) for sc in scs),
+                   "rigid_bodies": sum(len(sc.rigidbody_particle_id_ranges) for sc in scs),
+                   "graph_replay": graphs, "block_sizes": tune, "parallelism": "single GPU"},
         "roofline": roof,
         "kernel_ms": fam_times,
-        "hbm_copy_check": copy_bandwidth(),
+        "plain_loop": loop,
+        "speedup_vs_plain_loop": round(value / best_loop, 2),
     }
-    if not args.no_large and args.workload != "large_block_4m":
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"], out["cpu_baseline_reference_order"] = cpu_baseline_batch(scs, prms, subdt)
+    if not args.no_accuracy:
+        acc = {"batched_512_scenes": {f"scene_{k}": substep_errors(scs[k], prms[k], subdt) for k in (0, 1)}}
+        dsc, dprm, dsub = build_workload("dam_break_16k")
+        acc["dam_break_16k_horizon"] = horizon_errors(dsc, dprm, dsub)
+        out["accuracy"] = acc
+    del scs, prms
+    if not args.no_plain:
+        out["dam_break_16k"], s, *_ = plain_leg("dam_break_16k", 50, 10)
+        s.close()
+        out["mixed_phase_283k"], s, msc, mprm, msub = plain_leg("mixed_phase_283k", 50, 10)
+        s.close()
+        if not args.no_accuracy:
+            out["mixed_phase_283k"]["accuracy"] = substep_errors(msc, mprm, msub, substeps=2)
+        del msc
         # The large block's Akinci tension (gamma 0.5, no mass weighting,
         # fluid.cuh:436-489) drives it unstable after ~20 substeps; time
         # substeps 3-12 from the lattice.
-        lsc, lprm, lsub = build_workload("large_block_4m")
-        lsteps, lwarm = 10, 2
-        ls, lwall = time_plain(lsc, lprm, lsub, lsteps, lwarm)
-        lroof, ltimes = kernel_roofline(ls, lsub, substeps=2, workload="large_block_4m")
-        out["large_block_4m"] = {"value": round(ls.n * lsteps / lwall, 1),
-                                 "ms_per_step": round(lwall / lsteps * 1e3, 4), "steps": lsteps,
-                                 "warmup": lwarm, "block_sizes": ls.block_sizes, "roofline": lroof,
-                                 "kernel_ms": ltimes}
-        ls.close()
-    if not args.no_batched:
-        out["batched_512"] = time_batched(graphs=not args.no_graph)
+        out["large_block_4m"], s, *_ = plain_leg("large_block_4m", 10, 2, roof_substeps=2)
+        s.close()
     if not args.no_adjoint:
-        out["adjoint_64"] = time_adjoint(graphs=not args.no_graph)
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"], out["cpu_baseline_reference_order"] = cpu_baseline(sc, prm, subdt)
+        out["adjoint_64"] = time_adjoint(graphs=graphs)
+    out["hbm_copy_check"] = copy_bandwidth()
+    out["hbm_copy_check_elementwise"] = elementwise_copy_bandwidth()
     print(json.dumps(out), flush=True)
 
 
