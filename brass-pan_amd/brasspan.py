@@ -89,6 +89,7 @@ SIGNATURES = {
     "bp_debug_fetch": (_S, [_P, C.c_int, _P]),
     "bp_neighbor_grid": (_S, [C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int),
                               C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "bp_neighbor_grid_noalloc": (_S, [_P, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int), _P, _P, _P, _P, _P]),
     "bp_batch_create": (_S, [C.POINTER(bp_scene_desc), C.POINTER(bp_params), C.c_int, _P, C.POINTER(_P)]),
     "bp_batch_scene_range": (_S, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "bp_solver_set_graph": (_S, [_P, C.c_int]),

@@ -202,6 +202,20 @@ bp_status bp_neighbor_grid(const float *xs, int n, const float cell_size[3],
                            const int grid_dims[3], int *keys, int *perm,
                            int *cell_start, int *cell_end);
 
+/* bp_neighbor_grid without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops).  Device buffers throughout: positions = float4[n] (x, y, z,
+ * w ignored), keys[n] and perm[n] (required when n > 0), cell_start and
+ * cell_end [dims product] (may be NULL), results equal to bp_neighbor_grid's.
+ * Asynchronous on hip_stream (NULL = default stream); no hipMalloc, hipFree or
+ * memset.  At most 2^20 positions (BP_ERR_CAPACITY beyond).  The sort's
+ * scratch is one fixed array per GPU (about 9.4 MB, held until the process
+ * ends) that keeps the last call's contents: calls that overlap on one GPU
+ * (two streams, two threads) race and must be serialised by the caller. */
+bp_status bp_neighbor_grid_noalloc(const float *positions, int n, const float cell_size[3],
+                                   const int grid_dims[3], int *keys, int *perm,
+                                   int *cell_start, int *cell_end, void *hip_stream);
+
 /* Batched solver (no counterpart in the reference, which runs one scene):
  * `num_scenes` independent scenes laid end to end in one solver, each with its
  * own grid, kernel radius, planes and per-scene parameters (rest density, wind,
