@@ -116,8 +116,8 @@ __device__ __forceinline__ unsigned long long to_fixed(float v) {
 //   k's cell.
 // A pair k owns whose partner j's search does not reach k's cell is delivered
 // to j through corr[j] (fixed point).  k_position_apply adds both parts.
-template <class SP>
-__global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_position(const float4 *__restrict__ xs,
+template <class SP, bool Flat>
+__global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, Flat ? 5 : 6) void k_position(const float4 *__restrict__ xs,
                                                      const float4 *__restrict__ lam_rec,
                                                      const int *__restrict__ build_keys,
                                                      const int2 *__restrict__ ranges,
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_position(const 
         rec.in = kf ? srec : nullptr;
         rec.slot = k;
         rec.n = n;
-        for_each_neighbor_nb<kBodyBatch>(
+        walk_form<Flat, kBodyBatch>(
             G, ranges, g, xs, 0,
             // (fluid particles replay the lambda pass's masks; a solid screens for
             // fluid partners only)
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_post_a(const fl
 // fluidVorticity (fluid.cuh:281-336, in place) followed by fluidAkinciTension
 // (fluid.cuh:436-489) for the same particle: both only change the particle's
 // own velocity, so the fused pass writes v in place.
-template <class SP>
+template <class SP, bool Flat>
 __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 1) void k_post_b(const float4 *__restrict__ xs, float4 *__restrict__ v,
                                                    const float4 *__restrict__ omega,
                                                    const float4 *__restrict__ xom,
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 1) void k_post_b(const fl
         // one 16-byte fetch of the second record whatever is used of it (a
         // conditional fetch is split into per-component loads)
         const float4 *__restrict__ second = do_akinci ? normal : xom;
-        for_each_neighbor_nb<kBodyBatch>(
+        walk_form<Flat, kBodyBatch>(
             G, ranges, g, xs, 0,
             [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
             [&](int j) { return Partner{xom[j], second[j]}; },
@@ -414,9 +414,13 @@ template <class SP>
 void launch_position(const float4 *xs, const float4 *lam_rec, const int *build_keys, const int2 *ranges,
                      const uint32_t *srec, const uint32_t *fluid_stamp, const uint32_t *stamp, float4 *sum_tmp,
                      unsigned long long *corr, float4 *xs_out, float4 *x_io, float4 *v_out, int n, const SP &sp,
-                     const Epilogue &E, int block, hipStream_t st) {
-    hipLaunchKernelGGL(k_position<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, lam_rec, build_keys,
-                       ranges, srec, fluid_stamp, stamp, sum_tmp, corr, n, sp);
+                     const Epilogue &E, int block, int flat, hipStream_t st) {
+    if (flat)
+        hipLaunchKernelGGL((k_position<SP, true>), dim3(grid_blocks(n, block)), dim3(block), flat_lds_bytes(block), st,
+                           xs, lam_rec, build_keys, ranges, srec, fluid_stamp, stamp, sum_tmp, corr, n, sp);
+    else
+        hipLaunchKernelGGL((k_position<SP, false>), dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, lam_rec,
+                           build_keys, ranges, srec, fluid_stamp, stamp, sum_tmp, corr, n, sp);
     hipLaunchKernelGGL(k_position_apply<SP>, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, sum_tmp, corr, xs_out,
                        x_io, v_out, n, sp, E);
 }
@@ -433,9 +437,13 @@ void launch_post_a(const float4 *xs, const float4 *v, const float *density, cons
 template <class SP>
 void launch_post_b(const float4 *xs, float4 *v, const float4 *omega, const float4 *xom, const float4 *normal,
                    const float *density, const int2 *ranges, const uint32_t *ccell, const uint32_t *srec, int n,
-                   const SP &sp, int do_vort, int do_akinci, float dt, int block, hipStream_t st) {
-    hipLaunchKernelGGL(k_post_b<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, v, omega, xom, normal,
-                       density, ranges, ccell, srec, n, sp, do_vort, do_akinci, dt);
+                   const SP &sp, int do_vort, int do_akinci, float dt, int block, int flat, hipStream_t st) {
+    if (flat)
+        hipLaunchKernelGGL((k_post_b<SP, true>), dim3(grid_blocks(n, block)), dim3(block), flat_lds_bytes(block), st, xs,
+                           v, omega, xom, normal, density, ranges, ccell, srec, n, sp, do_vort, do_akinci, dt);
+    else
+        hipLaunchKernelGGL((k_post_b<SP, false>), dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, v, omega, xom,
+                           normal, density, ranges, ccell, srec, n, sp, do_vort, do_akinci, dt);
 }
 template <class SP>
 void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const uint32_t *ccell, const uint32_t *srec,
@@ -450,13 +458,13 @@ void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const ui
     template void launch_position<SP>(const float4 *, const float4 *, const int *, const int2 *,                \
                                       const uint32_t *, const uint32_t *, const uint32_t *, float4 *,           \
                                       unsigned long long *, float4 *, float4 *, float4 *, int, const SP &,      \
-                                      const Epilogue &, int, hipStream_t);                                      \
+                                      const Epilogue &, int, int, hipStream_t);                                 \
     template void launch_post_a<SP>(const float4 *, const float4 *, const float *, const int2 *, float4 *,      \
                                     float4 *, float4 *, uint32_t *, uint32_t *, const LambdaIn *, int, int,     \
                                     const SP &, int, int, int, hipStream_t);                                    \
     template void launch_post_b<SP>(const float4 *, float4 *, const float4 *, const float4 *, const float4 *,   \
                                     const float *, const int2 *, const uint32_t *, const uint32_t *, int,       \
-                                    const SP &, int, int, float, int, hipStream_t);                             \
+                                    const SP &, int, int, float, int, int, hipStream_t);                        \
     template void launch_xsph<SP>(const float4 *, const float4 *, const int2 *, const uint32_t *,               \
                                   const uint32_t *, float4 *, int, const SP &, int, int, hipStream_t);
 BP_INSTANTIATE_FLUID(OneScene)

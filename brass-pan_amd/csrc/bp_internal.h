@@ -11,10 +11,10 @@ constexpr int kMaxPlanes = 16;
 constexpr int kMaxRigidParticles = 64;  // NUM_MAX_PARTICLE_PER_RIGID_BODY (cubrasstower.cuh:22)
 constexpr int kBlock = 256;      // streaming kernels
 constexpr int kMaxBlock = 512;   // neighbour passes: block size chosen at run time (autotuner)
-// Minimum waves per SIMD a neighbour pass asks the compiler for (SP::kWaves;
-// 6 = at most 80 VGPRs).  The passes wait on memory, so occupancy pays where
-// the walk fits the budget without spilling: single-scene passes other than
-// vorticity/tension.  1 leaves the register budget open.
+// Minimum waves per SIMD a neighbour pass asks the compiler for (the smaller
+// of SP::kWaves and the pass's own W; 6 = at most 80 VGPRs, 5 = 96).  The
+// passes wait on memory, so occupancy pays where the walk fits the budget
+// with few or no spills.  1 leaves the register budget open.
 #define kPassWaves(SP, W) __attribute__((amdgpu_waves_per_eu(SP::kWaves < W ? SP::kWaves : W)))
 
 // ------------------------------------------------------------ small vectors
@@ -197,7 +197,9 @@ struct OneScene {
     template <class F> __device__ __forceinline__ void with_scene(int, F &&f) const { f(c); }
 };
 struct SceneTable {
-    static constexpr int kWaves = 1;
+    // 96 VGPRs: the omega/normal pass spills 2 and runs 10% faster than
+    // uncapped at 98 (4 waves); lambda and XSPH fit without spills
+    static constexpr int kWaves = 5;
     const SceneConst *__restrict__ table;
     const int *__restrict__ sid;  // slot -> scene
     __device__ __forceinline__ const SceneConst &at(int slot) const { return table[sid[slot]]; }
@@ -511,6 +513,167 @@ __device__ __forceinline__ void for_each_neighbor(const GridDesc &G, const int2 
                                                   const int g[3], const float4 *__restrict__ xs, int jmin,
                                                   Pick &&pick, Body &&body) {
     for_each_neighbor(G, ranges, g, xs, jmin, pick, body, AllowAll{});
+}
+
+// ------------------------------------------------------------ flat walk
+// The walk with its pair bodies run over the whole walk rather than per
+// chunk.  A wave runs a chunk's bodies as often as its busiest lane has
+// partners in that chunk, so the per-chunk form costs the sum over chunks of
+// that maximum; the flat form costs the maximum over lanes of all partners.
+// The walk appends each chunk's (base, mask) to a list of kListCap entries per
+// lane in LDS, and the bodies then drain the list in order (chunks in walk
+// order, partners ascending within a chunk): the pairs and their summation
+// order are those of for_each_neighbor_nb, so results are bit-identical.
+// The wave drains when one of its lanes' lists is full and after the walk.
+// A column's runs share one chunk loop, so the chunk code appears once in the
+// kernel.  Launch with flat_lds_bytes(block) bytes of dynamic LDS.
+constexpr int kListCap = 12;
+inline size_t flat_lds_bytes(int block) { return (size_t)block * kListCap * 8; }
+typedef unsigned int u2v __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) u2v lds_u2;
+extern __shared__ u2v bp_flat_lds[];
+
+template <int NB, class Pick, class Fetch, class Use, class Allow>
+__device__ __forceinline__ void for_each_neighbor_flat(const GridDesc &G, const int2 *__restrict__ ranges,
+                                                       const int g[3], const float4 *__restrict__ xs, int jmin,
+                                                       Pick &&pick, Fetch &&fetch, Use &&use, Allow &&allow,
+                                                       ScreenRec rec) {
+    lds_u2 *const list = (lds_u2 *)bp_flat_lds + threadIdx.x;  // entry e at list[e * stride]
+    const int stride = (int)blockDim.x;
+    const uint32_t zc[3] = {(uint32_t)(g[2] - 1) & G.mask[2], (uint32_t)g[2] & G.mask[2],
+                            (uint32_t)(g[2] + 1) & G.mask[2]};
+    const bool brk01 = zc[1] != zc[0] + 1u, brk12 = zc[2] != zc[1] + 1u;
+    const bool adjacent = !brk01 && !brk12;
+    // cells [0, kw) and [kw, 3) of a column are one run each (kw = 3: no wrap)
+    const int kw = brk01 ? 1 : (brk12 ? 2 : 3);
+    auto column = [&](int ci) {
+        const int dy = ci / 3 - 1, dx = ci % 3 - 1;
+        return G.base + ((uint32_t)(g[0] + dx) & G.mask[0]) * G.stride_x +
+               ((uint32_t)(g[1] + dy) & G.mask[1]) * (uint32_t)G.dims[2];
+    };
+    auto load_column = [&](uint32_t col, int2 r[3]) {
+        if (adjacent) {
+            const int4_a8 a = *reinterpret_cast<const int4_a8 *>(ranges + col + zc[0]);
+            r[0] = make_int2(a.x, a.y);
+            r[1] = make_int2(a.z, a.w);
+            r[2] = ranges[col + zc[2]];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) r[q] = ranges[col + zc[q]];
+        }
+    };
+    // a column's runs, [s0, e0) then [s1, e1), from jmin on; an empty run has s >= e
+    int s0, e0, s1, e1;
+    auto runs_of = [&](const int2 r[3]) {
+        s0 = -1, e0 = -1, s1 = -1, e1 = -1;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            if (r[q].x >= 0) {
+                if (q < kw) {
+                    if (s0 < 0) s0 = r[q].x;
+                    e0 = r[q].y;
+                } else {
+                    if (s1 < 0) s1 = r[q].x;
+                    e1 = r[q].y;
+                }
+            }
+        }
+        s0 = s0 < jmin ? jmin : s0;
+        s1 = s1 < jmin ? jmin : s1;
+    };
+
+    // the drain: every listed partner, NB at a time
+    int cnt = 0;
+    auto drain = [&]() {
+        int e = 0, b = 0;
+        uint32_t m = 0u;
+        u2v nxt = cnt > 0 ? list[0] : u2v{0u, 0u};
+        while (__builtin_amdgcn_ballot_w64((m != 0u) | (e < cnt))) {
+            if ((m != 0u) | (e < cnt)) {
+                run_bodies<NB>(
+                    [&](int, bool &ok) {
+                        if ((m == 0u) & (e < cnt)) {
+                            b = (int)nxt.x;
+                            m = nxt.y;
+                            ++e;
+                            if (e < cnt) nxt = list[e * stride];
+                        }
+                        ok = m != 0u;
+                        const int j = b + __ffs(m) - 1;
+                        m &= m - 1u;
+                        return j;
+                    },
+                    fetch, use);
+            }
+        }
+        cnt = 0;
+    };
+
+    int c = 0;  // chunk number in walk order
+    uint32_t ahead = rec.in ? rec.in[rec.slot] : 0u;
+    int2 nr[3];
+    load_column(column(0), nr);
+    for (int ci = 0; ci < 9; ++ci) {
+        const int2 r[3] = {nr[0], nr[1], nr[2]};
+        if (ci + 1 < 9) load_column(column(ci + 1), nr);
+        runs_of(r);
+#pragma unroll 1
+        for (int rr = 0; rr < 2; ++rr) {
+            const int end = rr ? e1 : e0;
+            for (int base = rr ? s1 : s0; base < end; base += 32, ++c) {
+                const int cn = (end - base) < 32 ? (end - base) : 32;
+                const uint32_t valid = (cn >= 32) ? 0xffffffffu : ((1u << cn) - 1u);
+                const uint32_t keep = allow(base) & valid;
+                uint32_t mask = 0u;
+                if (rec.in && c < kRecChunks) {
+                    mask = ahead;
+                    if (c + 1 < kRecChunks) ahead = rec.in[(size_t)(c + 1) * rec.n + rec.slot];
+                } else if (keep != 0u || rec.out) {
+                    for (int t0 = 0; t0 < cn; t0 += 8) {
+                        float4 bb[8];
+                        if (rec.planes) {
+                            const float *__restrict__ px = rec.planes + (base + t0);
+                            float4_a4 q[6];
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) {
+                                q[2 * a] = *reinterpret_cast<const float4_a4 *>(px + (size_t)a * rec.pstride);
+                                q[2 * a + 1] = *reinterpret_cast<const float4_a4 *>(px + (size_t)a * rec.pstride + 4);
+                            }
+#pragma unroll
+                            for (int u = 0; u < 8; ++u)
+                                bb[u] = make_float4(q[u >> 2][u & 3], q[2 + (u >> 2)][u & 3], q[4 + (u >> 2)][u & 3],
+                                                    0.0f);
+                        } else {
+                            const float4 *__restrict__ p = xs + (base + t0);
+#pragma unroll
+                            for (int u = 0; u < 8; ++u) bb[u] = p[u];
+                        }
+                        uint32_t m8 = 0u;
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) m8 |= (uint32_t)pick(base + t0 + u, bb[u]) << u;
+                        mask |= m8 << t0;
+                    }
+                    mask &= valid;
+                    if (rec.out && c < kRecChunks) rec.out[(size_t)c * rec.n + rec.slot] = mask;
+                }
+                mask &= keep;
+                if (mask) {
+                    list[cnt * stride] = u2v{(uint32_t)base, mask};
+                    ++cnt;
+                }
+                if (__builtin_amdgcn_ballot_w64(cnt == kListCap) != 0ull) drain();
+            }
+        }
+    }
+    drain();
+}
+
+// The walk in its per-chunk (Flat = false) or flat form; same pairs, same order.
+template <bool Flat, int NB, class... A> __device__ __forceinline__ void walk_form(A &&...a) {
+    if constexpr (Flat)
+        for_each_neighbor_flat<NB>(static_cast<A &&>(a)...);
+    else
+        for_each_neighbor_nb<NB>(static_cast<A &&>(a)...);
 }
 
 // Walks the CSR entries [e0, e1) in order, eight at a time: the entry

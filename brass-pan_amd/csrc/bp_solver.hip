@@ -240,7 +240,8 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
             }
             TIMED(F_POSITION, launch_position(s->xs.cur(), s->lam_rec, sk, s->ranges, H.rec ? H.rec[it] : s->srec,
                                               s->fluid_stamp, s->stamp, s->sum_tmp, s->corr, s->xs.alt(),
-                                              s->x.cur(), s->v.cur(), N, sp(), E, s->blk_position, st));
+                                              s->x.cur(), s->v.cur(), N, sp(), E, s->blk_position,
+                                              s->flat_position, st));
             s->xs.flip();
             planes_done = E.planes_next;
             finished = E.finish;
@@ -276,7 +277,8 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
                                           finished && planes_ready ? 1 : 0, N, sp(), do_vort, do_ak, s->blk_post_a,
                                           st));
             TIMED(F_POST_B, launch_post_b(s->xs.cur(), s->v.cur(), s->omega, s->xom, s->normal, s->density, s->ranges,
-                                          s->ccell, s->srec, N, sp(), do_vort, do_ak, subdt, s->blk_post_b, st));
+                                          s->ccell, s->srec, N, sp(), do_vort, do_ak, subdt, s->blk_post_b,
+                                          s->flat_post_b, st));
         }
         if (P.xsph_c != 0.0f) {
             TIMED(F_XSPH, launch_xsph(s->xs.cur(), s->v.cur(), s->ranges, s->ccell, s->srec, s->v.alt(), N, sp(),
@@ -783,6 +785,14 @@ bp_status bp_solver_set_fusion(bp_solver *s, int enable) {
     return BP_OK;
 }
 
+bp_status bp_solver_set_walk(bp_solver *s, int position_flat, int vorticity_tension_flat) {
+    if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
+    s->flat_position = position_flat != 0;
+    s->flat_post_b = vorticity_tension_flat != 0;
+    s->drop_graph();
+    return BP_OK;
+}
+
 namespace {
 // GPU time of `count` substeps (HIP events on the handle's stream)
 bp_status time_substeps(bp_solver *s, int count, float dt, float *ms) {
@@ -807,8 +817,8 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
     const int cand[4] = {64, 128, 256, 512};
     const int fams[6] = {F_LAMBDA, F_POSITION, F_POST_A, F_POST_B, F_XSPH, F_PP};
     float best_ms[6];
-    int best_blk[6];
-    for (int f = 0; f < 6; ++f) { best_ms[f] = 1e30f; best_blk[f] = 256; }
+    int best_blk[6], best_flat[6];
+    for (int f = 0; f < 6; ++f) { best_ms[f] = 1e30f; best_blk[f] = 256; best_flat[f] = 1; }
     // the trials advance a copy of the state; the caller's state is restored after
     std::vector<float> keep_x(3 * (size_t)s->N), keep_v(3 * (size_t)s->N), keep_q(4 * (size_t)s->nr + 4),
         keep_cm(3 * (size_t)s->nr + 3);
@@ -820,8 +830,12 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
     s->use_graph = false;
     st = bp_solver_enable_kernel_timing(s, 1);
     if (st != BP_OK) return st;
-    for (int c = 0; c < 4; ++c) {
+    // every workgroup size with each walk form of the position and
+    // vorticity/tension passes (the other passes have one form)
+    for (int t = 0; t < 8; ++t) {
+        const int c = t >> 1, flat = t & 1;
         s->blk_lambda = s->blk_position = s->blk_post_a = s->blk_post_b = s->blk_xsph = s->blk_pp = cand[c];
+        s->flat_position = s->flat_post_b = flat;
         for (int f = 0; f < F_COUNT; ++f) { s->fam_ms[f] = 0.f; s->fam_n[f] = 0; }
         st = run_substeps(s, substeps_per_trial, dt);
         if (st == BP_OK) st = collect_timing(s);
@@ -830,7 +844,7 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
             const int fam = fams[f];
             if (s->fam_n[fam] == 0) continue;
             const float avg = s->fam_ms[fam] / (float)s->fam_n[fam];
-            if (avg < best_ms[f]) { best_ms[f] = avg; best_blk[f] = cand[c]; }
+            if (avg < best_ms[f]) { best_ms[f] = avg; best_blk[f] = cand[c]; best_flat[f] = flat; }
         }
     }
     s->blk_lambda = best_blk[0];
@@ -839,6 +853,8 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
     s->blk_post_b = best_blk[3];
     s->blk_xsph = best_blk[4];
     s->blk_pp = best_blk[5];
+    s->flat_position = best_flat[1];
+    s->flat_post_b = best_flat[3];
     (void)bp_solver_enable_kernel_timing(s, 0);
     // iteration fusion: the epilogue passes (next iteration's plane
     // collisions, the velocity/position update, the lambda pass's inputs) in
@@ -862,8 +878,9 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
         std::snprintf(report, (size_t)report_len,
                       "{\"fluid_lambda\": %d, \"fluid_position\": %d, \"omega_normal\": %d, "
                       "\"vorticity_tension\": %d, \"xsph\": %d, \"particle_particle\": %d, "
-                      "\"iteration_fusion\": %d}",
-                      best_blk[0], best_blk[1], best_blk[2], best_blk[3], best_blk[4], best_blk[5], s->fuse ? 1 : 0);
+                      "\"iteration_fusion\": %d, \"position_walk\": \"%s\", \"vorticity_tension_walk\": \"%s\"}",
+                      best_blk[0], best_blk[1], best_blk[2], best_blk[3], best_blk[4], best_blk[5], s->fuse ? 1 : 0,
+                      s->flat_position ? "flat" : "chunk", s->flat_post_b ? "flat" : "chunk");
     return BP_OK;
 }
 
