@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_lambda(const fl
         // one record per partner, so four partners per fetch fit the register budget
         for_each_neighbor_nb<4>(
             G, ranges, g, xs, srec ? 0 : i, [&](int, float4 b) { return dist2(pi, xyz(b)) <= K.h2; },
-            [&](int j) { return L.xm[j]; },  // (x*, m), m negated for a non-fluid partner
+            [&](int j) { return ld32(L.xm, j); },  // (x*, m), m negated for a non-fluid partner
             [&](int, const float4 &b) {
                 const float d2 = dist2(pi, xyz(b));
                 const float mj = fabsf(b.w);
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_lambda(const fl
 // so |delta| = |c| r and the clamp |delta| <= 5 becomes |c| <= 5 / r.
 __device__ __forceinline__ float pair_coef(float d2, float lam_sum, const SphConst &K, const FluidParams &F) {
     float sl = lam_sum;
-    if (!F.akinci) sl += -F.scorr_k * ipowf(K.poly6(d2) / K.wdq, F.scorr_n);
+    if (!F.akinci) sl += -F.scorr_k * ipowf(K.over_wdq(K.poly6(d2)), F.scorr_n);
     const float r = fsqrt(d2);
     const float ir = frcp(r);
     float c = sl * K.grad_coef(r, ir);
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, Flat ? 5 : 6) void k_posi
             // (fluid particles replay the lambda pass's masks; a solid screens for
             // fluid partners only)
             [&](int, float4 b) { return (dist2(pk, xyz(b)) <= K.h2) & (kf | (phase_of(b) <= 0)); },
-            [&](int j) { return Partner{xs[j], lam_rec[j]}; },
+            [&](int j) { return Partner{ld32(xs, j), ld32(lam_rec, j)}; },
             [&](int j, const Partner &P) {
                 const float4 b = P.b;
                 // coincident partners are skipped; a solid partner's pair is k's only if j > k
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_post_a(const fl
         for_each_neighbor_nb<kBodyBatch>(
             G, ranges, g, xs, 0,
             [&](int, float4 b) { return (any_phase | (phase_of(b) == ph)) & (dist2(pi, xyz(b)) <= K.h2); },
-            [&](int j) { return Partner{xs[j], v[j]}; },
+            [&](int j) { return Partner{ld32(xs, j), ld32(v, j)}; },
             [&](int, const Partner &P) {
                 const float4 b = P.b;
                 if (!(dist2(pi, xyz(b)) > 0.0f)) return;  // coincident partners are skipped
@@ -329,7 +329,7 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 1) void k_post_b(const fl
         walk_form<Flat, kBodyBatch>(
             G, ranges, g, xs, 0,
             [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
-            [&](int j) { return Partner{xom[j], second[j]}; },
+            [&](int j) { return Partner{ld32(xom, j), ld32(second, j)}; },
             [&](int, const Partner &P) {
                 const float4 b = P.xo;
                 if (!(dist2(pi, xyz(b)) > 0.0f)) return;  // coincident partners are skipped
@@ -394,7 +394,7 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_xsph(const floa
         for_each_neighbor_nb<kBodyBatch>(
             G, ranges, g, xs, 0,
             [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
-            [&](int j) { return Partner{xs[j], v[j]}; },
+            [&](int j) { return Partner{ld32(xs, j), ld32(v, j)}; },
             [&](int, const Partner &P) { vn += (xyz(P.v) - vi) * K.poly6(dist2(pi, xyz(P.b))); }, AllowNotSelf{i},
             rec);
         v_out[i] = f4(vi + c * vn, 0.0f);

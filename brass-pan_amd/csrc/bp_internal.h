@@ -65,6 +65,13 @@ __device__ __forceinline__ float ipowf(float x, int n) {
     return r;
 }
 __device__ __forceinline__ float4 f4(f3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+// p[j] addressed by a 32-bit byte offset from the (uniform) base, so the load
+// takes the scalar-base form with no 64-bit address arithmetic per lane.
+// Slot arrays hold at most kMaxSlots records (checked at create).
+constexpr int kMaxSlots = 1 << 27;
+template <class T> __device__ __forceinline__ T ld32(const T *__restrict__ p, int j) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + (uint32_t)j * (uint32_t)sizeof(T));
+}
 
 // ------------------------------------------------------------ parameters
 struct Planes {
@@ -112,9 +119,24 @@ struct GridDesc {
 // SPH kernel constants, SetKernelRadius (fluid.cuh:11-27), computed on the host.
 struct SphConst {
     float c1, c2, c3, c4, h, h2, halfh, wdq;
+    float inv_wdq;  // 1/wdq rounded to float, or 0 where over_wdq must divide
     __device__ __forceinline__ float poly6(float r2) const {
         float t = h2 - r2;
         return c1 * t * t * t;
+    }
+    // a / wdq (the sCorr ratio, fluid.cuh:208) for a = poly6(r^2) >= 0.  The
+    // product with the rounded reciprocal, corrected by one FMA step
+    // (Markstein), is the correctly rounded quotient when the residual does
+    // not underflow: it equals IEEE division for every float a in
+    // [2^-100, W(0)] at kernel radii 0.033 to 1 (a host check over all such
+    // floats); the first mismatches lie below 2^-104.  Smaller a, and a wdq
+    // outside [2^-20, 2^20] (inv_wdq = 0), take the division.
+    __device__ __forceinline__ float over_wdq(float a) const {
+        if ((inv_wdq != 0.0f) & (a >= 0x1p-100f)) {
+            const float q0 = a * inv_wdq;
+            return fmaf(fmaf(-q0, wdq, a), inv_wdq, q0);
+        }
+        return a / wdq;
     }
     __device__ __forceinline__ f3 grad(f3 v, float r2) const {
         float r = sqrtf(r2);
@@ -506,7 +528,7 @@ __device__ __forceinline__ void for_each_neighbor(const GridDesc &G, const int2 
                                                   Pick &&pick, Body &&body, Allow &&allow,
                                                   ScreenRec rec = ScreenRec{}) {
     for_each_neighbor_nb<1>(
-        G, ranges, g, xs, jmin, pick, [&](int j) { return xs[j]; }, body, allow, rec);
+        G, ranges, g, xs, jmin, pick, [&](int j) { return ld32(xs, j); }, body, allow, rec);
 }
 template <class Pick, class Body>
 __device__ __forceinline__ void for_each_neighbor(const GridDesc &G, const int2 *__restrict__ ranges,

@@ -81,6 +81,7 @@ void set_kernel_const(SphConst &K, float h, float scorr_dq) {
     const float r2 = powf(scorr_dq * h, 2.0f);
     const float t = K.h2 - r2;
     K.wdq = K.c1 * t * t * t;
+    K.inv_wdq = (K.wdq >= 0x1p-20f && K.wdq <= 0x1p20f) ? 1.0f / K.wdq : 0.0f;
 }
 
 // per-scene constants from the scene description and its parameters
@@ -433,8 +434,8 @@ bp_status create_impl(const bp_scene_desc *scs, const bp_params *params, int ns,
         total += scs[k].num_particles;
         cells_total += (long long)scs[k].grid_dims[0] * scs[k].grid_dims[1] * scs[k].grid_dims[2];
     }
-    if (total > (1LL << 30) || cells_total > (1LL << 30))
-        return fail(BP_ERR_CAPACITY, "at most 2^30 particles and 2^30 cells per solver");
+    if (total > kMaxSlots || cells_total > (1LL << 30))
+        return fail(BP_ERR_CAPACITY, "at most 2^27 particles and 2^30 cells per solver");
 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(BP_ERR_NO_DEVICE, "no HIP device visible");
