@@ -95,7 +95,7 @@ SIGNATURES = {
     "bp_solver_set_graph": (_S, [_P, C.c_int]),
     "bp_solver_autotune": (_S, [_P, C.c_float, C.c_int, C.c_char_p, C.c_int]),
     "bp_solver_set_fusion": (_S, [_P, C.c_int]),
-    "bp_solver_set_walk": (_S, [_P, C.c_int, C.c_int]),
+    "bp_solver_set_walk": (_S, [_P, C.c_int, C.c_int, C.c_int]),
     "bp_forward_checkpointed": (_S, [_P, C.c_int, C.c_float, C.c_int]),
     "bp_backward": (_S, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
                          C.POINTER(C.c_float), C.POINTER(C.c_float)]),
@@ -246,10 +246,11 @@ class Solver:
         """Iteration fusion on or off (bp_solver_set_fusion; results are bit-identical)."""
         _check(self.lib.bp_solver_set_fusion(self.h, 1 if on else 0))
 
-    def set_walk(self, position_flat: bool = True, vorticity_tension_flat: bool = True):
-        """Walk form of the position and vorticity/tension passes (bp_solver_set_walk;
-        results are bit-identical)."""
-        _check(self.lib.bp_solver_set_walk(self.h, 1 if position_flat else 0, 1 if vorticity_tension_flat else 0))
+    def set_walk(self, lambda_staged: bool = True, position_flat: bool = True, vorticity_tension_flat: bool = True):
+        """Walk forms of the lambda, position and vorticity/tension passes
+        (bp_solver_set_walk; results are bit-identical)."""
+        _check(self.lib.bp_solver_set_walk(self.h, int(bool(lambda_staged)), int(bool(position_flat)),
+                                           int(bool(vorticity_tension_flat))))
 
     def autotune(self, dt: float, substeps_per_trial: int = 2) -> dict:
         import json

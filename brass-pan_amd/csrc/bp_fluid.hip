@@ -24,8 +24,8 @@ __global__ __launch_bounds__(kBlock) void k_lambda_in(const float4 *__restrict__
 // reads for this particle as self and as partner, (lambda, m, current cell
 // packed, 0), so that pass fetches two words-of-four per partner (x* and this)
 // instead of four separate arrays; it searches from the same x* and cell.
-template <class SP>
-__global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_lambda(const float4 *__restrict__ xs, const float *__restrict__ m,
+template <class SP, bool Staged>
+__global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, Staged ? 5 : 6) void k_lambda(const float4 *__restrict__ xs, const float *__restrict__ m,
                                                    const int2 *__restrict__ ranges, float *__restrict__ lambda,
                                                    float *__restrict__ density, float4 *__restrict__ lam_rec,
                                                    uint32_t *__restrict__ srec, LambdaIn L, int n, SP sp) {
@@ -58,11 +58,8 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_lambda(const fl
         rec.n = n;
         rec.planes = L.planes;  // the screen tests distance only
         rec.pstride = L.pstride;
-        // one record per partner, so four partners per fetch fit the register budget
-        for_each_neighbor_nb<4>(
-            G, ranges, g, xs, srec ? 0 : i, [&](int, float4 b) { return dist2(pi, xyz(b)) <= K.h2; },
-            [&](int j) { return ld32(L.xm, j); },  // (x*, m), m negated for a non-fluid partner
-            [&](int, const float4 &b) {
+        auto pick = [&](int, float4 b) { return dist2(pi, xyz(b)) <= K.h2; };
+        auto body = [&](int, const float4 &b) {
                 const float d2 = dist2(pi, xyz(b));
                 const float mj = fabsf(b.w);
                 const float ld = mj * K.poly6(d2);
@@ -75,8 +72,15 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_lambda(const fl
                     sumg2 += gc * gc * d2;
                     gi -= gr;
                 }
-            },
-            [&](int base) { return ~chunk_below(i, base); }, rec);
+        };
+        auto allow = [&](int base) { return ~chunk_below(i, base); };
+        // one record per partner, so four partners per fetch fit the register
+        // budget; the partner record is (x*, m), m negated for a non-fluid partner
+        if constexpr (Staged)
+            for_each_neighbor_staged<4>(G, ranges, g, L.xm, srec ? 0 : i, pick, body, allow, rec);
+        else
+            for_each_neighbor_nb<4>(
+                G, ranges, g, xs, srec ? 0 : i, pick, [&](int j) { return ld32(L.xm, j); }, body, allow, rec);
         if (F.lambda_mode == 1) sumg2 += dot(gi, gi);
         float C = dens / F.rest_density - 1.0f;
         if (F.akinci) C = fmaxf(C, 0.0f);
@@ -405,10 +409,14 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_xsph(const floa
 template <class SP>
 void launch_lambda(const float4 *xs, const float *m, const int2 *ranges, float *lambda, float *density,
                    float4 *lam_rec, uint32_t *srec, const LambdaIn &L, int lin_ready, int n, const SP &sp,
-                   int block, hipStream_t st) {
+                   int block, int staged, hipStream_t st) {
     if (!lin_ready) hipLaunchKernelGGL(k_lambda_in, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, m, L, n);
-    hipLaunchKernelGGL(k_lambda<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, m, ranges, lambda, density,
-                       lam_rec, srec, L, n, sp);
+    if (staged)
+        hipLaunchKernelGGL((k_lambda<SP, true>), dim3(grid_blocks(n, block)), dim3(block), staged_lds_bytes(block), st,
+                           xs, m, ranges, lambda, density, lam_rec, srec, L, n, sp);
+    else
+        hipLaunchKernelGGL((k_lambda<SP, false>), dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, m, ranges,
+                           lambda, density, lam_rec, srec, L, n, sp);
 }
 template <class SP>
 void launch_position(const float4 *xs, const float4 *lam_rec, const int *build_keys, const int2 *ranges,
@@ -454,7 +462,7 @@ void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const ui
 
 #define BP_INSTANTIATE_FLUID(SP)                                                                                \
     template void launch_lambda<SP>(const float4 *, const float *, const int2 *, float *, float *, float4 *,    \
-                                    uint32_t *, const LambdaIn &, int, int, const SP &, int, hipStream_t);      \
+                                    uint32_t *, const LambdaIn &, int, int, const SP &, int, int, hipStream_t); \
     template void launch_position<SP>(const float4 *, const float4 *, const int *, const int2 *,                \
                                       const uint32_t *, const uint32_t *, const uint32_t *, float4 *,           \
                                       unsigned long long *, float4 *, float4 *, float4 *, int, const SP &,      \

@@ -15,7 +15,7 @@ constexpr int kMaxBlock = 512;   // neighbour passes: block size chosen at run t
 // of SP::kWaves and the pass's own W; 6 = at most 80 VGPRs, 5 = 96).  The
 // passes wait on memory, so occupancy pays where the walk fits the budget
 // with few or no spills.  1 leaves the register budget open.
-#define kPassWaves(SP, W) __attribute__((amdgpu_waves_per_eu(SP::kWaves < W ? SP::kWaves : W)))
+#define kPassWaves(SP, W) __attribute__((amdgpu_waves_per_eu((SP::kWaves < (W)) ? SP::kWaves : (W))))
 
 // ------------------------------------------------------------ small vectors
 // Own 3-vector type: component-wise operators with the left-to-right order of
@@ -688,6 +688,223 @@ __device__ __forceinline__ void for_each_neighbor_flat(const GridDesc &G, const 
         }
     }
     drain();
+}
+
+// ------------------------------------------------------------ staged walk
+// The per-chunk walk with its candidates read from LDS.  A wave's lanes hold
+// consecutive sorted slots, so they search few distinct (x, y) cell columns
+// and overlapping z windows: for each of the 9 column offsets, the union of
+// the lanes' candidate runs is a few contiguous slot segments (one per group
+// of lanes sharing a current column).  The wave loads those segments' records
+// into its LDS once, with coalesced loads spread over its lanes, and each lane
+// then screens its runs and fetches its partners from LDS, in the walk's
+// order: same pairs, same summation order, same results.  A column whose
+// segments exceed kStageCap records, and a walk whose lanes fall in more than
+// kStageGroups columns or whose z window wraps, reads global memory as the
+// per-chunk walk does (the x* planes for the screen, `src` for the bodies).
+// Launch with staged_lds_bytes(block) bytes of dynamic LDS.
+constexpr int kStageCap = 256;   // records per wave and column offset
+constexpr int kStageGroups = 4;  // lane groups (distinct current columns) per wave
+constexpr int kStagePad = 8;     // records after the region: a screen batch reads up to 7 past a run
+inline size_t staged_lds_bytes(int block) { return (size_t)(block / 64) * (kStageCap + kStagePad) * 16; }
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) f4v lds_f4;
+__device__ __forceinline__ float4 as_float4(f4v v) { return make_float4(v.x, v.y, v.z, v.w); }
+__device__ __forceinline__ f4v as_f4v(float4 a) { return f4v{a.x, a.y, a.z, a.w}; }
+
+template <int NB, class Pick, class Use, class Allow>
+__device__ __forceinline__ void for_each_neighbor_staged(const GridDesc &G, const int2 *__restrict__ ranges,
+                                                         const int g[3], const float4 *__restrict__ src, int jmin,
+                                                         Pick &&pick, Use &&use, Allow &&allow, ScreenRec rec) {
+    lds_f4 *const lds = (lds_f4 *)bp_flat_lds + (threadIdx.x >> 6) * (kStageCap + kStagePad);
+    const uint64_t act = __builtin_amdgcn_ballot_w64(true);
+    const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
+    const int nact = __builtin_popcountll(act);
+    // ---- lane groups: lanes sharing a current (x, y) column.  A group's z
+    // window spans its first and last lanes' z cells +- 2 (every lane of the
+    // group must lie within one cell of those two, else no staging).  Its wn
+    // cells' ranges are loaded by the active lanes of ranks wl .. wl + wn - 1
+    // (lane mask wmask).
+    const uint32_t ckey = (((uint32_t)g[0] & G.mask[0]) << 10) | ((uint32_t)g[1] & G.mask[1]);
+    uint64_t wmask[kStageGroups];
+    int ng = 0, wl = 0, grp = 0;
+    int wq_cx = 0, wq_cy = 0, wq_z = 0;  // the window cell this lane loads (wq)
+    bool wq = false, stage = true;
+    uint64_t todo = act;
+#pragma unroll
+    for (int q = 0; q < kStageGroups; ++q) {
+        wmask[q] = 0ull;
+        if (todo == 0ull || !stage) continue;
+        const int lead = __builtin_ctzll(todo);
+        const uint32_t k0 = __builtin_amdgcn_readlane(ckey, lead);
+        const bool mine = ckey == k0;
+        const uint64_t m = __builtin_amdgcn_ballot_w64(mine);
+        const int za = __builtin_amdgcn_readlane(g[2], __builtin_ctzll(m));
+        const int zb = __builtin_amdgcn_readlane(g[2], 63 - __builtin_clzll(m));
+        const int zlo = (za < zb ? za : zb) - 1, zhi = (za < zb ? zb : za) + 1;
+        const int wn = zhi - zlo + 3;  // window cells zlo - 1 .. zhi + 1
+        const uint32_t wz0 = (uint32_t)(zlo - 1) & G.mask[2];
+        const bool outside = mine && (g[2] < zlo || g[2] > zhi);
+        if (__builtin_amdgcn_ballot_w64(outside) != 0ull || wl + wn > nact ||
+            wz0 + (uint32_t)wn > (uint32_t)G.dims[2]) {
+            stage = false;
+            continue;
+        }
+        if (mine) grp = q;
+        const bool inq = rank >= wl && rank < wl + wn;
+        wmask[q] = __builtin_amdgcn_ballot_w64(inq);
+        if (inq) {
+            wq = true;
+            wq_cx = (int)(k0 >> 10);
+            wq_cy = (int)(k0 & 1023u);
+            wq_z = (int)wz0 + (rank - wl);
+        }
+        wl += wn;
+        ng = q + 1;
+        todo &= ~m;
+    }
+    if (todo != 0ull) stage = false;  // more columns than groups
+
+    const uint32_t zc[3] = {(uint32_t)(g[2] - 1) & G.mask[2], (uint32_t)g[2] & G.mask[2],
+                            (uint32_t)(g[2] + 1) & G.mask[2]};
+    const bool brk01 = zc[1] != zc[0] + 1u, brk12 = zc[2] != zc[1] + 1u;
+    const bool adjacent = !brk01 && !brk12;
+    auto column = [&](int ci) {
+        const int dy = ci / 3 - 1, dx = ci % 3 - 1;
+        return G.base + ((uint32_t)(g[0] + dx) & G.mask[0]) * G.stride_x +
+               ((uint32_t)(g[1] + dy) & G.mask[1]) * (uint32_t)G.dims[2];
+    };
+    auto load_column = [&](uint32_t col, int2 r[3]) {
+        if (adjacent) {
+            const int4_a8 a = *reinterpret_cast<const int4_a8 *>(ranges + col + zc[0]);
+            r[0] = make_int2(a.x, a.y);
+            r[1] = make_int2(a.z, a.w);
+            r[2] = ranges[col + zc[2]];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) r[q] = ranges[col + zc[q]];
+        }
+    };
+    int c = 0;  // chunk number in walk order
+    uint32_t ahead = rec.in ? rec.in[rec.slot] : 0u;
+    int2 nr[3];
+    load_column(column(0), nr);
+    for (int ci = 0; ci < 9; ++ci) {
+        const int dy = ci / 3 - 1, dx = ci % 3 - 1;
+        const int2 r[3] = {nr[0], nr[1], nr[2]};
+        if (ci + 1 < 9) load_column(column(ci + 1), nr);
+        // ---- this column offset's segments, staged when they fit
+        bool staged = false;
+        int ldsbase = 0;  // LDS index of slot j: j + ldsbase (this lane's group)
+        if (stage) {
+            int2 wr = make_int2(-1, -1);
+            if (wq)
+                wr = ranges[G.base + ((uint32_t)(wq_cx + dx) & G.mask[0]) * G.stride_x +
+                            ((uint32_t)(wq_cy + dy) & G.mask[1]) * (uint32_t)G.dims[2] + (uint32_t)wq_z];
+            const uint64_t ne = __builtin_amdgcn_ballot_w64(wr.x >= 0);
+            int tot = 0, seg0[kStageGroups], sbase[kStageGroups];
+#pragma unroll
+            for (int q = 0; q < kStageGroups; ++q) {
+                seg0[q] = tot;
+                sbase[q] = 0;
+                if (q < ng) {
+                    const uint64_t n1 = ne & wmask[q];
+                    const int s1 = n1 ? __builtin_amdgcn_readlane(wr.x, __builtin_ctzll(n1)) : 0;
+                    const int e1 = n1 ? __builtin_amdgcn_readlane(wr.y, 63 - __builtin_clzll(n1)) : 0;
+                    sbase[q] = tot - s1;  // LDS index = slot + sbase
+                    tot += e1 - s1;
+                }
+            }
+            if (tot <= kStageCap) {
+                staged = true;
+                for (int t = rank; t < tot; t += nact) {
+                    int b = sbase[0];
+#pragma unroll
+                    for (int q = 1; q < kStageGroups; ++q)
+                        if (q < ng && t >= seg0[q]) b = sbase[q];
+                    lds[t] = as_f4v(ld32(src, t - b));
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+                for (int q = 0; q < kStageGroups; ++q)
+                    if (q == grp) ldsbase = sbase[q];
+            }
+        }
+        // ---- this lane's runs of the column, chunk by chunk (as walk_chunks)
+        auto run = [&](int s, int e) {
+            if (s < jmin) s = jmin;
+            for (int base = s; base < e; base += 32, ++c) {
+                const int cnt = (e - base) < 32 ? (e - base) : 32;
+                const uint32_t valid = (cnt >= 32) ? 0xffffffffu : ((1u << cnt) - 1u);
+                const uint32_t keep = allow(base) & valid;
+                uint32_t mask = 0u;
+                if (rec.in && c < kRecChunks) {
+                    mask = ahead;
+                    if (c + 1 < kRecChunks) ahead = rec.in[(size_t)(c + 1) * rec.n + rec.slot];
+                    if (keep == 0u) continue;
+                } else {
+                    if (keep == 0u && !rec.out) continue;
+                    for (int t0 = 0; t0 < cnt; t0 += 8) {
+                        float4 bb[8];
+                        if (staged) {
+                            const lds_f4 *q = lds + (base + t0 + ldsbase);
+#pragma unroll
+                            for (int u = 0; u < 8; ++u) bb[u] = as_float4(q[u]);
+                        } else {
+                            const float *__restrict__ px = rec.planes + (base + t0);
+                            float4_a4 q[6];
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) {
+                                q[2 * a] = *reinterpret_cast<const float4_a4 *>(px + (size_t)a * rec.pstride);
+                                q[2 * a + 1] = *reinterpret_cast<const float4_a4 *>(px + (size_t)a * rec.pstride + 4);
+                            }
+#pragma unroll
+                            for (int u = 0; u < 8; ++u)
+                                bb[u] = make_float4(q[u >> 2][u & 3], q[2 + (u >> 2)][u & 3], q[4 + (u >> 2)][u & 3],
+                                                    0.0f);
+                        }
+                        uint32_t m8 = 0u;
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) m8 |= (uint32_t)pick(base + t0 + u, bb[u]) << u;
+                        mask |= m8 << t0;
+                    }
+                    mask &= valid;
+                    if (rec.out && c < kRecChunks) rec.out[(size_t)c * rec.n + rec.slot] = mask;
+                }
+                mask &= keep;
+                while (mask) {
+                    run_bodies<NB>(
+                        [&](int, bool &ok) {
+                            ok = mask != 0u;
+                            const int j = base + __ffs(mask) - 1;
+                            mask &= mask - 1u;
+                            return j;
+                        },
+                        [&](int j) { return staged ? as_float4(lds[j + ldsbase]) : ld32(src, j); }, use);
+                }
+            }
+        };
+        int s = -1, e = -1;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            if ((q == 1 && brk01) || (q == 2 && brk12)) {
+                if (s >= 0) run(s, e);
+                s = -1;
+            }
+            if (r[q].x >= 0) {
+                if (s < 0) s = r[q].x;
+                e = r[q].y;
+            }
+        }
+        if (s >= 0) run(s, e);
+        if (staged) {  // every lane is done with the region before the next fill
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
 }
 
 // The walk in its per-chunk (Flat = false) or flat form; same pairs, same order.

@@ -31,7 +31,7 @@ void launch_reorder(const int *perm, const float4 *x_in, const float4 *v_in, con
 template <class SP>
 void launch_lambda(const float4 *xs, const float *m, const int2 *ranges, float *lambda, float *density,
                    float4 *lam_rec, uint32_t *srec, const LambdaIn &L, int lin_ready, int n, const SP &sp,
-                   int block, hipStream_t st);
+                   int block, int staged, hipStream_t st);
 template <class SP>
 void launch_position(const float4 *xs, const float4 *lam_rec, const int *build_keys, const int2 *ranges,
                      const uint32_t *srec, const uint32_t *fluid_stamp, const uint32_t *stamp, float4 *sum_tmp,

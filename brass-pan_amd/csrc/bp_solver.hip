@@ -225,7 +225,7 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
         if (s->has_fluid) {
             TIMED(F_LAMBDA, launch_lambda(s->xs.cur(), s->m.cur(), s->ranges, s->lambda, s->density, s->lam_rec,
                                           H.rec ? H.rec[it] : s->srec, s->lin, planes_ready ? 1 : 0, N, sp(),
-                                          s->blk_lambda, st));
+                                          s->blk_lambda, s->staged_lambda, st));
             if (H.lam) HIP_TRY(snap(H.lam[it], s->lambda, N, st));
             Epilogue E{};
             E.planes_next = (!no_fuse && s->any_planes && it + 1 < iters && !tail_groups && !pick) ? 1 : 0;
@@ -786,8 +786,9 @@ bp_status bp_solver_set_fusion(bp_solver *s, int enable) {
     return BP_OK;
 }
 
-bp_status bp_solver_set_walk(bp_solver *s, int position_flat, int vorticity_tension_flat) {
+bp_status bp_solver_set_walk(bp_solver *s, int lambda_staged, int position_flat, int vorticity_tension_flat) {
     if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
+    s->staged_lambda = lambda_staged != 0;
     s->flat_position = position_flat != 0;
     s->flat_post_b = vorticity_tension_flat != 0;
     s->drop_graph();
@@ -831,12 +832,12 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
     s->use_graph = false;
     st = bp_solver_enable_kernel_timing(s, 1);
     if (st != BP_OK) return st;
-    // every workgroup size with each walk form of the position and
+    // every workgroup size with each walk form of the lambda, position and
     // vorticity/tension passes (the other passes have one form)
     for (int t = 0; t < 8; ++t) {
         const int c = t >> 1, flat = t & 1;
         s->blk_lambda = s->blk_position = s->blk_post_a = s->blk_post_b = s->blk_xsph = s->blk_pp = cand[c];
-        s->flat_position = s->flat_post_b = flat;
+        s->staged_lambda = s->flat_position = s->flat_post_b = flat;
         for (int f = 0; f < F_COUNT; ++f) { s->fam_ms[f] = 0.f; s->fam_n[f] = 0; }
         st = run_substeps(s, substeps_per_trial, dt);
         if (st == BP_OK) st = collect_timing(s);
@@ -854,6 +855,7 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
     s->blk_post_b = best_blk[3];
     s->blk_xsph = best_blk[4];
     s->blk_pp = best_blk[5];
+    s->staged_lambda = best_flat[0];
     s->flat_position = best_flat[1];
     s->flat_post_b = best_flat[3];
     (void)bp_solver_enable_kernel_timing(s, 0);
@@ -879,9 +881,11 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
         std::snprintf(report, (size_t)report_len,
                       "{\"fluid_lambda\": %d, \"fluid_position\": %d, \"omega_normal\": %d, "
                       "\"vorticity_tension\": %d, \"xsph\": %d, \"particle_particle\": %d, "
-                      "\"iteration_fusion\": %d, \"position_walk\": \"%s\", \"vorticity_tension_walk\": \"%s\"}",
+                      "\"iteration_fusion\": %d, \"lambda_walk\": \"%s\", \"position_walk\": \"%s\", "
+                      "\"vorticity_tension_walk\": \"%s\"}",
                       best_blk[0], best_blk[1], best_blk[2], best_blk[3], best_blk[4], best_blk[5], s->fuse ? 1 : 0,
-                      s->flat_position ? "flat" : "chunk", s->flat_post_b ? "flat" : "chunk");
+                      s->staged_lambda ? "staged" : "global", s->flat_position ? "flat" : "chunk",
+                      s->flat_post_b ? "flat" : "chunk");
     return BP_OK;
 }
 

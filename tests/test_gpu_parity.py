@@ -364,11 +364,12 @@ def test_iteration_fusion_is_bit_identical(label):
 
 @pytest.mark.parametrize("label", ["akinci block 8k", "mixed phase small"])
 def test_walk_forms_are_bit_identical(label):
-    """The flat walk (pair terms after the whole candidate walk) and the
-    per-chunk walk of the position and vorticity/tension passes visit the same
-    pairs in the same order: same bits over several substeps, on a scene that
-    runs vorticity confinement and Akinci tension and on one with solids,
-    cloth and wind (deliveries to partners whose search misses the pair)."""
+    """The walk forms visit the same pairs in the same order: the lambda pass
+    with its candidates staged in LDS or read from global memory, the position
+    and vorticity/tension passes flat (pair terms after the whole candidate
+    walk) or per chunk.  Same bits over several substeps, on a scene that runs
+    vorticity confinement and Akinci tension and on one with solids, cloth and
+    wind (deliveries to partners whose search misses the pair)."""
     if label == "akinci block 8k":
         sc, prm = scenes.large_block(dims=(32, 16, 16), grid=(256, 128, 128)), scenes.large_block_params()
     else:
@@ -376,8 +377,8 @@ def test_walk_forms_are_bit_identical(label):
         prm = scenes.mixed_phase_params()
     case(f"walk forms: {label}")
     a, b = bp.Solver(sc, prm), bp.Solver(sc, prm)
-    a.set_walk(True, True)
-    b.set_walk(False, False)
+    a.set_walk(True, True, True)
+    b.set_walk(False, False, False)
     a.step(4, 4 * DT)
     b.step(4, 4 * DT)
     pa, va, ma = a.get_state()
