@@ -76,11 +76,15 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, Staged ? 5 : 6) void k_la
         auto allow = [&](int base) { return ~chunk_below(i, base); };
         // one record per partner, so four partners per fetch fit the register
         // budget; the partner record is (x*, m), m negated for a non-fluid partner
-        if constexpr (Staged)
-            for_each_neighbor_staged<4>(G, ranges, g, L.xm, srec ? 0 : i, pick, body, allow, rec);
-        else
+        if constexpr (Staged) {
+            const float4 *const srcs[1] = {L.xm};
+            for_each_neighbor_staged<4, 1>(
+                G, ranges, g, srcs, srec ? 0 : i, pick, [&](int j, const StagedRec<1> &R) { body(j, R.r[0]); },
+                allow, rec);
+        } else {
             for_each_neighbor_nb<4>(
                 G, ranges, g, xs, srec ? 0 : i, pick, [&](int j) { return ld32(L.xm, j); }, body, allow, rec);
+        }
         if (F.lambda_mode == 1) sumg2 += dot(gi, gi);
         float C = dens / F.rest_density - 1.0f;
         if (F.akinci) C = fmaxf(C, 0.0f);

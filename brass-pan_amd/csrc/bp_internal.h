@@ -701,22 +701,32 @@ __device__ __forceinline__ void for_each_neighbor_flat(const GridDesc &G, const 
 // order: same pairs, same summation order, same results.  A column whose
 // segments exceed kStageCap records, and a walk whose lanes fall in more than
 // kStageGroups columns or whose z window wraps, reads global memory as the
-// per-chunk walk does (the x* planes for the screen, `src` for the bodies).
-// Launch with staged_lds_bytes(block) bytes of dynamic LDS.
+// per-chunk walk does (the x* planes, or srcs[0], for the screen; srcs for
+// the bodies).  srcs[0] holds x* (the screen's records); the pair body gets
+// srcs[0..NR-1][j].  Launch with staged_lds_bytes(block, NR) bytes of
+// dynamic LDS.
 constexpr int kStageCap = 256;   // records per wave and column offset
 constexpr int kStageGroups = 4;  // lane groups (distinct current columns) per wave
 constexpr int kStagePad = 8;     // records after the region: a screen batch reads up to 7 past a run
-inline size_t staged_lds_bytes(int block) { return (size_t)(block / 64) * (kStageCap + kStagePad) * 16; }
+// nr staged arrays per wave
+inline size_t staged_lds_bytes(int block, int nr = 1) {
+    return (size_t)(block / 64) * nr * (kStageCap + kStagePad) * 16;
+}
+// The records a staged walk's pair body receives: srcs[0..NR-1][j]
+template <int NR> struct StagedRec {
+    float4 r[NR];
+};
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) f4v lds_f4;
 __device__ __forceinline__ float4 as_float4(f4v v) { return make_float4(v.x, v.y, v.z, v.w); }
 __device__ __forceinline__ f4v as_f4v(float4 a) { return f4v{a.x, a.y, a.z, a.w}; }
 
-template <int NB, class Pick, class Use, class Allow>
+template <int NB, int NR, class Pick, class Use, class Allow>
 __device__ __forceinline__ void for_each_neighbor_staged(const GridDesc &G, const int2 *__restrict__ ranges,
-                                                         const int g[3], const float4 *__restrict__ src, int jmin,
+                                                         const int g[3], const float4 *const (&srcs)[NR], int jmin,
                                                          Pick &&pick, Use &&use, Allow &&allow, ScreenRec rec) {
-    lds_f4 *const lds = (lds_f4 *)bp_flat_lds + (threadIdx.x >> 6) * (kStageCap + kStagePad);
+    constexpr int kRegion = kStageCap + kStagePad;
+    lds_f4 *const lds = (lds_f4 *)bp_flat_lds + (threadIdx.x >> 6) * (NR * kRegion);
     const uint64_t act = __builtin_amdgcn_ballot_w64(true);
     const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
     const int nact = __builtin_popcountll(act);
@@ -769,6 +779,7 @@ __device__ __forceinline__ void for_each_neighbor_staged(const GridDesc &G, cons
                             (uint32_t)(g[2] + 1) & G.mask[2]};
     const bool brk01 = zc[1] != zc[0] + 1u, brk12 = zc[2] != zc[1] + 1u;
     const bool adjacent = !brk01 && !brk12;
+    const int kw = brk01 ? 1 : (brk12 ? 2 : 3);
     auto column = [&](int ci) {
         const int dy = ci / 3 - 1, dx = ci % 3 - 1;
         return G.base + ((uint32_t)(g[0] + dx) & G.mask[0]) * G.stride_x +
@@ -822,7 +833,8 @@ __device__ __forceinline__ void for_each_neighbor_staged(const GridDesc &G, cons
 #pragma unroll
                     for (int q = 1; q < kStageGroups; ++q)
                         if (q < ng && t >= seg0[q]) b = sbase[q];
-                    lds[t] = as_f4v(ld32(src, t - b));
+#pragma unroll
+                    for (int a = 0; a < NR; ++a) lds[a * kRegion + t] = as_f4v(ld32(srcs[a], t - b));
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
@@ -852,6 +864,10 @@ __device__ __forceinline__ void for_each_neighbor_staged(const GridDesc &G, cons
                             const lds_f4 *q = lds + (base + t0 + ldsbase);
 #pragma unroll
                             for (int u = 0; u < 8; ++u) bb[u] = as_float4(q[u]);
+                        } else if (!rec.planes) {
+                            const float4 *__restrict__ p = srcs[0] + (base + t0);
+#pragma unroll
+                            for (int u = 0; u < 8; ++u) bb[u] = p[u];
                         } else {
                             const float *__restrict__ px = rec.planes + (base + t0);
                             float4_a4 q[6];
@@ -882,23 +898,36 @@ __device__ __forceinline__ void for_each_neighbor_staged(const GridDesc &G, cons
                             mask &= mask - 1u;
                             return j;
                         },
-                        [&](int j) { return staged ? as_float4(lds[j + ldsbase]) : ld32(src, j); }, use);
+                        [&](int j) {
+                            StagedRec<NR> R;
+#pragma unroll
+                            for (int a = 0; a < NR; ++a)
+                                R.r[a] = staged ? as_float4(lds[a * kRegion + j + ldsbase]) : ld32(srcs[a], j);
+                            return R;
+                        },
+                        use);
                 }
             }
         };
-        int s = -1, e = -1;
+        // the column's runs: cells [0, kw) and [kw, 3) (kw = 3 without a z wrap)
+        int s0 = -1, e0 = -1, s1 = -1, e1 = -1;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            if ((q == 1 && brk01) || (q == 2 && brk12)) {
-                if (s >= 0) run(s, e);
-                s = -1;
-            }
             if (r[q].x >= 0) {
-                if (s < 0) s = r[q].x;
-                e = r[q].y;
+                if (q < kw) {
+                    if (s0 < 0) s0 = r[q].x;
+                    e0 = r[q].y;
+                } else {
+                    if (s1 < 0) s1 = r[q].x;
+                    e1 = r[q].y;
+                }
             }
         }
-        if (s >= 0) run(s, e);
+#pragma unroll 1
+        for (int rr = 0; rr < 2; ++rr) {
+            const int rs = rr ? s1 : s0, re = rr ? e1 : e0;
+            if (rs >= 0) run(rs, re);
+        }
         if (staged) {  // every lane is done with the region before the next fill
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
