@@ -58,11 +58,33 @@ def build_workload(name):
     return sc, pbuilder(), dt / substeps
 
 
-def time_plain(sc, prm, subdt, steps, warmup):
-    """Workgroup sizes from bp_solver_autotune (which restores the state it
-    started from), then `warmup` untimed and `steps` timed substeps."""
+TUNE_FILE = None  # --tune-file: record the first tuning of each workload, replay it in later runs
+
+
+def tune(s, subdt, key):
+    """bp_solver_autotune (which restores the state it started from), or, with
+    --tune-file, the choices a previous run recorded there for `key`, so that
+    the runs of one profiling session time the same kernel forms."""
+    saved = {}
+    if TUNE_FILE and os.path.exists(TUNE_FILE):
+        with open(TUNE_FILE) as f:
+            saved = json.load(f)
+    if key in saved:
+        s.apply_tuning(saved[key])
+        return saved[key]
+    report = s.autotune(subdt, 2)
+    if TUNE_FILE:
+        saved[key] = report
+        with open(TUNE_FILE, "w") as f:
+            json.dump(saved, f)
+    return report
+
+
+def time_plain(sc, prm, subdt, steps, warmup, key="plain"):
+    """Workgroup sizes and kernel forms from the autotuner (tune()), then
+    `warmup` untimed and `steps` timed substeps."""
     s = bp.Solver(sc, prm)
-    s.block_sizes = s.autotune(subdt, 2)
+    s.block_sizes = tune(s, subdt, key)
     for _ in range(warmup):
         s.step(1, subdt)
     s.sync()
@@ -149,14 +171,14 @@ def batched_headline(scs, prms, steps, warmup, graphs=True):
     from the handle's hipGraph."""
     subdt = BATCH_DT / BATCH_SUB
     bs = bp.BatchSolver(scs, prms)
-    tune = bs.autotune(subdt, 2)
+    choice = tune(bs, subdt, "batched_512_scenes")
     bs.set_graph(graphs)
     run_substeps(bs, warmup, subdt)
     bs.sync()
     t0 = time.perf_counter()
     run_substeps(bs, steps, subdt)
     bs.sync()
-    return bs, tune, time.perf_counter() - t0
+    return bs, choice, time.perf_counter() - t0
 
 
 def plain_loop(scs, prms, frames=3, graphs=True):
@@ -204,7 +226,7 @@ def time_batched(frames=3, warmup=1, num=512, graphs=True):
     """The batched leg alone (for profiling): the batch over `frames` frames
     and the plain loop over the same scenes."""
     scs, prms = scenes.batched_scenes(num)
-    bs, tune, wall = batched_headline(scs, prms, BATCH_SUB * frames, BATCH_SUB * warmup, graphs)
+    bs, choice, wall = batched_headline(scs, prms, BATCH_SUB * frames, BATCH_SUB * warmup, graphs)
     n = bs.n
     bs.close()
     ps = n * BATCH_SUB * frames
@@ -212,7 +234,7 @@ def time_batched(frames=3, warmup=1, num=512, graphs=True):
     return {"value": round(ps / wall, 1), "unit": "particle-substeps/s",
             "ms_per_substep": round(wall / (frames * BATCH_SUB) * 1e3, 3), "plain_loop": loop,
             "speedup_vs_plain_loop": round(ps / wall / max(loop["eager"], loop["graph"]), 2),
-            "config": {"workload": f"batched_{num}_scenes", "scenes": num, "particles": n, "block_sizes": tune}}
+            "config": {"workload": f"batched_{num}_scenes", "scenes": num, "particles": n, "block_sizes": choice}}
 
 
 def cpu_baseline_batch(scs, prms, subdt, budget_s=15.0, ref_budget_s=5.0):
@@ -340,7 +362,7 @@ def time_adjoint(num=64, graphs=True):
     ts.close()
 
     s = bp.BatchSolver(scs, prms)
-    tune = s.autotune(dt, 2)
+    choice = tune(s, dt, "adjoint_64_scenes")
     pos0, vel0, mp0 = s.get_state()
 
     def run_once():
@@ -378,7 +400,7 @@ def time_adjoint(num=64, graphs=True):
         "grad_theta_scene0": [float(x) for x in gth[0]], "grad_v0_norm": float(np.linalg.norm(gv0)),
         "config": {"workload": f"adjoint_{num}_scenes", "scenes": num, "particles": n, "substeps": K,
                    "substep_dt": dt, "checkpoint_every": every, "iterations": prms[0].iterations,
-                   "block_sizes": tune},
+                   "block_sizes": choice},
     }
 
 
@@ -429,38 +451,10 @@ def copy_calibration(mib):
     hip.hipFree(b)
 
 
-def elementwise_copy_bandwidth(mib=1024, reps=5):
-    """A second STREAM-style check of the HBM peak with a compute-kernel
-    copy (torch's vectorized elementwise copy) rather than the runtime's DMA
-    path: 1 GiB of float32, HIP events on torch's stream, best of `reps`,
-    bytes read plus bytes written.  None if torch is not importable."""
-    try:
-        import torch
-    except ImportError:
-        return None
-    if not torch.cuda.is_available():
-        return None
-    n = (mib << 20) // 4
-    a = torch.ones(n, device="cuda", dtype=torch.float32)
-    b = torch.empty_like(a)
-    b.copy_(a)
-    best = float("inf")
-    for _ in range(reps):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        b.copy_(a)
-        e1.record()
-        e1.synchronize()
-        best = min(best, e0.elapsed_time(e1))
-    del a, b
-    torch.cuda.empty_cache()
-    return {"bytes": 2 * (mib << 20), "ms": round(best, 4), "GBs": round(2 * (mib << 20) / (best * 1e-3) / 1e9, 1)}
-
-
 def plain_leg(name, steps, warmup, roof_substeps=5):
     """A plain-solver workload: timing, roofline of its dominant kernel."""
     sc, prm, subdt = build_workload(name)
-    s, wall = time_plain(sc, prm, subdt, steps, warmup)
+    s, wall = time_plain(sc, prm, subdt, steps, warmup, key=name)
     roof, times = kernel_roofline(s, subdt, substeps=roof_substeps, workload=name)
     out = {"value": round(s.n * steps / wall, 1), "unit": "particle-substeps/s",
            "ms_per_step": round(wall / steps * 1e3, 4), "steps": steps, "warmup": warmup, "particles": s.n,
@@ -484,9 +478,14 @@ def main():
                          "plain (--workload): run only that measurement and print its object (for profiling)")
     ap.add_argument("--workload", default="mixed_phase_283k", choices=sorted(scenes.WORKLOADS),
                     help="the plain workload of --leg plain")
+    ap.add_argument("--tune-file", default=None, metavar="JSON",
+                    help="record each workload's autotuner choices in JSON on the first run and reuse them on "
+                         "later runs (so the kernel trace and the counter runs of a profile time the same forms)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device (rocprof counter calibration)")
     args = ap.parse_args()
+    global TUNE_FILE
+    TUNE_FILE = args.tune_file
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU slab sharding is a design, not part of this build)", file=sys.stderr)
@@ -511,14 +510,14 @@ def main():
     scs, prms = scenes.batched_scenes(512)
     subdt = BATCH_DT / BATCH_SUB
     graphs = not args.no_graph
-    bs, tune, wall = batched_headline(scs, prms, args.steps, args.warmup, graphs)
+    bs, choice, wall = batched_headline(scs, prms, args.steps, args.warmup, graphs)
     n = bs.n
     value = n * args.steps / wall
     roof, fam_times = kernel_roofline(bs, subdt, substeps=2, workload="batched_512_scenes")
     bs.close()
     if args.leg == "headline":
         print(json.dumps({"batched_512_scenes": {"value": round(value, 1), "ms_per_step": round(wall / args.steps * 1e3, 4),
-                                                 "roofline": roof, "kernel_ms": fam_times, "block_sizes": tune}}),
+                                                 "roofline": roof, "kernel_ms": fam_times, "block_sizes": choice}}),
               flush=True)
         return
     loop = plain_loop(scs, prms, frames=3, graphs=graphs)
@@ -536,7 +535,7 @@ def main():
                    "distance_constraints": sum(len(sc.distance_pairs) for sc in scs),
                    "wind_faces": sum(len(sc.faces) for sc in scs),
                    "rigid_bodies": sum(len(sc.rigidbody_particle_id_ranges) for sc in scs),
-                   "graph_replay": graphs, "block_sizes": tune, "parallelism": "single GPU"},
+                   "graph_replay": graphs, "block_sizes": choice, "parallelism": "single GPU"},
         "roofline": roof,
         "kernel_ms": fam_times,
         "plain_loop": loop,
@@ -566,7 +565,8 @@ def main():
     if not args.no_adjoint:
         out["adjoint_64"] = time_adjoint(graphs=graphs)
     out["hbm_copy_check"] = copy_bandwidth()
-    out["hbm_copy_check_elementwise"] = elementwise_copy_bandwidth()
+    # the same check with this library's streaming copy kernel instead of the runtime's copy
+    out["hbm_copy_check_kernel"] = bp.stream_copy_bandwidth(1 << 30, 5)
     print(json.dumps(out), flush=True)
 
 

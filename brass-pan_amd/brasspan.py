@@ -96,6 +96,8 @@ SIGNATURES = {
     "bp_solver_autotune": (_S, [_P, C.c_float, C.c_int, C.c_char_p, C.c_int]),
     "bp_solver_set_fusion": (_S, [_P, C.c_int]),
     "bp_solver_set_walk": (_S, [_P, C.c_int, C.c_int, C.c_int]),
+    "bp_solver_set_block_sizes": (_S, [_P, C.POINTER(C.c_int)]),
+    "bp_stream_copy_bandwidth": (_S, [C.c_size_t, C.c_int, C.POINTER(C.c_float)]),
     "bp_forward_checkpointed": (_S, [_P, C.c_int, C.c_float, C.c_int]),
     "bp_backward": (_S, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
                          C.POINTER(C.c_float), C.POINTER(C.c_float)]),
@@ -258,6 +260,18 @@ class Solver:
         _check(self.lib.bp_solver_autotune(self.h, float(dt), int(substeps_per_trial), buf, 512))
         return json.loads(buf.value.decode())
 
+    TUNED_PASSES = ("fluid_lambda", "fluid_position", "omega_normal", "vorticity_tension", "xsph",
+                    "particle_particle")
+
+    def apply_tuning(self, report: dict):
+        """Applies an autotune() report (workgroup sizes, iteration fusion, walk
+        forms) without timing anything."""
+        sizes = (C.c_int * 6)(*[int(report[k]) for k in self.TUNED_PASSES])
+        _check(self.lib.bp_solver_set_block_sizes(self.h, sizes))
+        self.set_fusion(bool(report["iteration_fusion"]))
+        self.set_walk(report["lambda_walk"] == "staged", report["position_walk"] == "flat",
+                      report["vorticity_tension_walk"] == "flat")
+
     # ---------------------------------------------------------- adjoint
     def forward_checkpointed(self, substeps: int, subdt: float, every: int = 8):
         """Differentiable rollout (bp_forward_checkpointed)."""
@@ -401,6 +415,14 @@ class Solver:
             nm = raw[32 * i: 32 * i + 32].split(b"\0", 1)[0].decode()
             out[nm] = (float(ms[i]), int(ln[i]))
         return out
+
+
+def stream_copy_bandwidth(nbytes: int = 1 << 30, reps: int = 5) -> dict:
+    """bp_stream_copy_bandwidth: the library's streaming copy kernel over
+    `nbytes`, best time and bytes read plus written per second."""
+    ms = C.c_float()
+    _check(load_library().bp_stream_copy_bandwidth(C.c_size_t(nbytes), int(reps), C.byref(ms)))
+    return {"bytes": 2 * nbytes, "ms": round(ms.value, 4), "GBs": round(2 * nbytes / (ms.value * 1e-3) / 1e9, 1)}
 
 
 def neighbor_grid(xs, cell_size, grid_dims):

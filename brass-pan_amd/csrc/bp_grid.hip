@@ -353,6 +353,25 @@ void launch_first_nonfinite(const float4 *x, int n, int *first, hipStream_t st) 
     hipLaunchKernelGGL(k_first_nonfinite, dim3(grid_blocks(n)), dim3(kBlock), 0, st, x, n, first);
 }
 
+// Streaming copy (the bandwidth check of bp_stream_copy_bandwidth): every
+// thread moves four float4 per round, grid-stride, all loads before the stores.
+__global__ __launch_bounds__(kBlock) void k_stream_copy(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                                        size_t n) {
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const float4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+        dst[i] = a;
+        dst[i + stride] = b;
+        dst[i + 2 * stride] = c;
+        dst[i + 3 * stride] = d;
+    }
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+void launch_stream_copy(const float4 *src, float4 *dst, size_t n, int blocks, hipStream_t st) {
+    hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(kBlock), 0, st, src, dst, n);
+}
+
 #define BP_INSTANTIATE_GRID(SP)                                                                                 \
     template void launch_predict_hash<SP>(const float4 *, const float4 *, int *, int, float, f3, const SP &,    \
                                           const int *, int2 *, hipStream_t);                                    \

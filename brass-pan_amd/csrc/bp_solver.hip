@@ -786,6 +786,21 @@ bp_status bp_solver_set_fusion(bp_solver *s, int enable) {
     return BP_OK;
 }
 
+bp_status bp_solver_set_block_sizes(bp_solver *s, const int sizes[6]) {
+    if (!s || !sizes) return fail(BP_ERR_INVALID_ARG, "NULL argument");
+    for (int f = 0; f < 6; ++f)
+        if (sizes[f] != 64 && sizes[f] != 128 && sizes[f] != 256 && sizes[f] != 512)
+            return fail(BP_ERR_INVALID_ARG, "workgroup sizes must be 64, 128, 256 or 512");
+    s->blk_lambda = sizes[0];
+    s->blk_position = sizes[1];
+    s->blk_post_a = sizes[2];
+    s->blk_post_b = sizes[3];
+    s->blk_xsph = sizes[4];
+    s->blk_pp = sizes[5];
+    s->drop_graph();
+    return BP_OK;
+}
+
 bp_status bp_solver_set_walk(bp_solver *s, int lambda_staged, int position_flat, int vorticity_tension_flat) {
     if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
     s->staged_lambda = lambda_staged != 0;
@@ -1071,6 +1086,42 @@ static void rigid_matrix(const float *q, const float *cm, float *m) {
     m[13] = cm[1];
     m[14] = cm[2];
     m[15] = 1.0f;
+}
+
+bp_status bp_stream_copy_bandwidth(size_t bytes, int reps, float *best_ms) {
+    if (!best_ms || reps <= 0 || bytes == 0 || bytes % 16 != 0)
+        return fail(BP_ERR_INVALID_ARG, "bytes must be a positive multiple of 16, reps > 0, best_ms non-NULL");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(BP_ERR_NO_DEVICE, "no HIP device visible");
+    float4 *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc((void **)&a, bytes);
+    if (e == hipSuccess) e = hipMalloc((void **)&b, bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 0, bytes, nullptr);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    float best = 1e30f;
+    const size_t n = bytes / 16;
+    for (int blocks = 1024; blocks <= 8192 && e == hipSuccess; blocks *= 2) {
+        launch_stream_copy(a, b, n, blocks, nullptr);  // warm-up
+        for (int r = 0; r < reps && e == hipSuccess; ++r) {
+            e = hipEventRecord(e0, nullptr);
+            launch_stream_copy(a, b, n, blocks, nullptr);
+            if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e == hipSuccess && ms < best) best = ms;
+        }
+    }
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e != hipSuccess) return fail(BP_ERR_HIP, std::string("stream copy: ") + hipGetErrorString(e));
+    *best_ms = best;
+    return BP_OK;
 }
 
 bp_status bp_map_particle_infos(bp_solver *s, void *device_records) {

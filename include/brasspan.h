@@ -20,6 +20,8 @@
 #ifndef BRASSPAN_H_
 #define BRASSPAN_H_
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -290,10 +292,25 @@ bp_status bp_solver_set_walk(bp_solver *s, int lambda_staged, int position_flat,
  * reference. */
 bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, char *report, int report_len);
 
+/* Sets the neighbour passes' workgroup sizes, each 64, 128, 256 or 512, in
+ * the order lambda, position, omega/normal, vorticity/tension, XSPH,
+ * particle-particle (the sizes bp_solver_autotune reports), e.g. to reuse one
+ * tuning across runs.  Results do not depend on them.  No counterpart in the
+ * reference. */
+bp_status bp_solver_set_block_sizes(bp_solver *s, const int sizes[6]);
+
 /* Timing helper for benchmarks: runs `substeps` substeps of size dt and
  * returns the elapsed GPU time in ms measured with HIP events on the
  * solver's stream (bracketing all launches). */
 bp_status bp_solver_timed_substeps(bp_solver *s, int substeps, float dt, float *out_ms);
+
+/* HBM bandwidth check for roofline reporting (no counterpart in the
+ * reference): a device-to-device copy of `bytes` (a multiple of 16) by this
+ * library's streaming kernel on the default stream, at 1024, 2048, 4096 and
+ * 8192 workgroups of 256, best of `reps` each after one warm-up, timed with
+ * HIP events.  *best_ms = the fastest copy; bytes read plus written over it
+ * is the bandwidth.  Allocates and frees its two buffers. */
+bp_status bp_stream_copy_bandwidth(size_t bytes, int reps, float *best_ms);
 
 /* Per-kernel timing over the last bp_solver_timed_substeps call when
  * profiling is enabled with bp_solver_enable_kernel_timing(s, 1): names and
