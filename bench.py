@@ -237,6 +237,23 @@ def time_batched(frames=3, warmup=1, num=512, graphs=True):
             "config": {"workload": f"batched_{num}_scenes", "scenes": num, "particles": n, "block_sizes": choice}}
 
 
+def host_info():
+    """The machine the line was measured on: CPU model and logical CPUs of the
+    host, the GPU's name and architecture as the HIP runtime reports them."""
+    cpu = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), None)
+    except OSError:
+        pass
+    gpu = None
+    try:
+        gpu = bp.device_name()
+    except Exception:  # noqa: BLE001 - a missing name does not void the line
+        pass
+    return {"cpu": cpu, "logical_cpus": os.cpu_count(), "gpu": gpu}
+
+
 def cpu_baseline_batch(scs, prms, subdt, budget_s=15.0, ref_budget_s=5.0):
     """The FP32 oracle on a bounded sample of the batched workload: scenes in
     batch order, each one warm-up substep and two timed substeps, until the
@@ -540,6 +557,7 @@ def main():
         "kernel_ms": fam_times,
         "plain_loop": loop,
         "speedup_vs_plain_loop": round(value / best_loop, 2),
+        "host": host_info(),
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"], out["cpu_baseline_reference_order"] = cpu_baseline_batch(scs, prms, subdt)

@@ -139,6 +139,15 @@ def hip_runtime() -> C.CDLL:
     return C.CDLL("libamdhip64.so.7")
 
 
+def device_name(device: int = 0):
+    """The HIP runtime's name for a device (hipDeviceGetName), or None."""
+    hip = hip_runtime()
+    buf = C.create_string_buffer(256)
+    if hip.hipDeviceGetName(buf, 256, int(device)) != 0:
+        return None
+    return buf.value.decode(errors="replace") or None
+
+
 def _check(status: int):
     if status != BP_OK:
         msg = load_library().bp_last_error_string().decode(errors="replace")

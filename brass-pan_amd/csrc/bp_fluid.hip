@@ -236,14 +236,18 @@ __global__ __launch_bounds__(kBlock) void k_position_apply(const float4 *__restr
 
 // fluidOmega (fluid.cuh:231-279) and fluidNormal (fluid.cuh:387-434) in one
 // sweep (independent sums over the same neighbours).  omega.w = |omega|.
-template <class SP>
+// Mode: bit 0 = omega (vorticity confinement follows), bit 1 = normals
+// (Akinci tension follows); one kernel per combination, so a pass carries the
+// registers of the terms it computes only.
+template <class SP, int Mode>
 __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_post_a(const float4 *__restrict__ xs, const float4 *__restrict__ v,
                                                    const float *__restrict__ density,
                                                    const int2 *__restrict__ ranges, float4 *__restrict__ omega,
                                                    float4 *__restrict__ xom,
                                                    float4 *__restrict__ normal, uint32_t *__restrict__ ccell,
                                                    uint32_t *__restrict__ srec, const float *__restrict__ planes,
-                                                   int pstride, int n, SP sp, int do_omega, int do_normal) {
+                                                   int pstride, int n, SP sp) {
+    constexpr bool do_omega = (Mode & 1) != 0, do_normal = (Mode & 2) != 0;
     const int i = pass_slot();
     if (i >= n) return;
     sp.with_scene(i, [&](const SceneConst &S) {
@@ -297,7 +301,9 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_post_a(const fl
 // fluidVorticity (fluid.cuh:281-336, in place) followed by fluidAkinciTension
 // (fluid.cuh:436-489) for the same particle: both only change the particle's
 // own velocity, so the fused pass writes v in place.
-template <class SP, bool Flat>
+// Mode: bit 0 = vorticity confinement, bit 1 = Akinci tension (one kernel per
+// combination, as for k_post_a).
+template <class SP, bool Flat, int Mode>
 __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 1) void k_post_b(const float4 *__restrict__ xs, float4 *__restrict__ v,
                                                    const float4 *__restrict__ omega,
                                                    const float4 *__restrict__ xom,
@@ -305,8 +311,8 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 1) void k_post_b(const fl
                                                    const float *__restrict__ density,
                                                    const int2 *__restrict__ ranges,
                                                    const uint32_t *__restrict__ ccell,
-                                                   const uint32_t *__restrict__ srec, int n, SP sp, int do_vort,
-                                                   int do_akinci, float dt) {
+                                                   const uint32_t *__restrict__ srec, int n, SP sp, float dt) {
+    constexpr bool do_vort = (Mode & 1) != 0, do_akinci = (Mode & 2) != 0;
     const int i = pass_slot();
     if (i >= n) return;
     sp.with_scene(i, [&](const SceneConst &S) {
@@ -442,20 +448,45 @@ void launch_post_a(const float4 *xs, const float4 *v, const float *density, cons
                    int planes_ready, int n, const SP &sp, int do_omega, int do_normal, int block, hipStream_t st) {
     if (one_phase && !planes_ready)
         hipLaunchKernelGGL(k_lambda_in, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, nullptr, *one_phase, n);
-    hipLaunchKernelGGL(k_post_a<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, v, density, ranges, omega,
-                       xom, normal, ccell, srec, one_phase ? one_phase->planes : nullptr,
-                       one_phase ? one_phase->pstride : 0, n, sp, do_omega, do_normal);
+    const float *planes = one_phase ? one_phase->planes : nullptr;
+    const int pstride = one_phase ? one_phase->pstride : 0;
+    const dim3 grid(grid_blocks(n, block)), blk(block);
+    switch ((do_omega ? 1 : 0) | (do_normal ? 2 : 0)) {
+    case 1:
+        hipLaunchKernelGGL((k_post_a<SP, 1>), grid, blk, 0, st, xs, v, density, ranges, omega, xom, normal, ccell, srec,
+                           planes, pstride, n, sp);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_post_a<SP, 2>), grid, blk, 0, st, xs, v, density, ranges, omega, xom, normal, ccell, srec,
+                           planes, pstride, n, sp);
+        break;
+    case 3:
+        hipLaunchKernelGGL((k_post_a<SP, 3>), grid, blk, 0, st, xs, v, density, ranges, omega, xom, normal, ccell, srec,
+                           planes, pstride, n, sp);
+        break;
+    default:
+        break;
+    }
 }
 template <class SP>
 void launch_post_b(const float4 *xs, float4 *v, const float4 *omega, const float4 *xom, const float4 *normal,
                    const float *density, const int2 *ranges, const uint32_t *ccell, const uint32_t *srec, int n,
                    const SP &sp, int do_vort, int do_akinci, float dt, int block, int flat, hipStream_t st) {
-    if (flat)
-        hipLaunchKernelGGL((k_post_b<SP, true>), dim3(grid_blocks(n, block)), dim3(block), flat_lds_bytes(block), st, xs,
-                           v, omega, xom, normal, density, ranges, ccell, srec, n, sp, do_vort, do_akinci, dt);
-    else
-        hipLaunchKernelGGL((k_post_b<SP, false>), dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, v, omega, xom,
-                           normal, density, ranges, ccell, srec, n, sp, do_vort, do_akinci, dt);
+    const dim3 grid(grid_blocks(n, block)), blk(block);
+    const size_t lds = flat ? flat_lds_bytes(block) : 0;
+#define BP_POST_B(F, M)                                                                                           \
+    hipLaunchKernelGGL((k_post_b<SP, F, M>), grid, blk, lds, st, xs, v, omega, xom, normal, density, ranges, ccell, \
+                       srec, n, sp, dt)
+    switch ((do_vort ? 1 : 0) | (do_akinci ? 2 : 0) | (flat ? 4 : 0)) {
+    case 1: BP_POST_B(false, 1); break;
+    case 2: BP_POST_B(false, 2); break;
+    case 3: BP_POST_B(false, 3); break;
+    case 5: BP_POST_B(true, 1); break;
+    case 6: BP_POST_B(true, 2); break;
+    case 7: BP_POST_B(true, 3); break;
+    default: break;
+    }
+#undef BP_POST_B
 }
 template <class SP>
 void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const uint32_t *ccell, const uint32_t *srec,

@@ -354,22 +354,37 @@ void launch_first_nonfinite(const float4 *x, int n, int *first, hipStream_t st) 
 }
 
 // Streaming copy (the bandwidth check of bp_stream_copy_bandwidth): every
-// thread moves four float4 per round, grid-stride, all loads before the stores.
+// thread moves U float4 per round, grid-stride, all loads before the stores;
+// NT marks them non-temporal (streamed past the caches).
+template <bool NT, int U>
 __global__ __launch_bounds__(kBlock) void k_stream_copy(const float4 *__restrict__ src, float4 *__restrict__ dst,
                                                         size_t n) {
+    typedef float f4e __attribute__((ext_vector_type(4)));
+    const f4e *s4 = reinterpret_cast<const f4e *>(src);
+    f4e *d4 = reinterpret_cast<f4e *>(dst);
     const size_t stride = (size_t)gridDim.x * kBlock;
     size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    for (; i + 3 * stride < n; i += 4 * stride) {
-        const float4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
-        dst[i] = a;
-        dst[i + stride] = b;
-        dst[i + 2 * stride] = c;
-        dst[i + 3 * stride] = d;
+    for (; i + (U - 1) * stride < n; i += U * stride) {
+        f4e r[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) r[u] = NT ? __builtin_nontemporal_load(s4 + i + u * stride) : s4[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (NT)
+                __builtin_nontemporal_store(r[u], d4 + i + u * stride);
+            else
+                d4[i + u * stride] = r[u];
+        }
     }
-    for (; i < n; i += stride) dst[i] = src[i];
+    for (; i < n; i += stride) d4[i] = s4[i];
 }
-void launch_stream_copy(const float4 *src, float4 *dst, size_t n, int blocks, hipStream_t st) {
-    hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(kBlock), 0, st, src, dst, n);
+void launch_stream_copy(const float4 *src, float4 *dst, size_t n, int blocks, int variant, hipStream_t st) {
+    switch (variant) {
+    case 0: hipLaunchKernelGGL((k_stream_copy<false, 4>), dim3(blocks), dim3(kBlock), 0, st, src, dst, n); break;
+    case 1: hipLaunchKernelGGL((k_stream_copy<true, 4>), dim3(blocks), dim3(kBlock), 0, st, src, dst, n); break;
+    case 2: hipLaunchKernelGGL((k_stream_copy<false, 8>), dim3(blocks), dim3(kBlock), 0, st, src, dst, n); break;
+    default: hipLaunchKernelGGL((k_stream_copy<true, 8>), dim3(blocks), dim3(kBlock), 0, st, src, dst, n); break;
+    }
 }
 
 #define BP_INSTANTIATE_GRID(SP)                                                                                 \

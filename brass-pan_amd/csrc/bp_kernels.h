@@ -11,7 +11,7 @@ namespace bp {
 void launch_map_particle_infos(const float4 *x, const int *orig, const int *group, float4 *out, int n,
                                hipStream_t st);
 void launch_first_nonfinite(const float4 *x, int n, int *first, hipStream_t st);
-void launch_stream_copy(const float4 *src, float4 *dst, size_t n, int blocks, hipStream_t st);
+void launch_stream_copy(const float4 *src, float4 *dst, size_t n, int blocks, int variant, hipStream_t st);
 template <class SP>
 void launch_predict_hash(const float4 *x, const float4 *v, int *keys, int n, float dt, f3 gstep, const SP &sp,
                          const int *old_sorted_keys, int2 *ranges, hipStream_t st);

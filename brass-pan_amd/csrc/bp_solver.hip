@@ -1102,11 +1102,12 @@ bp_status bp_stream_copy_bandwidth(size_t bytes, int reps, float *best_ms) {
     if (e == hipSuccess) e = hipEventCreate(&e1);
     float best = 1e30f;
     const size_t n = bytes / 16;
-    for (int blocks = 1024; blocks <= 8192 && e == hipSuccess; blocks *= 2) {
-        launch_stream_copy(a, b, n, blocks, nullptr);  // warm-up
+    for (int t = 0; t < 16 && e == hipSuccess; ++t) {
+        const int blocks = 1024 << (t >> 2), variant = t & 3;
+        launch_stream_copy(a, b, n, blocks, variant, nullptr);  // warm-up
         for (int r = 0; r < reps && e == hipSuccess; ++r) {
             e = hipEventRecord(e0, nullptr);
-            launch_stream_copy(a, b, n, blocks, nullptr);
+            launch_stream_copy(a, b, n, blocks, variant, nullptr);
             if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
             if (e == hipSuccess) e = hipEventSynchronize(e1);
             float ms = 0.f;

@@ -307,8 +307,9 @@ bp_status bp_solver_timed_substeps(bp_solver *s, int substeps, float dt, float *
 /* HBM bandwidth check for roofline reporting (no counterpart in the
  * reference): a device-to-device copy of `bytes` (a multiple of 16) by this
  * library's streaming kernel on the default stream, at 1024, 2048, 4096 and
- * 8192 workgroups of 256, best of `reps` each after one warm-up, timed with
- * HIP events.  *best_ms = the fastest copy; bytes read plus written over it
+ * 8192 workgroups of 256, 4 or 8 loads in flight per lane, cached or
+ * non-temporal, best of `reps` each after one warm-up, timed with HIP
+ * events.  *best_ms = the fastest copy; bytes read plus written over it
  * is the bandwidth.  Allocates and frees its two buffers. */
 bp_status bp_stream_copy_bandwidth(size_t bytes, int reps, float *best_ms);
 
