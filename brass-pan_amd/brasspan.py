@@ -95,7 +95,7 @@ SIGNATURES = {
     "bp_solver_set_graph": (_S, [_P, C.c_int]),
     "bp_solver_autotune": (_S, [_P, C.c_float, C.c_int, C.c_char_p, C.c_int]),
     "bp_solver_set_fusion": (_S, [_P, C.c_int]),
-    "bp_solver_set_walk": (_S, [_P, C.c_int, C.c_int, C.c_int]),
+    "bp_solver_set_walk": (_S, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
     "bp_solver_set_block_sizes": (_S, [_P, C.POINTER(C.c_int)]),
     "bp_stream_copy_bandwidth": (_S, [C.c_size_t, C.c_int, C.POINTER(C.c_float)]),
     "bp_forward_checkpointed": (_S, [_P, C.c_int, C.c_float, C.c_int]),
@@ -257,11 +257,12 @@ class Solver:
         """Iteration fusion on or off (bp_solver_set_fusion; results are bit-identical)."""
         _check(self.lib.bp_solver_set_fusion(self.h, 1 if on else 0))
 
-    def set_walk(self, lambda_staged: bool = True, position_flat: bool = True, vorticity_tension_flat: bool = True):
-        """Walk forms of the lambda, position and vorticity/tension passes
-        (bp_solver_set_walk; results are bit-identical)."""
+    def set_walk(self, lambda_staged: bool = True, position_flat: bool = True, omega_normal_staged: bool = True,
+                 vorticity_tension_flat: bool = True):
+        """Walk forms of the lambda, position, omega/normal and vorticity/tension
+        passes (bp_solver_set_walk; results are bit-identical)."""
         _check(self.lib.bp_solver_set_walk(self.h, int(bool(lambda_staged)), int(bool(position_flat)),
-                                           int(bool(vorticity_tension_flat))))
+                                           int(bool(omega_normal_staged)), int(bool(vorticity_tension_flat))))
 
     def autotune(self, dt: float, substeps_per_trial: int = 2) -> dict:
         import json
@@ -279,6 +280,7 @@ class Solver:
         _check(self.lib.bp_solver_set_block_sizes(self.h, sizes))
         self.set_fusion(bool(report["iteration_fusion"]))
         self.set_walk(report["lambda_walk"] == "staged", report["position_walk"] == "flat",
+                      report.get("omega_normal_walk", "global") == "staged",
                       report["vorticity_tension_walk"] == "flat")
 
     # ---------------------------------------------------------- adjoint

@@ -238,9 +238,11 @@ __global__ __launch_bounds__(kBlock) void k_position_apply(const float4 *__restr
 // sweep (independent sums over the same neighbours).  omega.w = |omega|.
 // Mode: bit 0 = omega (vorticity confinement follows), bit 1 = normals
 // (Akinci tension follows); one kernel per combination, so a pass carries the
-// registers of the terms it computes only.
-template <class SP, int Mode>
-__global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_post_a(const float4 *__restrict__ xs, const float4 *__restrict__ v,
+// registers of the terms it computes only.  Staged: candidates and partner
+// records from the wave's LDS (for_each_neighbor_staged, 4 waves per SIMD: it
+// needs 111-121 VGPRs) or per lane from global memory.
+template <class SP, int Mode, bool Staged>
+__global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, Staged ? 4 : 6) void k_post_a(const float4 *__restrict__ xs, const float4 *__restrict__ v,
                                                    const float *__restrict__ density,
                                                    const int2 *__restrict__ ranges, float4 *__restrict__ omega,
                                                    float4 *__restrict__ xom,
@@ -272,23 +274,25 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_post_a(const fl
         rec.planes = planes;
         rec.pstride = pstride;
         const bool any_phase = planes != nullptr;
-        struct Partner {
-            float4 b, v;  // x*, (v, density)
+        // partner records: x*, (v, density)
+        auto pick = [&](int, float4 b) { return (any_phase | (phase_of(b) == ph)) & (dist2(pi, xyz(b)) <= K.h2); };
+        auto body = [&](int, const StagedRec<2> &P) {
+            const float4 b = P.r[0];
+            if (!(dist2(pi, xyz(b)) > 0.0f)) return;  // coincident partners are skipped
+            const f3 diff = pi - xyz(b);
+            const float r = fsqrt(dist2(pi, xyz(b)));
+            const f3 gr = K.grad_coef(r, frcp(r)) * diff;
+            if (do_omega) om += cross(xyz(P.r[1]) - vi, gr);
+            if (do_normal) nn += frcp(P.r[1].w) * gr;
         };
-        for_each_neighbor_nb<kBodyBatch>(
-            G, ranges, g, xs, 0,
-            [&](int, float4 b) { return (any_phase | (phase_of(b) == ph)) & (dist2(pi, xyz(b)) <= K.h2); },
-            [&](int j) { return Partner{ld32(xs, j), ld32(v, j)}; },
-            [&](int, const Partner &P) {
-                const float4 b = P.b;
-                if (!(dist2(pi, xyz(b)) > 0.0f)) return;  // coincident partners are skipped
-                const f3 diff = pi - xyz(b);
-                const float r = fsqrt(dist2(pi, xyz(b)));
-                const f3 gr = K.grad_coef(r, frcp(r)) * diff;
-                if (do_omega) om += cross(xyz(P.v) - vi, gr);
-                if (do_normal) nn += frcp(P.v.w) * gr;
-            },
-            AllowNotSelf{i}, rec);
+        if constexpr (Staged) {
+            const float4 *const srcs[2] = {xs, v};
+            for_each_neighbor_staged<kBodyBatch, 2>(G, ranges, g, srcs, 0, pick, body, AllowNotSelf{i}, rec);
+        } else {
+            for_each_neighbor_nb<kBodyBatch>(
+                G, ranges, g, xs, 0, pick, [&](int j) { return StagedRec<2>{{ld32(xs, j), ld32(v, j)}}; }, body,
+                AllowNotSelf{i}, rec);
+        }
         // (x, |omega|) and (normal, density) are the two records the
         // vorticity/tension pass fetches per partner
         const float mag = do_omega ? fsqrt(dot(om, om)) : 0.0f;
@@ -445,28 +449,27 @@ void launch_position(const float4 *xs, const float4 *lam_rec, const int *build_k
 template <class SP>
 void launch_post_a(const float4 *xs, const float4 *v, const float *density, const int2 *ranges, float4 *omega,
                    float4 *xom, float4 *normal, uint32_t *ccell, uint32_t *srec, const LambdaIn *one_phase,
-                   int planes_ready, int n, const SP &sp, int do_omega, int do_normal, int block, hipStream_t st) {
+                   int planes_ready, int n, const SP &sp, int do_omega, int do_normal, int block, int staged,
+                   hipStream_t st) {
     if (one_phase && !planes_ready)
         hipLaunchKernelGGL(k_lambda_in, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, nullptr, *one_phase, n);
     const float *planes = one_phase ? one_phase->planes : nullptr;
     const int pstride = one_phase ? one_phase->pstride : 0;
     const dim3 grid(grid_blocks(n, block)), blk(block);
-    switch ((do_omega ? 1 : 0) | (do_normal ? 2 : 0)) {
-    case 1:
-        hipLaunchKernelGGL((k_post_a<SP, 1>), grid, blk, 0, st, xs, v, density, ranges, omega, xom, normal, ccell, srec,
-                           planes, pstride, n, sp);
-        break;
-    case 2:
-        hipLaunchKernelGGL((k_post_a<SP, 2>), grid, blk, 0, st, xs, v, density, ranges, omega, xom, normal, ccell, srec,
-                           planes, pstride, n, sp);
-        break;
-    case 3:
-        hipLaunchKernelGGL((k_post_a<SP, 3>), grid, blk, 0, st, xs, v, density, ranges, omega, xom, normal, ccell, srec,
-                           planes, pstride, n, sp);
-        break;
-    default:
-        break;
+    const size_t lds = staged ? staged_lds_bytes(block, 2) : 0;
+#define BP_POST_A(M, S)                                                                                           \
+    hipLaunchKernelGGL((k_post_a<SP, M, S>), grid, blk, lds, st, xs, v, density, ranges, omega, xom, normal, ccell, \
+                       srec, planes, pstride, n, sp)
+    switch ((do_omega ? 1 : 0) | (do_normal ? 2 : 0) | (staged ? 4 : 0)) {
+    case 1: BP_POST_A(1, false); break;
+    case 2: BP_POST_A(2, false); break;
+    case 3: BP_POST_A(3, false); break;
+    case 5: BP_POST_A(1, true); break;
+    case 6: BP_POST_A(2, true); break;
+    case 7: BP_POST_A(3, true); break;
+    default: break;
     }
+#undef BP_POST_A
 }
 template <class SP>
 void launch_post_b(const float4 *xs, float4 *v, const float4 *omega, const float4 *xom, const float4 *normal,
@@ -504,7 +507,7 @@ void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const ui
                                       const Epilogue &, int, int, hipStream_t);                                 \
     template void launch_post_a<SP>(const float4 *, const float4 *, const float *, const int2 *, float4 *,      \
                                     float4 *, float4 *, uint32_t *, uint32_t *, const LambdaIn *, int, int,     \
-                                    const SP &, int, int, int, hipStream_t);                                    \
+                                    const SP &, int, int, int, int, hipStream_t);                               \
     template void launch_post_b<SP>(const float4 *, float4 *, const float4 *, const float4 *, const float4 *,   \
                                     const float *, const int2 *, const uint32_t *, const uint32_t *, int,       \
                                     const SP &, int, int, float, int, int, hipStream_t);                        \

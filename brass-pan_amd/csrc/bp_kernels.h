@@ -41,7 +41,7 @@ void launch_position(const float4 *xs, const float4 *lam_rec, const int *build_k
 template <class SP>
 void launch_post_a(const float4 *xs, const float4 *v, const float *density, const int2 *ranges, float4 *omega,
                    float4 *xom, float4 *normal, uint32_t *ccell, uint32_t *srec, const LambdaIn *one_phase,
-                   int planes_ready, int n, const SP &sp, int do_omega, int do_normal, int block, hipStream_t st);
+                   int planes_ready, int n, const SP &sp, int do_omega, int do_normal, int block, int staged, hipStream_t st);
 template <class SP>
 void launch_post_b(const float4 *xs, float4 *v, const float4 *omega, const float4 *xom, const float4 *normal,
                    const float *density, const int2 *ranges, const uint32_t *ccell, const uint32_t *srec, int n,

@@ -276,7 +276,7 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
             TIMED(F_POST_A, launch_post_a(s->xs.cur(), s->v.cur(), s->density, s->ranges, s->omega, s->xom, s->normal,
                                           s->ccell, s->srec, s->one_phase ? &s->lin : nullptr,
                                           finished && planes_ready ? 1 : 0, N, sp(), do_vort, do_ak, s->blk_post_a,
-                                          st));
+                                          s->staged_post_a, st));
             TIMED(F_POST_B, launch_post_b(s->xs.cur(), s->v.cur(), s->omega, s->xom, s->normal, s->density, s->ranges,
                                           s->ccell, s->srec, N, sp(), do_vort, do_ak, subdt, s->blk_post_b,
                                           s->flat_post_b, st));
@@ -801,9 +801,11 @@ bp_status bp_solver_set_block_sizes(bp_solver *s, const int sizes[6]) {
     return BP_OK;
 }
 
-bp_status bp_solver_set_walk(bp_solver *s, int lambda_staged, int position_flat, int vorticity_tension_flat) {
+bp_status bp_solver_set_walk(bp_solver *s, int lambda_staged, int position_flat, int omega_normal_staged,
+                             int vorticity_tension_flat) {
     if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
     s->staged_lambda = lambda_staged != 0;
+    s->staged_post_a = omega_normal_staged != 0;
     s->flat_position = position_flat != 0;
     s->flat_post_b = vorticity_tension_flat != 0;
     s->drop_graph();
@@ -847,12 +849,12 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
     s->use_graph = false;
     st = bp_solver_enable_kernel_timing(s, 1);
     if (st != BP_OK) return st;
-    // every workgroup size with each walk form of the lambda, position and
-    // vorticity/tension passes (the other passes have one form)
+    // every workgroup size with each walk form of the lambda, position,
+    // omega/normal and vorticity/tension passes (the others have one form)
     for (int t = 0; t < 8; ++t) {
         const int c = t >> 1, flat = t & 1;
         s->blk_lambda = s->blk_position = s->blk_post_a = s->blk_post_b = s->blk_xsph = s->blk_pp = cand[c];
-        s->staged_lambda = s->flat_position = s->flat_post_b = flat;
+        s->staged_lambda = s->flat_position = s->staged_post_a = s->flat_post_b = flat;
         for (int f = 0; f < F_COUNT; ++f) { s->fam_ms[f] = 0.f; s->fam_n[f] = 0; }
         st = run_substeps(s, substeps_per_trial, dt);
         if (st == BP_OK) st = collect_timing(s);
@@ -872,6 +874,7 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
     s->blk_pp = best_blk[5];
     s->staged_lambda = best_flat[0];
     s->flat_position = best_flat[1];
+    s->staged_post_a = best_flat[2];
     s->flat_post_b = best_flat[3];
     (void)bp_solver_enable_kernel_timing(s, 0);
     // iteration fusion: the epilogue passes (next iteration's plane
@@ -897,10 +900,10 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
                       "{\"fluid_lambda\": %d, \"fluid_position\": %d, \"omega_normal\": %d, "
                       "\"vorticity_tension\": %d, \"xsph\": %d, \"particle_particle\": %d, "
                       "\"iteration_fusion\": %d, \"lambda_walk\": \"%s\", \"position_walk\": \"%s\", "
-                      "\"vorticity_tension_walk\": \"%s\"}",
+                      "\"omega_normal_walk\": \"%s\", \"vorticity_tension_walk\": \"%s\"}",
                       best_blk[0], best_blk[1], best_blk[2], best_blk[3], best_blk[4], best_blk[5], s->fuse ? 1 : 0,
                       s->staged_lambda ? "staged" : "global", s->flat_position ? "flat" : "chunk",
-                      s->flat_post_b ? "flat" : "chunk");
+                      s->staged_post_a ? "staged" : "global", s->flat_post_b ? "flat" : "chunk");
     return BP_OK;
 }
 

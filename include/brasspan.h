@@ -270,26 +270,28 @@ bp_status bp_backward(bp_solver *s, const float *grad_x, const float *grad_v, fl
  * Results are bit-identical either way.  No counterpart in the reference. */
 bp_status bp_solver_set_fusion(bp_solver *s, int enable);
 
-/* Walk forms of the neighbour passes (default 1 for all three).  Lambda pass:
- * 1 = each wave stages its candidates' records in LDS and screens from there,
- * 0 = every lane reads its candidates from global memory.  Position and
- * vorticity/tension passes: 1 = the candidate walk records each 32-candidate
- * chunk's partner mask and the pair terms run after the walk over all
- * partners, 0 = the pair terms of each chunk run right after its screen.
- * Same pairs in the same order, so results are bit-identical; only the speed
- * differs (bp_solver_autotune chooses).  No counterpart in the reference. */
-bp_status bp_solver_set_walk(bp_solver *s, int lambda_staged, int position_flat, int vorticity_tension_flat);
+/* Walk forms of the neighbour passes (default 1 for all four).  Lambda and
+ * omega/normal passes: 1 = each wave stages its candidates' records in LDS
+ * and screens and fetches partners from there, 0 = every lane reads its
+ * candidates from global memory.  Position and vorticity/tension passes:
+ * 1 = the candidate walk records each 32-candidate chunk's partner mask and
+ * the pair terms run after the walk over all partners, 0 = the pair terms of
+ * each chunk run right after its screen.  Same pairs in the same order, so
+ * results are bit-identical; only the speed differs (bp_solver_autotune
+ * chooses).  No counterpart in the reference. */
+bp_status bp_solver_set_walk(bp_solver *s, int lambda_staged, int position_flat, int omega_normal_staged,
+                             int vorticity_tension_flat);
 
 /* Chooses the workgroup size of each neighbour pass among 64..512 and the
- * walk forms of the lambda, position and vorticity/tension passes
- * (bp_solver_set_walk) by timing
+ * walk forms of the lambda, position, omega/normal and vorticity/tension
+ * passes (bp_solver_set_walk) by timing
  * `substeps_per_trial` substeps of size dt per candidate, and iteration fusion
  * on or off (best of two trials each), then restores the particle and rigid
  * state it started from (results do not depend on any of these choices).
  * Writes a JSON object {pass name: size, "iteration_fusion": 0|1,
- * "lambda_walk": "staged"|"global", "position_walk"/"vorticity_tension_walk":
- * "flat"|"chunk"} into `report` (may be NULL).  No counterpart in the
- * reference. */
+ * "lambda_walk"/"omega_normal_walk": "staged"|"global",
+ * "position_walk"/"vorticity_tension_walk": "flat"|"chunk"} into `report`
+ * (may be NULL).  No counterpart in the reference. */
 bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, char *report, int report_len);
 
 /* Sets the neighbour passes' workgroup sizes, each 64, 128, 256 or 512, in

@@ -109,8 +109,9 @@ def test_autotune_keeps_results():
     report = tuned.autotune(DT / 2, 2)
     print("autotune report", report)
     passes = {"fluid_lambda", "fluid_position", "omega_normal", "vorticity_tension", "xsph", "particle_particle"}
-    assert set(report) == passes | {"iteration_fusion", "lambda_walk", "position_walk", "vorticity_tension_walk"}
-    assert report["lambda_walk"] in ("staged", "global")
+    assert set(report) == passes | {"iteration_fusion", "lambda_walk", "position_walk", "omega_normal_walk",
+                                    "vorticity_tension_walk"}
+    assert report["lambda_walk"] in ("staged", "global") and report["omega_normal_walk"] in ("staged", "global")
     assert all(report[k] in (64, 128, 256, 512) for k in passes)
     assert report["iteration_fusion"] in (0, 1)
     assert report["position_walk"] in ("flat", "chunk") and report["vorticity_tension_walk"] in ("flat", "chunk")

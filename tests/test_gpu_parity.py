@@ -364,10 +364,10 @@ def test_iteration_fusion_is_bit_identical(label):
 
 @pytest.mark.parametrize("label", ["akinci block 8k", "mixed phase small"])
 def test_walk_forms_are_bit_identical(label):
-    """The walk forms visit the same pairs in the same order: the lambda pass
-    with its candidates staged in LDS or read from global memory, the position
-    and vorticity/tension passes flat (pair terms after the whole candidate
-    walk) or per chunk.  Same bits over several substeps, on a scene that runs
+    """The walk forms visit the same pairs in the same order: the lambda and
+    omega/normal passes with their candidates staged in LDS or read from
+    global memory, the position and vorticity/tension passes flat (pair terms
+    after the whole candidate walk) or per chunk.  Same bits over several substeps, on a scene that runs
     vorticity confinement and Akinci tension and on one with solids, cloth and
     wind (deliveries to partners whose search misses the pair)."""
     if label == "akinci block 8k":
@@ -377,8 +377,8 @@ def test_walk_forms_are_bit_identical(label):
         prm = scenes.mixed_phase_params()
     case(f"walk forms: {label}")
     a, b = bp.Solver(sc, prm), bp.Solver(sc, prm)
-    a.set_walk(True, True, True)
-    b.set_walk(False, False, False)
+    a.set_walk(True, True, True, True)
+    b.set_walk(False, False, False, False)
     a.step(4, 4 * DT)
     b.step(4, 4 * DT)
     pa, va, ma = a.get_state()
