@@ -3,6 +3,7 @@
 // and the fused reorder.  Replaces updateGrid (particlesolver.cuh:712-765),
 // predictPositions/applyGravity (:49-69), inverseMapping (util.cuh:348-359),
 // computeInvScaledMasses (:84-98) and planeStabilize (particleplane.cuh:3-19).
+#include "bp_guard.h"
 #include "bp_kernels.h"
 
 namespace bp {
@@ -347,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_first_nonfinite(const float4 *__rest
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 p = x[i];
-    if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z))) atomicMin(first, i);
+    if (guard_flags(p.x, p.y, p.z)) atomicMin(first, i);
 }
 void launch_first_nonfinite(const float4 *x, int n, int *first, hipStream_t st) {
     hipLaunchKernelGGL(k_first_nonfinite, dim3(grid_blocks(n)), dim3(kBlock), 0, st, x, n, first);

@@ -18,6 +18,7 @@
 #include <string>
 #include <vector>
 
+#include "bp_guard.h"
 #include "bp_solver_impl.h"
 
 using namespace bp;
@@ -765,8 +766,7 @@ bp_status bp_solver_step(bp_solver *s, int num_substeps, float dt, int picked, c
         if (first < s->N) {
             int orig_id = -1;
             HIP_TRY(hipMemcpy(&orig_id, s->orig.cur() + first, sizeof(int), hipMemcpyDeviceToHost));
-            return fail(BP_ERR_NOT_FINITE, "non-finite position at slot " + std::to_string(first) +
-                                               " (original id " + std::to_string(orig_id) + ")");
+            return fail(BP_ERR_NOT_FINITE, guard_message(first, orig_id));
         }
     }
     return BP_OK;
