@@ -23,25 +23,43 @@ __global__ __launch_bounds__(kBlock) void k_plane_collide(float4 *__restrict__ x
     xs[i] = f4(out, a.w);
 }
 
-// Cells holding a solid this substep: cell_stamp[cell] == *stamp; cells
-// holding a fluid particle: fluid_stamp[cell] == *stamp.  The stamp advances
-// once per substep (k_stamp_next), so no table needs clearing.
+// Cells whose 27-cell block holds a solid this substep (by the grid build's
+// cells): cell_stamp[cell] == *stamp, so a particle learns whether its walk
+// can meet a solid from one entry.  The relation is symmetric (a solid's cell
+// lies in the block around c iff c lies in the block around it), so each
+// solid marks the 27 cells around its own.  Cells holding a fluid particle:
+// fluid_stamp[cell] == *stamp.  The stamp advances once per substep
+// (k_stamp_next), so no table needs clearing.
 __global__ void k_stamp_next(uint32_t *__restrict__ stamp) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *stamp = *stamp + 1u;
 }
+template <class SP>
 __global__ __launch_bounds__(kBlock) void k_mark_cells(const int *__restrict__ sorted_keys,
                                                        const int *__restrict__ ph, int n,
                                                        const uint32_t *__restrict__ stamp,
                                                        uint32_t *__restrict__ cell_stamp,
-                                                       uint32_t *__restrict__ fluid_stamp) {
+                                                       uint32_t *__restrict__ fluid_stamp, SP sp) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    (ph[i] > 0 ? cell_stamp : fluid_stamp)[sorted_keys[i]] = *stamp;
+    const uint32_t st = *stamp;
+    const int key = sorted_keys[i];
+    if (ph[i] <= 0) {
+        fluid_stamp[key] = st;
+        return;
+    }
+    const GridDesc &G = sp.at(i).G;
+    int g[3];
+    unpack_cell(G.pack_addr(key), g);
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+            for (int dz = -1; dz <= 1; ++dz) cell_stamp[G.addr(g[0] + dx, g[1] + dy, g[2] + dz)] = st;
 }
 
 // particleParticleCollisionConstraint (particleparticle.cuh:5-101) + Δx apply.
 // A fluid particle can only touch solids (the pair needs a positive phase),
-// so it skips the walk when none of its 27 cells holds one.
+// so it skips the walk when none of its 27 cells holds one (one cell_stamp
+// entry, k_mark_cells).
 template <class SP>
 __global__ __launch_bounds__(kMaxBlock) void k_particle_particle(const float4 *__restrict__ xs,
                                                                  const float4 *__restrict__ x,
@@ -65,7 +83,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_particle_particle(const float4 *_
         int cnt = 0;
         int g[3];
         G.pos(xip, g);
-        if (cell_stamp && pi <= 0 && !stamp_near(G, g, cell_stamp, *stamp)) {
+        if (cell_stamp && pi <= 0 && cell_stamp[G.addr(g[0], g[1], g[2])] != *stamp) {
             xs_out[i] = a;
             if (L.planes) store_lambda_in(L, i, xip, m[i], pi);
             return;
@@ -437,11 +455,12 @@ void launch_particle_particle(const float4 *xs, const float4 *x, const int2 *ran
     hipLaunchKernelGGL(k_particle_particle<SP>, dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, x, ranges,
                        xs_out, n, sp, contacts, cell_stamp, stamp, L, m);
 }
+template <class SP>
 void launch_mark_cells(const int *sorted_keys, const int *ph, int n, uint32_t *stamp, uint32_t *cell_stamp,
-                       uint32_t *fluid_stamp, hipStream_t st) {
+                       uint32_t *fluid_stamp, const SP &sp, hipStream_t st) {
     hipLaunchKernelGGL(k_stamp_next, dim3(1), dim3(64), 0, st, stamp);
-    hipLaunchKernelGGL(k_mark_cells, dim3(grid_blocks(n)), dim3(kBlock), 0, st, sorted_keys, ph, n, stamp,
-                       cell_stamp, fluid_stamp);
+    hipLaunchKernelGGL(k_mark_cells<SP>, dim3(grid_blocks(n)), dim3(kBlock), 0, st, sorted_keys, ph, n, stamp,
+                       cell_stamp, fluid_stamp, sp);
 }
 void launch_distance(float4 *xs, const float4 *x, const int2 *pairs, const float2 *params, const int *map_to_new,
                      float4 *kbuf, int nc, const int *verts, const int *offs, const int *ents, int nv,
@@ -488,6 +507,8 @@ void launch_finish(const float4 *xs, float4 *x, float4 *v, const float *vw, int 
 
 #define BP_INSTANTIATE_CONSTRAINTS(SP)                                                                          \
     template void launch_plane_collide<SP>(float4 *, const float4 *, int, const SP &, hipStream_t);              \
+    template void launch_mark_cells<SP>(const int *, const int *, int, uint32_t *, uint32_t *, uint32_t *,       \
+                                        const SP &, hipStream_t);                                                \
     template void launch_particle_particle<SP>(const float4 *, const float4 *, const int2 *, float4 *, int,      \
                                                const SP &, int, int *, const uint32_t *, const uint32_t *,       \
                                                const LambdaIn &, const float *, hipStream_t);                    \

@@ -57,10 +57,11 @@ template <class SP>
 void launch_particle_particle(const float4 *xs, const float4 *x, const int2 *ranges, float4 *xs_out, int n,
                               const SP &sp, int block, int *contacts, const uint32_t *cell_stamp,
                               const uint32_t *stamp, const LambdaIn &L, const float *m, hipStream_t st);
-// advances the substep stamp and marks the cells holding a solid (cell_stamp)
-// and those holding a fluid particle (fluid_stamp)
+// advances the substep stamp and marks the cells whose 27-cell block holds a
+// solid (cell_stamp) and those holding a fluid particle (fluid_stamp)
+template <class SP>
 void launch_mark_cells(const int *sorted_keys, const int *ph, int n, uint32_t *stamp, uint32_t *cell_stamp,
-                       uint32_t *fluid_stamp, hipStream_t st);
+                       uint32_t *fluid_stamp, const SP &sp, hipStream_t st);
 void launch_distance(float4 *xs, const float4 *x, const int2 *pairs, const float2 *params, const int *map_to_new,
                      float4 *kbuf, int nc, const int *verts, const int *offs, const int *ents, int nv,
                      hipStream_t st);

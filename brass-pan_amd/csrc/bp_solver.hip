@@ -194,7 +194,7 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
     s->orig.flip();
     if (s->sid.b[0]) s->sid.flip();
     if (s->cell_stamp)
-        TIMED(F_PP, launch_mark_cells(sk, s->ph.cur(), N, s->stamp, s->cell_stamp, s->fluid_stamp, st));
+        TIMED(F_PP, launch_mark_cells(sk, s->ph.cur(), N, s->stamp, s->cell_stamp, s->fluid_stamp, sp(), st));
     HIP_TRY(snap(H.x1, s->x.cur(), N, st));
     HIP_TRY(snap(H.xsw, s->xs.cur(), N, st));
     HIP_TRY(snap(H.m, s->m.cur(), N, st));

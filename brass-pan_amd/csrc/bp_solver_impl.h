@@ -90,7 +90,7 @@ struct bp_solver {
     unsigned long long *corr = nullptr;  // fixed-point pair corrections, kept zero between passes
     float4 *omega = nullptr, *normal = nullptr;  // normal.w: the density it was computed with
     float4 *xom = nullptr;                        // (x, |omega|) from the omega/normal pass
-    uint32_t *cell_stamp = nullptr, *stamp = nullptr;  // cells holding a solid (fluid + solid scenes)
+    uint32_t *cell_stamp = nullptr, *stamp = nullptr;  // cells near a solid (fluid + solid scenes)
     uint32_t *fluid_stamp = nullptr;                    // cells holding a fluid particle (same scenes)
 
     int nd = 0;
