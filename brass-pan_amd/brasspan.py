@@ -257,12 +257,15 @@ class Solver:
         """Iteration fusion on or off (bp_solver_set_fusion; results are bit-identical)."""
         _check(self.lib.bp_solver_set_fusion(self.h, 1 if on else 0))
 
-    def set_walk(self, lambda_staged: bool = True, position_flat: bool = True, omega_normal_staged: bool = True,
+    def set_walk(self, lambda_staged=True, position_flat: bool = True, omega_normal_staged=True,
                  vorticity_tension_flat: bool = True):
         """Walk forms of the lambda, position, omega/normal and vorticity/tension
-        passes (bp_solver_set_walk; results are bit-identical)."""
-        _check(self.lib.bp_solver_set_walk(self.h, int(bool(lambda_staged)), int(bool(position_flat)),
-                                           int(bool(omega_normal_staged)), int(bool(vorticity_tension_flat))))
+        passes (bp_solver_set_walk; results are bit-identical).  The staged
+        arguments take a bool (True: the default LDS region) or the region
+        size in records per wave and column offset (128, 256 or 512; 0 = the
+        per-lane walk)."""
+        _check(self.lib.bp_solver_set_walk(self.h, int(lambda_staged), int(bool(position_flat)),
+                                           int(omega_normal_staged), int(bool(vorticity_tension_flat))))
 
     def autotune(self, dt: float, substeps_per_trial: int = 2) -> dict:
         import json
@@ -279,8 +282,12 @@ class Solver:
         sizes = (C.c_int * 6)(*[int(report[k]) for k in self.TUNED_PASSES])
         _check(self.lib.bp_solver_set_block_sizes(self.h, sizes))
         self.set_fusion(bool(report["iteration_fusion"]))
-        self.set_walk(report["lambda_walk"] == "staged", report["position_walk"] == "flat",
-                      report.get("omega_normal_walk", "global") == "staged",
+        def region(walk, records):  # the staged region size; reports without one use the default
+            if report.get(walk, "global") != "staged":
+                return 0
+            return int(report.get(records, 1))
+        self.set_walk(region("lambda_walk", "lambda_stage_records"), report["position_walk"] == "flat",
+                      region("omega_normal_walk", "omega_normal_stage_records"),
                       report["vorticity_tension_walk"] == "flat")
 
     # ---------------------------------------------------------- adjoint

@@ -17,10 +17,11 @@ __global__ __launch_bounds__(kBlock) void k_plane_collide(float4 *__restrict__ x
                                                           int n, SP sp) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const SceneConst &S = sp.at(i);
-    const float4 a = xs[i];
-    const f3 out = plane_collide_one(xyz(a), xyz(x[i]), S.pl, S.radius, S.friction_static, S.friction_dynamic);
-    xs[i] = f4(out, a.w);
+    sp.with_scene_ref(i, [&](const SceneConst &S) {
+        const float4 a = xs[i];
+        const f3 out = plane_collide_one(xyz(a), xyz(x[i]), S.pl, S.radius, S.friction_static, S.friction_dynamic);
+        xs[i] = f4(out, a.w);
+    });
 }
 
 // Cells whose 27-cell block holds a solid this substep (by the grid build's

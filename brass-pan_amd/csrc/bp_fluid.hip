@@ -24,8 +24,10 @@ __global__ __launch_bounds__(kBlock) void k_lambda_in(const float4 *__restrict__
 // reads for this particle as self and as partner, (lambda, m, current cell
 // packed, 0), so that pass fetches two words-of-four per partner (x* and this)
 // instead of four separate arrays; it searches from the same x* and cell.
-template <class SP, bool Staged>
-__global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, Staged ? 5 : 6) void k_lambda(const float4 *__restrict__ xs, const float *__restrict__ m,
+// Cap: 0 = every lane reads its candidates from global memory; else the
+// staged walk with a region of Cap records per wave and column offset.
+template <class SP, int Cap>
+__global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, Cap ? 5 : 6) void k_lambda(const float4 *__restrict__ xs, const float *__restrict__ m,
                                                    const int2 *__restrict__ ranges, float *__restrict__ lambda,
                                                    float *__restrict__ density, float4 *__restrict__ lam_rec,
                                                    uint32_t *__restrict__ srec, LambdaIn L, int n, SP sp) {
@@ -76,9 +78,9 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, Staged ? 5 : 6) void k_la
         auto allow = [&](int base) { return ~chunk_below(i, base); };
         // one record per partner, so four partners per fetch fit the register
         // budget; the partner record is (x*, m), m negated for a non-fluid partner
-        if constexpr (Staged) {
+        if constexpr (Cap != 0) {
             const float4 *const srcs[1] = {L.xm};
-            for_each_neighbor_staged<4, 1>(
+            for_each_neighbor_staged<4, 1, Cap>(
                 G, ranges, g, srcs, srec ? 0 : i, pick, [&](int j, const StagedRec<1> &R) { body(j, R.r[0]); },
                 allow, rec);
         } else {
@@ -238,11 +240,12 @@ __global__ __launch_bounds__(kBlock) void k_position_apply(const float4 *__restr
 // sweep (independent sums over the same neighbours).  omega.w = |omega|.
 // Mode: bit 0 = omega (vorticity confinement follows), bit 1 = normals
 // (Akinci tension follows); one kernel per combination, so a pass carries the
-// registers of the terms it computes only.  Staged: candidates and partner
-// records from the wave's LDS (for_each_neighbor_staged, 4 waves per SIMD: it
-// needs 111-121 VGPRs) or per lane from global memory.
-template <class SP, int Mode, bool Staged>
-__global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, Staged ? 4 : 6) void k_post_a(const float4 *__restrict__ xs, const float4 *__restrict__ v,
+// registers of the terms it computes only.  Cap != 0: candidates and partner
+// records from the wave's LDS region of Cap records per column offset
+// (for_each_neighbor_staged, 4 waves per SIMD: it needs 111-121 VGPRs);
+// Cap = 0: per lane from global memory.
+template <class SP, int Mode, int Cap>
+__global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, Cap ? 4 : 6) void k_post_a(const float4 *__restrict__ xs, const float4 *__restrict__ v,
                                                    const float *__restrict__ density,
                                                    const int2 *__restrict__ ranges, float4 *__restrict__ omega,
                                                    float4 *__restrict__ xom,
@@ -285,9 +288,9 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, Staged ? 4 : 6) void k_po
             if (do_omega) om += cross(xyz(P.r[1]) - vi, gr);
             if (do_normal) nn += frcp(P.r[1].w) * gr;
         };
-        if constexpr (Staged) {
+        if constexpr (Cap != 0) {
             const float4 *const srcs[2] = {xs, v};
-            for_each_neighbor_staged<kBodyBatch, 2>(G, ranges, g, srcs, 0, pick, body, AllowNotSelf{i}, rec);
+            for_each_neighbor_staged<kBodyBatch, 2, Cap>(G, ranges, g, srcs, 0, pick, body, AllowNotSelf{i}, rec);
         } else {
             for_each_neighbor_nb<kBodyBatch>(
                 G, ranges, g, xs, 0, pick, [&](int j) { return StagedRec<2>{{ld32(xs, j), ld32(v, j)}}; }, body,
@@ -425,12 +428,17 @@ void launch_lambda(const float4 *xs, const float *m, const int2 *ranges, float *
                    float4 *lam_rec, uint32_t *srec, const LambdaIn &L, int lin_ready, int n, const SP &sp,
                    int block, int staged, hipStream_t st) {
     if (!lin_ready) hipLaunchKernelGGL(k_lambda_in, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, m, L, n);
-    if (staged)
-        hipLaunchKernelGGL((k_lambda<SP, true>), dim3(grid_blocks(n, block)), dim3(block), staged_lds_bytes(block), st,
-                           xs, m, ranges, lambda, density, lam_rec, srec, L, n, sp);
-    else
-        hipLaunchKernelGGL((k_lambda<SP, false>), dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, m, ranges,
-                           lambda, density, lam_rec, srec, L, n, sp);
+    const dim3 grid(grid_blocks(n, block)), blk(block);
+    const size_t lds = staged ? staged_lds_bytes(block, 1, staged) : 0;
+#define BP_LAMBDA(C) \
+    hipLaunchKernelGGL((k_lambda<SP, C>), grid, blk, lds, st, xs, m, ranges, lambda, density, lam_rec, srec, L, n, sp)
+    switch (staged) {
+    case 128: BP_LAMBDA(128); break;
+    case 256: BP_LAMBDA(256); break;
+    case 512: BP_LAMBDA(512); break;
+    default: BP_LAMBDA(0); break;
+    }
+#undef BP_LAMBDA
 }
 template <class SP>
 void launch_position(const float4 *xs, const float4 *lam_rec, const int *build_keys, const int2 *ranges,
@@ -456,19 +464,24 @@ void launch_post_a(const float4 *xs, const float4 *v, const float *density, cons
     const float *planes = one_phase ? one_phase->planes : nullptr;
     const int pstride = one_phase ? one_phase->pstride : 0;
     const dim3 grid(grid_blocks(n, block)), blk(block);
-    const size_t lds = staged ? staged_lds_bytes(block, 2) : 0;
-#define BP_POST_A(M, S)                                                                                           \
-    hipLaunchKernelGGL((k_post_a<SP, M, S>), grid, blk, lds, st, xs, v, density, ranges, omega, xom, normal, ccell, \
+    const size_t lds = staged ? staged_lds_bytes(block, 2, staged) : 0;
+#define BP_POST_A(M, C)                                                                                           \
+    hipLaunchKernelGGL((k_post_a<SP, M, C>), grid, blk, lds, st, xs, v, density, ranges, omega, xom, normal, ccell, \
                        srec, planes, pstride, n, sp)
-    switch ((do_omega ? 1 : 0) | (do_normal ? 2 : 0) | (staged ? 4 : 0)) {
-    case 1: BP_POST_A(1, false); break;
-    case 2: BP_POST_A(2, false); break;
-    case 3: BP_POST_A(3, false); break;
-    case 5: BP_POST_A(1, true); break;
-    case 6: BP_POST_A(2, true); break;
-    case 7: BP_POST_A(3, true); break;
+#define BP_POST_A_CAP(M)                  \
+    switch (staged) {                     \
+    case 128: BP_POST_A(M, 128); break;   \
+    case 256: BP_POST_A(M, 256); break;   \
+    case 512: BP_POST_A(M, 512); break;   \
+    default: BP_POST_A(M, 0); break;      \
+    }
+    switch ((do_omega ? 1 : 0) | (do_normal ? 2 : 0)) {
+    case 1: BP_POST_A_CAP(1); break;
+    case 2: BP_POST_A_CAP(2); break;
+    case 3: BP_POST_A_CAP(3); break;
     default: break;
     }
+#undef BP_POST_A_CAP
 #undef BP_POST_A
 }
 template <class SP>

@@ -217,6 +217,7 @@ struct OneScene {
     SceneConst c;
     __device__ __forceinline__ const SceneConst &at(int) const { return c; }
     template <class F> __device__ __forceinline__ void with_scene(int, F &&f) const { f(c); }
+    template <class F> __device__ __forceinline__ void with_scene_ref(int, F &&f) const { f(c); }
 };
 struct SceneTable {
     // 96 VGPRs: the omega/normal pass spills 2 and runs 10% faster than
@@ -251,6 +252,25 @@ struct SceneTable {
             todo &= ~__builtin_amdgcn_ballot_w64(mine);
         }
     }
+    // As with_scene, but f gets a reference to the wave-uniform table entry
+    // (read through the constant address space) instead of a local copy: for
+    // a pass that reads many fields, such as the plane arrays, whose copy
+    // would be placed in scratch memory.
+    template <class F> __device__ __forceinline__ void with_scene_ref(int slot, F &&f) const {
+        const int s = sid[slot];
+        uint64_t todo = __builtin_amdgcn_ballot_w64(true);
+        while (todo) {
+            const int s0 = __builtin_amdgcn_readlane(s, __builtin_ctzll(todo));
+            int u = s0;
+            asm volatile("" : "+s"(u));
+            const bool mine = s == s0;
+            if (mine) {
+                typedef const __attribute__((address_space(4))) SceneConst cscene;
+                f(*(const SceneConst *)((const cscene *)table + u));
+            }
+            todo &= ~__builtin_amdgcn_ballot_w64(mine);
+        }
+    }
 };
 
 // ------------------------------------------------------------ device helpers
@@ -262,9 +282,14 @@ __device__ __forceinline__ void unpack_cell(uint32_t c, int g[3]) {
     g[2] = (int)(c & 1023u);
 }
 
-// particlePlaneCollisionConstraint (particleplane.cuh:21-58) for one particle
+// particlePlaneCollisionConstraint (particleplane.cuh:21-58) for one particle.
+// The loop is unrolled over the kMaxPlanes slots, so a wave-uniform copy of
+// the planes (with_scene) is indexed by constants and stays in scalar
+// registers; the planes are applied in order, as before.
 __device__ __forceinline__ f3 plane_collide_one(f3 xs, f3 pos, const Planes &pl, float r, float fs, float fd) {
-    for (int p = 0; p < pl.n; ++p) {
+#pragma unroll
+    for (int p = 0; p < kMaxPlanes; ++p) {
+        if (p >= pl.n) break;
         f3 o = mk3(pl.o[p][0], pl.o[p][1], pl.o[p][2]);
         f3 n = mk3(pl.nrm[p][0], pl.nrm[p][1], pl.nrm[p][2]);
         f3 o2p = o - xs;
@@ -699,18 +724,25 @@ __device__ __forceinline__ void for_each_neighbor_flat(const GridDesc &G, const 
 // into its LDS once, with coalesced loads spread over its lanes, and each lane
 // then screens its runs and fetches its partners from LDS, in the walk's
 // order: same pairs, same summation order, same results.  A column whose
-// segments exceed kStageCap records, and a walk whose lanes fall in more than
+// segments exceed Cap records, and a walk whose lanes fall in more than
 // kStageGroups columns or whose z window wraps, reads global memory as the
 // per-chunk walk does (the x* planes, or srcs[0], for the screen; srcs for
 // the bodies).  srcs[0] holds x* (the screen's records); the pair body gets
-// srcs[0..NR-1][j].  Launch with staged_lds_bytes(block, NR) bytes of
+// srcs[0..NR-1][j].  Launch with staged_lds_bytes(block, NR, Cap) bytes of
 // dynamic LDS.
-constexpr int kStageCap = 256;   // records per wave and column offset
+//
+// Cap, the region's size in records per wave and column offset (about eight
+// per occupied cell), is a tuning knob: a larger region stages the wider
+// segments of waves that span several columns, a smaller one leaves more LDS
+// for resident waves.  kStageCaps lists the sizes built.
+constexpr int kStageCaps[3] = {128, 256, 512};
+constexpr int kStageCapDefault = 256;
 constexpr int kStageGroups = 4;  // lane groups (distinct current columns) per wave
 constexpr int kStagePad = 8;     // records after the region: a screen batch reads up to 7 past a run
-// nr staged arrays per wave
-inline size_t staged_lds_bytes(int block, int nr = 1) {
-    return (size_t)(block / 64) * nr * (kStageCap + kStagePad) * 16;
+inline bool stage_cap_valid(int cap) { return cap == 128 || cap == 256 || cap == 512; }
+// nr staged arrays per wave of `cap` records each
+inline size_t staged_lds_bytes(int block, int nr, int cap) {
+    return (size_t)(block / 64) * nr * (cap + kStagePad) * 16;
 }
 // The records a staged walk's pair body receives: srcs[0..NR-1][j]
 template <int NR> struct StagedRec {
@@ -721,11 +753,12 @@ typedef __attribute__((address_space(3))) f4v lds_f4;
 __device__ __forceinline__ float4 as_float4(f4v v) { return make_float4(v.x, v.y, v.z, v.w); }
 __device__ __forceinline__ f4v as_f4v(float4 a) { return f4v{a.x, a.y, a.z, a.w}; }
 
-template <int NB, int NR, class Pick, class Use, class Allow>
+template <int NB, int NR, int Cap, class Pick, class Use, class Allow>
 __device__ __forceinline__ void for_each_neighbor_staged(const GridDesc &G, const int2 *__restrict__ ranges,
                                                          const int g[3], const float4 *const (&srcs)[NR], int jmin,
                                                          Pick &&pick, Use &&use, Allow &&allow, ScreenRec rec) {
-    constexpr int kRegion = kStageCap + kStagePad;
+    static_assert(Cap == 128 || Cap == 256 || Cap == 512, "staging region sizes are those of kStageCaps");
+    constexpr int kRegion = Cap + kStagePad;
     lds_f4 *const lds = (lds_f4 *)bp_flat_lds + (threadIdx.x >> 6) * (NR * kRegion);
     const uint64_t act = __builtin_amdgcn_ballot_w64(true);
     const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
@@ -826,7 +859,7 @@ __device__ __forceinline__ void for_each_neighbor_staged(const GridDesc &G, cons
                     tot += e1 - s1;
                 }
             }
-            if (tot <= kStageCap) {
+            if (tot <= Cap) {
                 staged = true;
                 for (int t = rank; t < tot; t += nact) {
                     int b = sbase[0];

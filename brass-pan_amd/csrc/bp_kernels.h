@@ -28,7 +28,9 @@ void launch_reorder(const int *perm, const float4 *x_in, const float4 *v_in, con
 // bp_fluid.hip (`block`: threads per workgroup of the neighbour pass, <= kMaxBlock).
 // `srec` (kRecChunks x n candidate masks) is written by the lambda and
 // omega/normal passes and replayed by the position pass of the same iteration
-// and by the vorticity/tension and XSPH passes.
+// and by the vorticity/tension and XSPH passes.  `staged` (lambda and
+// omega/normal): 0 = per-lane walk, else the staged walk's region size in
+// records per wave and column offset (one of kStageCaps).
 template <class SP>
 void launch_lambda(const float4 *xs, const float *m, const int2 *ranges, float *lambda, float *density,
                    float4 *lam_rec, uint32_t *srec, const LambdaIn &L, int lin_ready, int n, const SP &sp,

@@ -804,8 +804,13 @@ bp_status bp_solver_set_block_sizes(bp_solver *s, const int sizes[6]) {
 bp_status bp_solver_set_walk(bp_solver *s, int lambda_staged, int position_flat, int omega_normal_staged,
                              int vorticity_tension_flat) {
     if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
-    s->staged_lambda = lambda_staged != 0;
-    s->staged_post_a = omega_normal_staged != 0;
+    // 1 = the default region size
+    const int cl = lambda_staged == 1 ? kStageCapDefault : lambda_staged;
+    const int co = omega_normal_staged == 1 ? kStageCapDefault : omega_normal_staged;
+    if ((cl != 0 && !stage_cap_valid(cl)) || (co != 0 && !stage_cap_valid(co)))
+        return fail(BP_ERR_INVALID_ARG, "staged walks take 0 (per lane), 1 (default region), 128, 256 or 512");
+    s->staged_lambda = cl;
+    s->staged_post_a = co;
     s->flat_position = position_flat != 0;
     s->flat_post_b = vorticity_tension_flat != 0;
     s->drop_graph();
@@ -837,34 +842,67 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
     const int fams[6] = {F_LAMBDA, F_POSITION, F_POST_A, F_POST_B, F_XSPH, F_PP};
     float best_ms[6];
     int best_blk[6], best_flat[6];
-    for (int f = 0; f < 6; ++f) { best_ms[f] = 1e30f; best_blk[f] = 256; best_flat[f] = 1; }
-    // the trials advance a copy of the state; the caller's state is restored after
+    for (int f = 0; f < 6; ++f) {
+        best_ms[f] = 1e30f;
+        best_blk[f] = 256;
+        best_flat[f] = (f == 0 || f == 2) ? kStageCapDefault : 1;
+    }
+    // Every trial starts from the caller's state (the passes cost more as a
+    // scene evolves, so trials run one after another on an advancing state
+    // would favour the early ones), and that state is restored at the end.
     std::vector<float> keep_x(3 * (size_t)s->N), keep_v(3 * (size_t)s->N), keep_q(4 * (size_t)s->nr + 4),
         keep_cm(3 * (size_t)s->nr + 3);
     std::vector<int> keep_map(s->N);
     bp_status st = bp_get_state(s, keep_x.data(), keep_v.data(), keep_map.data());
     if (st == BP_OK && s->nr > 0) st = bp_get_rigid_state(s, keep_q.data(), keep_cm.data());
     if (st != BP_OK) return st;
+    auto restore = [&]() {
+        bp_status r = bp_set_state(s, keep_x.data(), keep_v.data(), keep_map.data());
+        if (r == BP_OK && s->nr > 0) r = bp_set_rigid_state(s, keep_q.data());
+        return r;
+    };
     const bool g = s->use_graph;
     s->use_graph = false;
     st = bp_solver_enable_kernel_timing(s, 1);
     if (st != BP_OK) return st;
     // every workgroup size with each walk form of the lambda, position,
-    // omega/normal and vorticity/tension passes (the others have one form)
-    for (int t = 0; t < 8; ++t) {
-        const int c = t >> 1, flat = t & 1;
-        s->blk_lambda = s->blk_position = s->blk_post_a = s->blk_post_b = s->blk_xsph = s->blk_pp = cand[c];
-        s->staged_lambda = s->flat_position = s->staged_post_a = s->flat_post_b = flat;
-        for (int f = 0; f < F_COUNT; ++f) { s->fam_ms[f] = 0.f; s->fam_n[f] = 0; }
-        st = run_substeps(s, substeps_per_trial, dt);
-        if (st == BP_OK) st = collect_timing(s);
-        if (st != BP_OK) break;
-        for (int f = 0; f < 6; ++f) {
-            const int fam = fams[f];
-            if (s->fam_n[fam] == 0) continue;
-            const float avg = s->fam_ms[fam] / (float)s->fam_n[fam];
-            if (avg < best_ms[f]) { best_ms[f] = avg; best_blk[f] = cand[c]; best_flat[f] = flat; }
+    // omega/normal and vorticity/tension passes (the others have one form):
+    // form 0 = per lane / per chunk; forms 1-3 = staged with each region size
+    // (kStageCaps) for the lambda and omega/normal passes, flat (form 2) or
+    // per chunk (forms 1, 3) for the position and vorticity/tension passes.
+    // Two rounds over the candidates; a pass keeps the candidate with the
+    // lowest mean launch time summed over both rounds.
+    constexpr int kTrials = 16, kRounds = 2;
+    float sum_ms[kTrials][6] = {};
+    bool seen[kTrials][6] = {};
+    for (int round = 0; round < kRounds && st == BP_OK; ++round)
+        for (int t = 0; t < kTrials; ++t) {
+            const int c = t >> 2, form = t & 3;
+            s->blk_lambda = s->blk_position = s->blk_post_a = s->blk_post_b = s->blk_xsph = s->blk_pp = cand[c];
+            s->staged_lambda = s->staged_post_a = form ? kStageCaps[form - 1] : 0;
+            s->flat_position = s->flat_post_b = form == 2 ? 1 : 0;
+            st = restore();
+            if (st != BP_OK) break;
+            for (int f = 0; f < F_COUNT; ++f) { s->fam_ms[f] = 0.f; s->fam_n[f] = 0; }
+            st = run_substeps(s, substeps_per_trial, dt);
+            if (st == BP_OK) st = collect_timing(s);
+            if (st != BP_OK) break;
+            for (int f = 0; f < 6; ++f) {
+                const int fam = fams[f];
+                if (s->fam_n[fam] == 0) continue;
+                sum_ms[t][f] += s->fam_ms[fam] / (float)s->fam_n[fam];
+                seen[t][f] = true;
+            }
         }
+    for (int t = 0; t < kTrials && st == BP_OK; ++t) {
+        const int c = t >> 2, form = t & 3;
+        const int cap = form ? kStageCaps[form - 1] : 0, flat = form == 2 ? 1 : 0;
+        for (int f = 0; f < 6; ++f)
+            if (seen[t][f] && sum_ms[t][f] < best_ms[f]) {
+                best_ms[f] = sum_ms[t][f];
+                best_blk[f] = cand[c];
+                best_flat[f] = (f == 0 || f == 2) ? cap : flat;  // region size or flat flag
+            }
     }
     s->blk_lambda = best_blk[0];
     s->blk_position = best_blk[1];
@@ -885,25 +923,27 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
         for (int f = 0; f < 2 && st == BP_OK; ++f) {
             s->fuse = f == 1;
             float ms = 0.f;
-            st = time_substeps(s, substeps_per_trial, dt, &ms);
+            st = restore();
+            if (st == BP_OK) st = time_substeps(s, substeps_per_trial, dt, &ms);
             if (ms < fused_ms[f]) fused_ms[f] = ms;
         }
     s->fuse = fused_ms[1] <= fused_ms[0];
     s->use_graph = g;
     s->drop_graph();
     if (st != BP_OK) return st;
-    st = bp_set_state(s, keep_x.data(), keep_v.data(), keep_map.data());
-    if (st == BP_OK && s->nr > 0) st = bp_set_rigid_state(s, keep_q.data());
+    st = restore();
     if (st != BP_OK) return st;
     if (report && report_len > 0)
         std::snprintf(report, (size_t)report_len,
                       "{\"fluid_lambda\": %d, \"fluid_position\": %d, \"omega_normal\": %d, "
                       "\"vorticity_tension\": %d, \"xsph\": %d, \"particle_particle\": %d, "
                       "\"iteration_fusion\": %d, \"lambda_walk\": \"%s\", \"position_walk\": \"%s\", "
-                      "\"omega_normal_walk\": \"%s\", \"vorticity_tension_walk\": \"%s\"}",
+                      "\"omega_normal_walk\": \"%s\", \"vorticity_tension_walk\": \"%s\", "
+                      "\"lambda_stage_records\": %d, \"omega_normal_stage_records\": %d}",
                       best_blk[0], best_blk[1], best_blk[2], best_blk[3], best_blk[4], best_blk[5], s->fuse ? 1 : 0,
                       s->staged_lambda ? "staged" : "global", s->flat_position ? "flat" : "chunk",
-                      s->staged_post_a ? "staged" : "global", s->flat_post_b ? "flat" : "chunk");
+                      s->staged_post_a ? "staged" : "global", s->flat_post_b ? "flat" : "chunk", s->staged_lambda,
+                      s->staged_post_a);
     return BP_OK;
 }
 

@@ -120,10 +120,12 @@ struct bp_solver {
     // neighbour-pass workgroup sizes (bp_solver_autotune)
     int blk_lambda = 128, blk_position = 128, blk_post_a = 128, blk_post_b = 128, blk_xsph = 128, blk_pp = 128;
     // walk forms (bp_solver_set_walk, bp_solver_autotune): the lambda and
-    // omega/normal passes' candidates staged in LDS (1) or read from global
-    // memory (0); the position and vorticity/tension passes flat (1, bodies
-    // over the whole walk) or per chunk (0)
-    int staged_lambda = 1, flat_position = 1, staged_post_a = 1, flat_post_b = 1;
+    // omega/normal passes' candidates staged in an LDS region of that many
+    // records per wave and column offset (128, 256 or 512) or read from
+    // global memory (0); the position and vorticity/tension passes flat (1,
+    // bodies over the whole walk) or per chunk (0)
+    int staged_lambda = bp::kStageCapDefault, flat_position = 1, staged_post_a = bp::kStageCapDefault,
+        flat_post_b = 1;
 
     // hipGraph of two substeps (the ping-pong buffers return to their start)
     bool use_graph = false;

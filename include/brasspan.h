@@ -271,9 +271,12 @@ bp_status bp_backward(bp_solver *s, const float *grad_x, const float *grad_v, fl
 bp_status bp_solver_set_fusion(bp_solver *s, int enable);
 
 /* Walk forms of the neighbour passes (default 1 for all four).  Lambda and
- * omega/normal passes: 1 = each wave stages its candidates' records in LDS
- * and screens and fetches partners from there, 0 = every lane reads its
- * candidates from global memory.  Position and vorticity/tension passes:
+ * omega/normal passes: 128, 256 or 512 = each wave stages its candidates'
+ * records in an LDS region of that many records per column offset (about
+ * eight per occupied cell; a column offset whose candidates do not fit reads
+ * global memory) and screens and fetches partners from there, 1 = the same
+ * with 256, 0 = every lane reads its candidates from global memory; any other
+ * value is BP_ERR_INVALID_ARG.  Position and vorticity/tension passes:
  * 1 = the candidate walk records each 32-candidate chunk's partner mask and
  * the pair terms run after the walk over all partners, 0 = the pair terms of
  * each chunk run right after its screen.  Same pairs in the same order, so
@@ -284,14 +287,17 @@ bp_status bp_solver_set_walk(bp_solver *s, int lambda_staged, int position_flat,
 
 /* Chooses the workgroup size of each neighbour pass among 64..512 and the
  * walk forms of the lambda, position, omega/normal and vorticity/tension
- * passes (bp_solver_set_walk) by timing
- * `substeps_per_trial` substeps of size dt per candidate, and iteration fusion
- * on or off (best of two trials each), then restores the particle and rigid
- * state it started from (results do not depend on any of these choices).
+ * passes (bp_solver_set_walk, including the staged walks' region sizes) by
+ * timing `substeps_per_trial` substeps of size dt per candidate, and
+ * iteration fusion on or off (best of two trials each).  Every trial starts
+ * from the particle and rigid state the call started from, and that state is
+ * restored at the end (results do not depend on any of these choices).
  * Writes a JSON object {pass name: size, "iteration_fusion": 0|1,
  * "lambda_walk"/"omega_normal_walk": "staged"|"global",
- * "position_walk"/"vorticity_tension_walk": "flat"|"chunk"} into `report`
- * (may be NULL).  No counterpart in the reference. */
+ * "position_walk"/"vorticity_tension_walk": "flat"|"chunk",
+ * "lambda_stage_records"/"omega_normal_stage_records": 0|128|256|512 (the
+ * staged region size, 0 for the per-lane walk)} into `report` (may be NULL;
+ * 512 bytes hold it).  No counterpart in the reference. */
 bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, char *report, int report_len);
 
 /* Sets the neighbour passes' workgroup sizes, each 64, 128, 256 or 512, in

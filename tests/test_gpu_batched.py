@@ -110,8 +110,12 @@ def test_autotune_keeps_results():
     print("autotune report", report)
     passes = {"fluid_lambda", "fluid_position", "omega_normal", "vorticity_tension", "xsph", "particle_particle"}
     assert set(report) == passes | {"iteration_fusion", "lambda_walk", "position_walk", "omega_normal_walk",
-                                    "vorticity_tension_walk"}
+                                    "vorticity_tension_walk", "lambda_stage_records", "omega_normal_stage_records"}
     assert report["lambda_walk"] in ("staged", "global") and report["omega_normal_walk"] in ("staged", "global")
+    for walk, records in (("lambda_walk", "lambda_stage_records"),
+                          ("omega_normal_walk", "omega_normal_stage_records")):
+        # a staged walk names its LDS region size, the per-lane walk none
+        assert report[records] in ((128, 256, 512) if report[walk] == "staged" else (0,))
     assert all(report[k] in (64, 128, 256, 512) for k in passes)
     assert report["iteration_fusion"] in (0, 1)
     assert report["position_walk"] in ("flat", "chunk") and report["vorticity_tension_walk"] in ("flat", "chunk")

@@ -363,21 +363,24 @@ def test_iteration_fusion_is_bit_identical(label):
 
 
 @pytest.mark.parametrize("label", ["akinci block 8k", "mixed phase small"])
-def test_walk_forms_are_bit_identical(label):
+@pytest.mark.parametrize("records", [128, 256, 512])
+def test_walk_forms_are_bit_identical(label, records):
     """The walk forms visit the same pairs in the same order: the lambda and
-    omega/normal passes with their candidates staged in LDS or read from
-    global memory, the position and vorticity/tension passes flat (pair terms
-    after the whole candidate walk) or per chunk.  Same bits over several substeps, on a scene that runs
-    vorticity confinement and Akinci tension and on one with solids, cloth and
-    wind (deliveries to partners whose search misses the pair)."""
+    omega/normal passes with their candidates staged in an LDS region of
+    `records` records per wave and column offset (a column offset that does
+    not fit reads global memory) or read from global memory, the position
+    and vorticity/tension passes flat (pair terms after the whole candidate
+    walk) or per chunk.  Same bits over several substeps, on a scene that
+    runs vorticity confinement and Akinci tension and on one with solids,
+    cloth and wind (deliveries to partners whose search misses the pair)."""
     if label == "akinci block 8k":
         sc, prm = scenes.large_block(dims=(32, 16, 16), grid=(256, 128, 128)), scenes.large_block_params()
     else:
         sc = scenes.mixed_phase(fluid_dims=(16, 16, 16), num_boxes=6, cloth_n=16, grid=(128, 128, 128))
         prm = scenes.mixed_phase_params()
-    case(f"walk forms: {label}")
+    case(f"walk forms: {label}, staged region {records}")
     a, b = bp.Solver(sc, prm), bp.Solver(sc, prm)
-    a.set_walk(True, True, True, True)
+    a.set_walk(records, True, records, True)
     b.set_walk(False, False, False, False)
     a.step(4, 4 * DT)
     b.step(4, 4 * DT)
@@ -386,6 +389,23 @@ def test_walk_forms_are_bit_identical(label):
     assert np.array_equal(ma, mb) and np.array_equal(pa, pb) and np.array_equal(va, vb)
     a.close()
     b.close()
+
+
+def test_set_walk_rejects_region_sizes_not_built():
+    """bp_solver_set_walk takes 0, 1, 128, 256 or 512 for a staged pass and
+    returns BP_ERR_INVALID_ARG for anything else, leaving the solver usable."""
+    sc, prm = scenes.dam_break(dims=(8, 8, 8), grid=(64, 64, 64)), scenes.dam_break_params()
+    case("set_walk argument check")
+    s = bp.Solver(sc, prm)
+    for bad in (2, 64, 300, 1024, -1):
+        with pytest.raises(bp.BrassPanError):
+            s.set_walk(bad, True, True, True)
+        with pytest.raises(bp.BrassPanError):
+            s.set_walk(True, True, bad, True)
+    s.set_walk(512, False, 128, False)
+    s.step(1, DT)
+    assert np.isfinite(s.get_state()[0]).all()
+    s.close()
 
 
 def test_substep_parity_dihedral_bending():
