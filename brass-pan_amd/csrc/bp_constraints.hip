@@ -48,13 +48,15 @@ __global__ __launch_bounds__(kBlock) void k_mark_cells(const int *__restrict__ s
         fluid_stamp[key] = st;
         return;
     }
-    const GridDesc &G = sp.at(i).G;
-    int g[3];
-    unpack_cell(G.pack_addr(key), g);
-    for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx)
+    sp.with_scene_ref(i, [&](const SceneConst &S) {
+        const GridDesc &G = S.G;
+        int g[3];
+        unpack_cell(G.pack_addr(key), g);
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx)
 #pragma unroll
-            for (int dz = -1; dz <= 1; ++dz) cell_stamp[G.addr(g[0] + dx, g[1] + dy, g[2] + dz)] = st;
+                for (int dz = -1; dz <= 1; ++dz) cell_stamp[G.addr(g[0] + dx, g[1] + dy, g[2] + dz)] = st;
+    });
 }
 
 // particleParticleCollisionConstraint (particleparticle.cuh:5-101) + Δx apply.

@@ -207,33 +207,35 @@ __global__ __launch_bounds__(kBlock) void k_position_apply(const float4 *__restr
                                                            float4 *__restrict__ v_out, int n, SP sp, Epilogue E) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const SceneConst &S = sp.at(k);
-    const FluidParams &F = S.F;
-    const float4 a = xs[k];
-    f3 sum = xyz(sum_in[k]);
-    const unsigned long long cx = corr[3 * k], cy = corr[3 * k + 1], cz = corr[3 * k + 2];
-    if ((cx | cy | cz) != 0ull) {
-        const float inv = (float)(1.0 / kCorrScale);
-        sum.x += (float)(long long)cx * inv;
-        sum.y += (float)(long long)cy * inv;
-        sum.z += (float)(long long)cz * inv;
-        corr[3 * k] = 0ull;
-        corr[3 * k + 1] = 0ull;
-        corr[3 * k + 2] = 0ull;
-    }
-    f3 out = xyz(a) + sum * F.inv_rest_density;
-    if (E.planes_next || E.finish) {
-        float4 x4 = x_io[k];
-        const f3 xp = xyz(x4);
-        if (E.planes_next) out = plane_collide_one(out, xp, S.pl, S.radius, S.friction_static, S.friction_dynamic);
-        if (E.finish) {  // updateVelocity + updatePositions (particlesolver.cuh:71-81,121-137)
-            v_out[k] = f4((out - xp) * E.inv_dt, (E.vw && phase_of(a) < 0) ? E.vw[k] : 0.0f);
-            const float d2 = length2(out - xp);
-            if (d2 >= E.sleep_eps2 || phase_of(a) < 0) x_io[k] = f4(out, x4.w);
+    sp.with_scene_ref(k, [&](const SceneConst &S) {
+        const FluidParams &F = S.F;
+        const float4 a = xs[k];
+        f3 sum = xyz(sum_in[k]);
+        const unsigned long long cx = corr[3 * k], cy = corr[3 * k + 1], cz = corr[3 * k + 2];
+        if ((cx | cy | cz) != 0ull) {
+            const float inv = (float)(1.0 / kCorrScale);
+            sum.x += (float)(long long)cx * inv;
+            sum.y += (float)(long long)cy * inv;
+            sum.z += (float)(long long)cz * inv;
+            corr[3 * k] = 0ull;
+            corr[3 * k + 1] = 0ull;
+            corr[3 * k + 2] = 0ull;
         }
-    }
-    xs_out[k] = f4(out, a.w);
-    if (E.lin.planes) store_lambda_in(E.lin, k, out, E.m[k], phase_of(a));
+        f3 out = xyz(a) + sum * F.inv_rest_density;
+        if (E.planes_next || E.finish) {
+            float4 x4 = x_io[k];
+            const f3 xp = xyz(x4);
+            if (E.planes_next)
+                out = plane_collide_one(out, xp, S.pl, S.radius, S.friction_static, S.friction_dynamic);
+            if (E.finish) {  // updateVelocity + updatePositions (particlesolver.cuh:71-81,121-137)
+                v_out[k] = f4((out - xp) * E.inv_dt, (E.vw && phase_of(a) < 0) ? E.vw[k] : 0.0f);
+                const float d2 = length2(out - xp);
+                if (d2 >= E.sleep_eps2 || phase_of(a) < 0) x_io[k] = f4(out, x4.w);
+            }
+        }
+        xs_out[k] = f4(out, a.w);
+        if (E.lin.planes) store_lambda_in(E.lin, k, out, E.m[k], phase_of(a));
+    });
 }
 
 // fluidOmega (fluid.cuh:231-279) and fluidNormal (fluid.cuh:387-434) in one

@@ -38,11 +38,12 @@ __global__ __launch_bounds__(kBlock) void k_predict_hash(const float4 *__restric
         }
         if (i == n - 1) ranges_flat[2 * u + 1] = -1;
     }
-    const GridDesc &G = sp.at(i).G;
-    const f3 xs = predict(x[i], v[i], dt, gstep);
-    int g[3];
-    G.pos(xs, g);
-    keys[i] = G.addr(g[0], g[1], g[2]);
+    sp.with_scene_ref(i, [&](const SceneConst &S) {
+        const f3 xs = predict(x[i], v[i], dt, gstep);
+        int g[3];
+        S.G.pos(xs, g);
+        keys[i] = S.G.addr(g[0], g[1], g[2]);
+    });
 }
 
 // ---------------------------------------------------------------- radix sort
@@ -228,41 +229,47 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const int *__restrict__ perm
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int p = perm[i];
-    const SceneConst &S = sp.at(p);  // the source slot's scene (scenes never mix)
-    const Planes &pl = S.pl;
-    const float4 x4 = x_in[p];
-    const float4 v4 = v_in[p];
-    const float m = m_in[p];
-    const int ph = ph_in[p];
-    const int o = o_in[p];
-    f3 xs = predict(x4, v4, dt, gstep);
-    f3 xp = xyz(x4);
-    float w = 0.0f;
-    if (need_w) {
-        const float scale = powf(2.7182818284f, -mass_k * xp.y);
-        w = 1.0f / (scale * m);
-    }
-    for (int s = 0; s < stab_passes; ++s) {
-        for (int q = 0; q < pl.n; ++q) {
-            const f3 on = mk3(pl.o[q][0], pl.o[q][1], pl.o[q][2]);
-            const f3 nn = mk3(pl.nrm[q][0], pl.nrm[q][1], pl.nrm[q][2]);
-            const f3 o2p = on - xp;
-            const float d = dot(o2p, nn) + S.radius;
-            if (d <= 0.0f) continue;
-            xp += d * nn;
-            xs += d * nn;
+    // the source slot's scene (scenes never mix), read wave-uniformly; the
+    // plane loops are unrolled over the plane slots so the planes stay in
+    // scalar registers
+    sp.with_scene_ref(p, [&](const SceneConst &S) {
+        const Planes &pl = S.pl;
+        const float4 x4 = x_in[p];
+        const float4 v4 = v_in[p];
+        const float m = m_in[p];
+        const int ph = ph_in[p];
+        const int o = o_in[p];
+        f3 xs = predict(x4, v4, dt, gstep);
+        f3 xp = xyz(x4);
+        float w = 0.0f;
+        if (need_w) {
+            const float scale = powf(2.7182818284f, -mass_k * xp.y);
+            w = 1.0f / (scale * m);
         }
-    }
-    if (collide) xs = plane_collide_one(xs, xp, pl, S.radius, S.friction_static, S.friction_dynamic);
-    xs_out[i] = f4(xs, __int_as_float(ph));
-    store_lambda_in(L, i, xs, m, ph);
-    x_out[i] = f4(xp, w);
-    v_out[i] = v4;
-    m_out[i] = m;
-    ph_out[i] = ph;
-    o_out[i] = o;
-    if (sid_out) sid_out[i] = sid_in[p];
-    map_to_new[o] = i;
+        for (int s = 0; s < stab_passes; ++s) {
+#pragma unroll
+            for (int q = 0; q < kMaxPlanes; ++q) {
+                if (q >= pl.n) break;
+                const f3 on = mk3(pl.o[q][0], pl.o[q][1], pl.o[q][2]);
+                const f3 nn = mk3(pl.nrm[q][0], pl.nrm[q][1], pl.nrm[q][2]);
+                const f3 o2p = on - xp;
+                const float d = dot(o2p, nn) + S.radius;
+                if (d <= 0.0f) continue;
+                xp += d * nn;
+                xs += d * nn;
+            }
+        }
+        if (collide) xs = plane_collide_one(xs, xp, pl, S.radius, S.friction_static, S.friction_dynamic);
+        xs_out[i] = f4(xs, __int_as_float(ph));
+        store_lambda_in(L, i, xs, m, ph);
+        x_out[i] = f4(xp, w);
+        v_out[i] = v4;
+        m_out[i] = m;
+        ph_out[i] = ph;
+        o_out[i] = o;
+        if (sid_out) sid_out[i] = sid_in[p];
+        map_to_new[o] = i;
+    });
 }
 
 // ------------------------------------------------------------ launchers

@@ -877,8 +877,13 @@ __device__ __forceinline__ void for_each_neighbor_staged(const GridDesc &G, cons
                     if (q == grp) ldsbase = sbase[q];
             }
         }
-        // ---- this lane's runs of the column, chunk by chunk (as walk_chunks)
-        auto run = [&](int s, int e) {
+        // ---- this lane's runs of the column, chunk by chunk (as walk_chunks).
+        // The staged and global forms are separate instances of the chunk
+        // loop: merged into one loop, the staged form's LDS reads waited for
+        // every global load in flight (the two forms' loads shared registers),
+        // including the next column's prefetched ranges.
+        auto run = [&](auto lds_c, int s, int e) {
+            constexpr bool kLds = decltype(lds_c)::value;
             if (s < jmin) s = jmin;
             for (int base = s; base < e; base += 32, ++c) {
                 const int cnt = (e - base) < 32 ? (e - base) : 32;
@@ -893,7 +898,7 @@ __device__ __forceinline__ void for_each_neighbor_staged(const GridDesc &G, cons
                     if (keep == 0u && !rec.out) continue;
                     for (int t0 = 0; t0 < cnt; t0 += 8) {
                         float4 bb[8];
-                        if (staged) {
+                        if constexpr (kLds) {
                             const lds_f4 *q = lds + (base + t0 + ldsbase);
 #pragma unroll
                             for (int u = 0; u < 8; ++u) bb[u] = as_float4(q[u]);
@@ -934,8 +939,12 @@ __device__ __forceinline__ void for_each_neighbor_staged(const GridDesc &G, cons
                         [&](int j) {
                             StagedRec<NR> R;
 #pragma unroll
-                            for (int a = 0; a < NR; ++a)
-                                R.r[a] = staged ? as_float4(lds[a * kRegion + j + ldsbase]) : ld32(srcs[a], j);
+                            for (int a = 0; a < NR; ++a) {
+                                if constexpr (kLds)
+                                    R.r[a] = as_float4(lds[a * kRegion + j + ldsbase]);
+                                else
+                                    R.r[a] = ld32(srcs[a], j);
+                            }
                             return R;
                         },
                         use);
@@ -959,7 +968,12 @@ __device__ __forceinline__ void for_each_neighbor_staged(const GridDesc &G, cons
 #pragma unroll 1
         for (int rr = 0; rr < 2; ++rr) {
             const int rs = rr ? s1 : s0, re = rr ? e1 : e0;
-            if (rs >= 0) run(rs, re);
+            if (rs >= 0) {
+                if (staged)
+                    run(std::true_type{}, rs, re);
+                else
+                    run(std::false_type{}, rs, re);
+            }
         }
         if (staged) {  // every lane is done with the region before the next fill
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
