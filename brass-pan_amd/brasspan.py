@@ -96,6 +96,7 @@ SIGNATURES = {
     "bp_solver_autotune": (_S, [_P, C.c_float, C.c_int, C.c_char_p, C.c_int]),
     "bp_solver_set_fusion": (_S, [_P, C.c_int]),
     "bp_solver_set_walk": (_S, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "bp_solver_set_partner_fetch": (_S, [_P, C.c_int]),
     "bp_solver_set_block_sizes": (_S, [_P, C.POINTER(C.c_int)]),
     "bp_stream_copy_bandwidth": (_S, [C.c_size_t, C.c_int, C.POINTER(C.c_float)]),
     "bp_forward_checkpointed": (_S, [_P, C.c_int, C.c_float, C.c_int]),
@@ -267,6 +268,11 @@ class Solver:
         _check(self.lib.bp_solver_set_walk(self.h, int(lambda_staged), int(bool(position_flat)),
                                            int(omega_normal_staged), int(bool(vorticity_tension_flat))))
 
+    def set_partner_fetch(self, position_partners: int = 2):
+        """Partners the position pass fetches per round trip, 2 or 4
+        (bp_solver_set_partner_fetch; results are bit-identical)."""
+        _check(self.lib.bp_solver_set_partner_fetch(self.h, int(position_partners)))
+
     def autotune(self, dt: float, substeps_per_trial: int = 2) -> dict:
         import json
         buf = C.create_string_buffer(512)
@@ -289,6 +295,7 @@ class Solver:
         self.set_walk(region("lambda_walk", "lambda_stage_records"), report["position_walk"] == "flat",
                       region("omega_normal_walk", "omega_normal_stage_records"),
                       report["vorticity_tension_walk"] == "flat")
+        self.set_partner_fetch(int(report.get("position_partners", 2)))
 
     # ---------------------------------------------------------- adjoint
     def forward_checkpointed(self, substeps: int, subdt: float, every: int = 8):

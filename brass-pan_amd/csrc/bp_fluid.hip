@@ -126,7 +126,9 @@ __device__ __forceinline__ unsigned long long to_fixed(float v) {
 //   k's cell.
 // A pair k owns whose partner j's search does not reach k's cell is delivered
 // to j through corr[j] (fixed point).  k_position_apply adds both parts.
-template <class SP, bool Flat>
+// NB: partners fetched together per round trip (2 or 4; the pass waits on
+// these loads, and more in flight per wave costs registers).
+template <class SP, bool Flat, int NB>
 __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, Flat ? 5 : 6) void k_position(const float4 *__restrict__ xs,
                                                      const float4 *__restrict__ lam_rec,
                                                      const int *__restrict__ build_keys,
@@ -166,7 +168,7 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, Flat ? 5 : 6) void k_posi
         rec.in = kf ? srec : nullptr;
         rec.slot = k;
         rec.n = n;
-        walk_form<Flat, kBodyBatch>(
+        walk_form<Flat, NB>(
             G, ranges, g, xs, 0,
             // (fluid particles replay the lambda pass's masks; a solid screens for
             // fluid partners only)
@@ -349,7 +351,9 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 1) void k_post_b(const fl
         // one 16-byte fetch of the second record whatever is used of it (a
         // conditional fetch is split into per-component loads)
         const float4 *__restrict__ second = do_akinci ? normal : xom;
-        walk_form<Flat, kBodyBatch>(
+        // four partners per round trip: the pass mostly waits on these loads
+        // (batch 0.79 -> 0.72 ms, 4M block 0.95 -> 0.90 ms against two)
+        walk_form<Flat, 4>(
             G, ranges, g, xs, 0,
             [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
             [&](int j) { return Partner{ld32(xom, j), ld32(second, j)}; },
@@ -446,13 +450,19 @@ template <class SP>
 void launch_position(const float4 *xs, const float4 *lam_rec, const int *build_keys, const int2 *ranges,
                      const uint32_t *srec, const uint32_t *fluid_stamp, const uint32_t *stamp, float4 *sum_tmp,
                      unsigned long long *corr, float4 *xs_out, float4 *x_io, float4 *v_out, int n, const SP &sp,
-                     const Epilogue &E, int block, int flat, hipStream_t st) {
-    if (flat)
-        hipLaunchKernelGGL((k_position<SP, true>), dim3(grid_blocks(n, block)), dim3(block), flat_lds_bytes(block), st,
-                           xs, lam_rec, build_keys, ranges, srec, fluid_stamp, stamp, sum_tmp, corr, n, sp);
-    else
-        hipLaunchKernelGGL((k_position<SP, false>), dim3(grid_blocks(n, block)), dim3(block), 0, st, xs, lam_rec,
-                           build_keys, ranges, srec, fluid_stamp, stamp, sum_tmp, corr, n, sp);
+                     const Epilogue &E, int block, int flat, int partners, hipStream_t st) {
+    const dim3 grid(grid_blocks(n, block)), blk(block);
+    const size_t lds = flat ? flat_lds_bytes(block) : 0;
+#define BP_POSITION(F, NB)                                                                                      \
+    hipLaunchKernelGGL((k_position<SP, F, NB>), grid, blk, lds, st, xs, lam_rec, build_keys, ranges, srec,    \
+                       fluid_stamp, stamp, sum_tmp, corr, n, sp)
+    switch ((flat ? 1 : 0) | (partners == 4 ? 2 : 0)) {
+    case 0: BP_POSITION(false, 2); break;
+    case 1: BP_POSITION(true, 2); break;
+    case 2: BP_POSITION(false, 4); break;
+    default: BP_POSITION(true, 4); break;
+    }
+#undef BP_POSITION
     hipLaunchKernelGGL(k_position_apply<SP>, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, sum_tmp, corr, xs_out,
                        x_io, v_out, n, sp, E);
 }
@@ -519,7 +529,7 @@ void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const ui
     template void launch_position<SP>(const float4 *, const float4 *, const int *, const int2 *,                \
                                       const uint32_t *, const uint32_t *, const uint32_t *, float4 *,           \
                                       unsigned long long *, float4 *, float4 *, float4 *, int, const SP &,      \
-                                      const Epilogue &, int, int, hipStream_t);                                 \
+                                      const Epilogue &, int, int, int, hipStream_t);                            \
     template void launch_post_a<SP>(const float4 *, const float4 *, const float *, const int2 *, float4 *,      \
                                     float4 *, float4 *, uint32_t *, uint32_t *, const LambdaIn *, int, int,     \
                                     const SP &, int, int, int, int, hipStream_t);                               \

@@ -243,7 +243,7 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
             TIMED(F_POSITION, launch_position(s->xs.cur(), s->lam_rec, sk, s->ranges, H.rec ? H.rec[it] : s->srec,
                                               s->fluid_stamp, s->stamp, s->sum_tmp, s->corr, s->xs.alt(),
                                               s->x.cur(), s->v.cur(), N, sp(), E, s->blk_position,
-                                              s->flat_position, st));
+                                              s->flat_position, s->pos_partners, st));
             s->xs.flip();
             planes_done = E.planes_next;
             finished = E.finish;
@@ -817,6 +817,15 @@ bp_status bp_solver_set_walk(bp_solver *s, int lambda_staged, int position_flat,
     return BP_OK;
 }
 
+bp_status bp_solver_set_partner_fetch(bp_solver *s, int position_partners) {
+    if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
+    if (position_partners != 2 && position_partners != 4)
+        return fail(BP_ERR_INVALID_ARG, "the position pass fetches 2 or 4 partners per round trip");
+    s->pos_partners = position_partners;
+    s->drop_graph();
+    return BP_OK;
+}
+
 namespace {
 // GPU time of `count` substeps (HIP events on the handle's stream)
 bp_status time_substeps(bp_solver *s, int count, float dt, float *ms) {
@@ -866,12 +875,13 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
     st = bp_solver_enable_kernel_timing(s, 1);
     if (st != BP_OK) return st;
     // every workgroup size with each walk form of the lambda, position,
-    // omega/normal and vorticity/tension passes (the others have one form):
-    // form 0 = per lane / per chunk; forms 1-3 = staged with each region size
-    // (kStageCaps) for the lambda and omega/normal passes, flat (form 2) or
-    // per chunk (forms 1, 3) for the position and vorticity/tension passes.
-    // Two rounds over the candidates; a pass keeps the candidate with the
-    // lowest mean launch time summed over both rounds.
+    // omega/normal and vorticity/tension passes (the others have one form).
+    // Form 0-3: the lambda and omega/normal passes per lane (0) or staged
+    // with each region size (kStageCaps); the position pass per chunk (0, 1)
+    // or flat (2, 3), fetching 2 (0, 2) or 4 (1, 3) partners per round trip;
+    // the vorticity/tension pass per chunk (0, 1) or flat (2, 3).  Two rounds
+    // over the candidates; a pass keeps the candidate with the lowest mean
+    // launch time summed over both rounds.
     constexpr int kTrials = 16, kRounds = 2;
     float sum_ms[kTrials][6] = {};
     bool seen[kTrials][6] = {};
@@ -880,7 +890,8 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
             const int c = t >> 2, form = t & 3;
             s->blk_lambda = s->blk_position = s->blk_post_a = s->blk_post_b = s->blk_xsph = s->blk_pp = cand[c];
             s->staged_lambda = s->staged_post_a = form ? kStageCaps[form - 1] : 0;
-            s->flat_position = s->flat_post_b = form == 2 ? 1 : 0;
+            s->flat_position = s->flat_post_b = form >= 2 ? 1 : 0;
+            s->pos_partners = (form & 1) ? 4 : 2;
             st = restore();
             if (st != BP_OK) break;
             for (int f = 0; f < F_COUNT; ++f) { s->fam_ms[f] = 0.f; s->fam_n[f] = 0; }
@@ -894,14 +905,16 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
                 seen[t][f] = true;
             }
         }
+    int best_partners = 2;
     for (int t = 0; t < kTrials && st == BP_OK; ++t) {
         const int c = t >> 2, form = t & 3;
-        const int cap = form ? kStageCaps[form - 1] : 0, flat = form == 2 ? 1 : 0;
+        const int cap = form ? kStageCaps[form - 1] : 0, flat = form >= 2 ? 1 : 0;
         for (int f = 0; f < 6; ++f)
             if (seen[t][f] && sum_ms[t][f] < best_ms[f]) {
                 best_ms[f] = sum_ms[t][f];
                 best_blk[f] = cand[c];
                 best_flat[f] = (f == 0 || f == 2) ? cap : flat;  // region size or flat flag
+                if (f == 1) best_partners = (form & 1) ? 4 : 2;
             }
     }
     s->blk_lambda = best_blk[0];
@@ -914,6 +927,7 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
     s->flat_position = best_flat[1];
     s->staged_post_a = best_flat[2];
     s->flat_post_b = best_flat[3];
+    s->pos_partners = best_partners;
     (void)bp_solver_enable_kernel_timing(s, 0);
     // iteration fusion: the epilogue passes (next iteration's plane
     // collisions, the velocity/position update, the lambda pass's inputs) in
@@ -939,11 +953,12 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
                       "\"vorticity_tension\": %d, \"xsph\": %d, \"particle_particle\": %d, "
                       "\"iteration_fusion\": %d, \"lambda_walk\": \"%s\", \"position_walk\": \"%s\", "
                       "\"omega_normal_walk\": \"%s\", \"vorticity_tension_walk\": \"%s\", "
-                      "\"lambda_stage_records\": %d, \"omega_normal_stage_records\": %d}",
+                      "\"lambda_stage_records\": %d, \"omega_normal_stage_records\": %d, "
+                      "\"position_partners\": %d}",
                       best_blk[0], best_blk[1], best_blk[2], best_blk[3], best_blk[4], best_blk[5], s->fuse ? 1 : 0,
                       s->staged_lambda ? "staged" : "global", s->flat_position ? "flat" : "chunk",
                       s->staged_post_a ? "staged" : "global", s->flat_post_b ? "flat" : "chunk", s->staged_lambda,
-                      s->staged_post_a);
+                      s->staged_post_a, s->pos_partners);
     return BP_OK;
 }
 

@@ -126,6 +126,8 @@ struct bp_solver {
     // bodies over the whole walk) or per chunk (0)
     int staged_lambda = bp::kStageCapDefault, flat_position = 1, staged_post_a = bp::kStageCapDefault,
         flat_post_b = 1;
+    // partners the position pass fetches per round trip (2 or 4)
+    int pos_partners = 2;
 
     // hipGraph of two substeps (the ping-pong buffers return to their start)
     bool use_graph = false;

@@ -285,6 +285,13 @@ bp_status bp_solver_set_fusion(bp_solver *s, int enable);
 bp_status bp_solver_set_walk(bp_solver *s, int lambda_staged, int position_flat, int omega_normal_staged,
                              int vorticity_tension_flat);
 
+/* Partners the position pass fetches together per round trip to memory, 2
+ * (the default) or 4; anything else is BP_ERR_INVALID_ARG.  The pass mostly
+ * waits on these loads; more in flight per wave costs registers.  Same pairs
+ * in the same order, so results are bit-identical (bp_solver_autotune
+ * chooses).  No counterpart in the reference. */
+bp_status bp_solver_set_partner_fetch(bp_solver *s, int position_partners);
+
 /* Chooses the workgroup size of each neighbour pass among 64..512 and the
  * walk forms of the lambda, position, omega/normal and vorticity/tension
  * passes (bp_solver_set_walk, including the staged walks' region sizes) by
@@ -296,8 +303,9 @@ bp_status bp_solver_set_walk(bp_solver *s, int lambda_staged, int position_flat,
  * "lambda_walk"/"omega_normal_walk": "staged"|"global",
  * "position_walk"/"vorticity_tension_walk": "flat"|"chunk",
  * "lambda_stage_records"/"omega_normal_stage_records": 0|128|256|512 (the
- * staged region size, 0 for the per-lane walk)} into `report` (may be NULL;
- * 512 bytes hold it).  No counterpart in the reference. */
+ * staged region size, 0 for the per-lane walk), "position_partners": 2|4
+ * (bp_solver_set_partner_fetch)} into `report` (may be NULL; 512 bytes hold
+ * it).  No counterpart in the reference. */
 bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, char *report, int report_len);
 
 /* Sets the neighbour passes' workgroup sizes, each 64, 128, 256 or 512, in

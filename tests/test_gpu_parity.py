@@ -382,6 +382,11 @@ def test_walk_forms_are_bit_identical(label, records):
     a, b = bp.Solver(sc, prm), bp.Solver(sc, prm)
     a.set_walk(records, True, records, True)
     b.set_walk(False, False, False, False)
+    # the position pass fetching four partners per round trip, flat in a and
+    # per chunk in b (b's other walks keep two)
+    a.set_partner_fetch(4)
+    if records == 256:
+        b.set_partner_fetch(4)
     a.step(4, 4 * DT)
     b.step(4, 4 * DT)
     pa, va, ma = a.get_state()
@@ -393,7 +398,8 @@ def test_walk_forms_are_bit_identical(label, records):
 
 def test_set_walk_rejects_region_sizes_not_built():
     """bp_solver_set_walk takes 0, 1, 128, 256 or 512 for a staged pass and
-    returns BP_ERR_INVALID_ARG for anything else, leaving the solver usable."""
+    bp_solver_set_partner_fetch 2 or 4; both return BP_ERR_INVALID_ARG for
+    anything else, leaving the solver usable."""
     sc, prm = scenes.dam_break(dims=(8, 8, 8), grid=(64, 64, 64)), scenes.dam_break_params()
     case("set_walk argument check")
     s = bp.Solver(sc, prm)
@@ -402,7 +408,11 @@ def test_set_walk_rejects_region_sizes_not_built():
             s.set_walk(bad, True, True, True)
         with pytest.raises(bp.BrassPanError):
             s.set_walk(True, True, bad, True)
+    for bad in (0, 1, 3, 8, -2):
+        with pytest.raises(bp.BrassPanError):
+            s.set_partner_fetch(bad)
     s.set_walk(512, False, 128, False)
+    s.set_partner_fetch(4)
     s.step(1, DT)
     assert np.isfinite(s.get_state()[0]).all()
     s.close()
