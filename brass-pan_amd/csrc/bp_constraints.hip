@@ -62,7 +62,9 @@ __global__ __launch_bounds__(kBlock) void k_mark_cells(const int *__restrict__ s
 // particleParticleCollisionConstraint (particleparticle.cuh:5-101) + Δx apply.
 // A fluid particle can only touch solids (the pair needs a positive phase),
 // so it skips the walk when none of its 27 cells holds one (one cell_stamp
-// entry, k_mark_cells).
+// entry, k_mark_cells).  A partner's x* and x are fetched together, ahead of
+// the pair term.  (The staged walk, with both records in LDS, measured
+// slower here: 0.323 -> 0.337 ms per launch on the 512-scene batch.)
 template <class SP>
 __global__ __launch_bounds__(kMaxBlock) void k_particle_particle(const float4 *__restrict__ xs,
                                                                  const float4 *__restrict__ x,
@@ -93,42 +95,44 @@ __global__ __launch_bounds__(kMaxBlock) void k_particle_particle(const float4 *_
         }
         const float r2x4 = r * r * 4.0f;
         const float thr = r * r * 0.001f * 0.001f;
-        for_each_neighbor(
-            G, ranges, g, xs, 0,
-            [&](int, float4 b) {  // pi != pj also excludes i itself
-                const int pj = phase_of(b);
-                const float d2 = dist2(xip, xyz(b));
-                return (pi != pj) & ((pi > 0) | (pj > 0)) & (d2 < r2x4) & (d2 > 1e-5f);
-            },
-            [&](int j, float4 b) {
-                const f3 xjp = xyz(b);
-                const f3 diff = xip - xjp;
-                const float dist = fsqrt(dist2(xip, xjp));
-                const float idist = frcp(dist);
-                const float4 xb = x[j];
-                const float wj = xb.w;
-                const float iw = frcp(wi + wj);
-                const float w1 = wi * iw;
-                const float w2 = wj * iw;
-                const f3 proj = diff * (2.0f * r * idist - 1.0f);
-                const f3 dxi = w1 * proj;
-                const f3 xistar = dxi + xip;
-                sumD += dxi;
-                const f3 dxj = -w2 * proj;
-                const float l2 = length2(dxi);
-                if (l2 > thr) {
-                    cnt += 1;
-                    const f3 xjstar = dxj + xjp;
-                    const f3 term1 = (xistar - xi) - (xjstar - xyz(xb));
-                    const f3 nn = diff * idist;
-                    const f3 tan = term1 - dot(term1, nn) * nn;
-                    const float tl2 = length2(tan);
-                    if (tl2 <= (fs * fs) * l2)
-                        sumF -= w1 * tan;
-                    else
-                        sumF -= w1 * tan * fminf(fd * fsqrt(l2 * frcp(tl2)), 1.0f);
-                }
-            });
+        auto pick = [&](int, float4 b) {  // pi != pj also excludes i itself
+            const int pj = phase_of(b);
+            const float d2 = dist2(xip, xyz(b));
+            return (pi != pj) & ((pi > 0) | (pj > 0)) & (d2 < r2x4) & (d2 > 1e-5f);
+        };
+        // partner records: b = (x*, phase), xb = (x, w)
+        auto body = [&](int, const StagedRec<2> &R) {
+            const float4 b = R.r[0], xb = R.r[1];
+            const f3 xjp = xyz(b);
+            const f3 diff = xip - xjp;
+            const float dist = fsqrt(dist2(xip, xjp));
+            const float idist = frcp(dist);
+            const float wj = xb.w;
+            const float iw = frcp(wi + wj);
+            const float w1 = wi * iw;
+            const float w2 = wj * iw;
+            const f3 proj = diff * (2.0f * r * idist - 1.0f);
+            const f3 dxi = w1 * proj;
+            const f3 xistar = dxi + xip;
+            sumD += dxi;
+            const f3 dxj = -w2 * proj;
+            const float l2 = length2(dxi);
+            if (l2 > thr) {
+                cnt += 1;
+                const f3 xjstar = dxj + xjp;
+                const f3 term1 = (xistar - xi) - (xjstar - xyz(xb));
+                const f3 nn = diff * idist;
+                const f3 tan = term1 - dot(term1, nn) * nn;
+                const float tl2 = length2(tan);
+                if (tl2 <= (fs * fs) * l2)
+                    sumF -= w1 * tan;
+                else
+                    sumF -= w1 * tan * fminf(fd * fsqrt(l2 * frcp(tl2)), 1.0f);
+            }
+        };
+        for_each_neighbor_nb<1>(
+            G, ranges, g, xs, 0, pick, [&](int j) { return StagedRec<2>{{ld32(xs, j), ld32(x, j)}}; }, body,
+            AllowAll{});
         const f3 d = (cnt == 0) ? sumD : sumD + sumF / (float)cnt;
         xs_out[i] = f4(xip + d, a.w);
         // the lambda pass, which follows, reads x* also as LambdaIn
