@@ -84,13 +84,22 @@ __global__ __launch_bounds__(kSortWaves * 64) void k_radix_hist(const int *__res
     h[tid] = 0;
     __syncthreads();
     const int base = blockIdx.x * sort_tile(R);
-    // each wave reads its own contiguous quarter of the tile in R rounds
+    // each wave reads its own contiguous quarter of the tile in R rounds; the
+    // R keys are loaded first, all in flight together (a load per round, each
+    // waited for at the start of its round, made R serial round trips)
     const int wave = tid >> 6, lane = tid & 63;
     const int wbase = base + wave * 64 * R;
+    int kreg[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int idx = wbase + r * 64 + lane;
+        kreg[r] = idx < n ? keys[idx] : 0;
+    }
+#pragma unroll
     for (int r = 0; r < R; ++r) {
         const int idx = wbase + r * 64 + lane;
         const bool valid = idx < n;
-        const uint32_t d = valid ? ((uint32_t)keys[idx] >> shift) & (kRadix - 1) : 0u;
+        const uint32_t d = valid ? ((uint32_t)kreg[r] >> shift) & (kRadix - 1) : 0u;
         const uint64_t m = match_digit(d, valid);
         if (valid && (m & lanemask_lt()) == 0ull) atomicAdd(&h[d], (int)__popcll(m));
     }
@@ -133,6 +142,16 @@ __global__ __launch_bounds__(kSortWaves * 64) void k_radix_scatter(const int *__
     __shared__ int wcnt[kSortWaves][kRadix];
     __shared__ int dbase[kRadix];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wbase = blockIdx.x * sort_tile(R) + wave * 64 * R;
+    // all R keys and values in flight together, before the digit scan and the
+    // ranking rounds
+    int kreg[R], vreg[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int idx = wbase + r * 64 + lane;
+        kreg[r] = idx < n ? kin[idx] : 0;
+        vreg[r] = (idx < n && vin) ? vin[idx] : idx;
+    }
 #pragma unroll
     for (int w = 0; w < kSortWaves; ++w) wcnt[w][tid] = 0;
     // exclusive scan of digit totals
@@ -149,15 +168,13 @@ __global__ __launch_bounds__(kSortWaves * 64) void k_radix_scatter(const int *__
     __syncthreads();
     dbase[tid] = excl;
 
-    const int wbase = blockIdx.x * sort_tile(R) + wave * 64 * R;
-    int kreg[R];
     int rreg[R];
     const uint64_t lt = lanemask_lt();
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int idx = wbase + r * 64 + lane;
         const bool valid = idx < n;
-        const int k = valid ? kin[idx] : 0;
+        const int k = kreg[r];
         const uint32_t d = ((uint32_t)k >> shift) & (kRadix - 1);
         const uint64_t m = match_digit(d, valid);
         // the lowest lane of each digit group reserves popc(m) places with one
@@ -187,7 +204,7 @@ __global__ __launch_bounds__(kSortWaves * 64) void k_radix_scatter(const int *__
             const uint32_t d = ((uint32_t)kreg[r] >> shift) & (kRadix - 1);
             const int pos = dbase[d] + wcnt[wave][d] + rreg[r];
             kout[pos] = kreg[r];
-            vout[pos] = vin ? vin[idx] : idx;
+            vout[pos] = vreg[r];
         }
     }
 }
