@@ -44,7 +44,11 @@ __global__ __launch_bounds__(kBlock) void k_mark_cells(const int *__restrict__ s
     if (i >= n) return;
     const uint32_t st = *stamp;
     const int key = sorted_keys[i];
-    if (ph[i] <= 0) {
+    const bool fluid = ph[i] <= 0;
+    // the previous slot in the same cell and of the same kind (fluid or solid)
+    // writes the same stamps: only the first of such a run writes them
+    if (i > 0 && sorted_keys[i - 1] == key && (ph[i - 1] <= 0) == fluid) return;
+    if (fluid) {
         fluid_stamp[key] = st;
         return;
     }

@@ -739,6 +739,7 @@ constexpr int kStageCaps[3] = {128, 256, 512};
 constexpr int kStageCapDefault = 256;
 constexpr int kStageGroups = 4;  // lane groups (distinct current columns) per wave
 constexpr int kStagePad = 8;     // records after the region: a screen batch reads up to 7 past a run
+constexpr int kCopyBatch = 2;    // records a lane copies into the region per round of loads
 inline bool stage_cap_valid(int cap) { return cap == 128 || cap == 256 || cap == 512; }
 // nr staged arrays per wave of `cap` records each
 inline size_t staged_lds_bytes(int block, int nr, int cap) {
@@ -861,13 +862,30 @@ __device__ __forceinline__ void for_each_neighbor_staged(const GridDesc &G, cons
             }
             if (tot <= Cap) {
                 staged = true;
-                for (int t = rank; t < tot; t += nact) {
-                    int b = sbase[0];
+                // kCopyBatch records per lane in flight before their LDS stores
+                // (one at a time, each store waited for its load)
+                for (int t0 = rank; t0 < tot; t0 += kCopyBatch * nact) {
+                    f4v buf[kCopyBatch][NR];
 #pragma unroll
-                    for (int q = 1; q < kStageGroups; ++q)
-                        if (q < ng && t >= seg0[q]) b = sbase[q];
+                    for (int u = 0; u < kCopyBatch; ++u) {
+                        const int t = t0 + u * nact;
+                        int b = sbase[0];
 #pragma unroll
-                    for (int a = 0; a < NR; ++a) lds[a * kRegion + t] = as_f4v(ld32(srcs[a], t - b));
+                        for (int q = 1; q < kStageGroups; ++q)
+                            if (q < ng && t >= seg0[q]) b = sbase[q];
+                        if (t < tot) {
+#pragma unroll
+                            for (int a = 0; a < NR; ++a) buf[u][a] = as_f4v(ld32(srcs[a], t - b));
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < kCopyBatch; ++u) {
+                        const int t = t0 + u * nact;
+                        if (t < tot) {
+#pragma unroll
+                            for (int a = 0; a < NR; ++a) lds[a * kRegion + t] = buf[u][a];
+                        }
+                    }
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
