@@ -168,7 +168,7 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, Flat ? 5 : 6) void k_posi
         rec.in = kf ? srec : nullptr;
         rec.slot = k;
         rec.n = n;
-        walk_form<Flat, NB>(
+        walk_form<Flat, NB, kReplayScreen>(
             G, ranges, g, xs, 0,
             // (fluid particles replay the lambda pass's masks; a solid screens for
             // fluid partners only)
@@ -353,7 +353,7 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 1) void k_post_b(const fl
         const float4 *__restrict__ second = do_akinci ? normal : xom;
         // four partners per round trip: the pass mostly waits on these loads
         // (batch 0.79 -> 0.72 ms, 4M block 0.95 -> 0.90 ms against two)
-        walk_form<Flat, 4>(
+        walk_form<Flat, 4, kReplayScreen>(
             G, ranges, g, xs, 0,
             [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
             [&](int j) { return Partner{ld32(xom, j), ld32(second, j)}; },
@@ -418,7 +418,7 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_xsph(const floa
         struct Partner {
             float4 b, v;
         };
-        for_each_neighbor_nb<kBodyBatch>(
+        for_each_neighbor_nb<kBodyBatch, kReplayScreen>(
             G, ranges, g, xs, 0,
             [&](int, float4 b) { return (phase_of(b) == ph) & (dist2(pi, xyz(b)) <= K.h2); },
             [&](int j) { return Partner{ld32(xs, j), ld32(v, j)}; },

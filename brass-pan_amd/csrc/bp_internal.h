@@ -439,7 +439,42 @@ __device__ __forceinline__ void store_lambda_in(const LambdaIn &L, int i, f3 p, 
     }
     if (L.xm) L.xm[i] = make_float4(p.x, p.y, p.z, ph < 0 ? m : -m);
 }
-template <class Pick, class Allow, class Consume>
+// Screens the SB candidates at slots base + t0 .. base + t0 + SB - 1 with
+// `pick(j, b)` and returns their bits (bit u for slot base + t0 + u).  The
+// candidates are read as x* records from xs or, where rec has them, as 3 x
+// SB/4 loads from the position planes (pick then sees w = 0).  A batch may
+// read past the end of its run (kWalkPad).  SB is 8, or 4 in the passes that
+// screen only the rare chunks past their screen record, where the batch's
+// registers would otherwise set the pass's register budget.
+template <int SB, class Pick>
+__device__ __forceinline__ uint32_t screen_batch(const float4 *__restrict__ xs, const ScreenRec &rec, int base,
+                                                 int t0, Pick &&pick) {
+    static_assert(SB == 4 || SB == 8, "screen batches of 4 or 8 candidates");
+    constexpr int QH = SB / 4;  // float4_a4 loads per coordinate plane
+    float4 bb[SB];
+    if (rec.planes) {
+        const float *__restrict__ px = rec.planes + (base + t0);
+        float4_a4 q[3 * QH];  // x, y, z of candidates 0-3 (and 4-7)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int h = 0; h < QH; ++h)
+                q[QH * a + h] = *reinterpret_cast<const float4_a4 *>(px + (size_t)a * rec.pstride + 4 * h);
+#pragma unroll
+        for (int u = 0; u < SB; ++u)
+            bb[u] = make_float4(q[u >> 2][u & 3], q[QH + (u >> 2)][u & 3], q[2 * QH + (u >> 2)][u & 3], 0.0f);
+    } else {
+        const float4 *__restrict__ p = xs + (base + t0);
+#pragma unroll
+        for (int u = 0; u < SB; ++u) bb[u] = p[u];
+    }
+    uint32_t m = 0u;  // constant shifts within the batch
+#pragma unroll
+    for (int u = 0; u < SB; ++u) m |= (uint32_t)pick(base + t0 + u, bb[u]) << u;
+    return m;
+}
+// SB: candidates per screen batch (screen_batch).
+template <int SB = 8, class Pick, class Allow, class Consume>
 __device__ __forceinline__ void walk_chunks(const GridDesc &G, const int2 *__restrict__ ranges, const int g[3],
                                             const float4 *__restrict__ xs, int jmin, Pick &&pick, Allow &&allow,
                                             ScreenRec rec, Consume &&consume) {
@@ -459,29 +494,7 @@ __device__ __forceinline__ void walk_chunks(const GridDesc &G, const int2 *__res
                 if (keep == 0u) continue;
             } else {
                 if (keep == 0u && !rec.out) continue;
-                for (int t0 = 0; t0 < cnt; t0 += 8) {
-                    float4 bb[8];
-                    if (rec.planes) {
-                        const float *__restrict__ px = rec.planes + (base + t0);
-                        float4_a4 q[6];  // x, y, z of candidates 0-3 and 4-7
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            q[2 * a] = *reinterpret_cast<const float4_a4 *>(px + (size_t)a * rec.pstride);
-                            q[2 * a + 1] = *reinterpret_cast<const float4_a4 *>(px + (size_t)a * rec.pstride + 4);
-                        }
-#pragma unroll
-                        for (int u = 0; u < 8; ++u)
-                            bb[u] = make_float4(q[u >> 2][u & 3], q[2 + (u >> 2)][u & 3], q[4 + (u >> 2)][u & 3], 0.0f);
-                    } else {
-                        const float4 *__restrict__ p = xs + (base + t0);
-#pragma unroll
-                        for (int u = 0; u < 8; ++u) bb[u] = p[u];
-                    }
-                    uint32_t m8 = 0u;  // constant shifts within the batch, one variable shift per batch
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) m8 |= (uint32_t)pick(base + t0 + u, bb[u]) << u;
-                    mask |= m8 << t0;
-                }
+                for (int t0 = 0; t0 < cnt; t0 += SB) mask |= screen_batch<SB>(xs, rec, base, t0, pick) << t0;
                 mask &= valid;
                 if (rec.out && c < kRecChunks) rec.out[(size_t)c * rec.n + rec.slot] = mask;
             }
@@ -529,12 +542,18 @@ __device__ __forceinline__ void run_bodies(Next &&next, Fetch &&fetch, Use &&use
 // sum-over-chunks(max over lanes of the chunk's partners) times, in ascending
 // j per particle.
 constexpr int kBodyBatch = 2;
-template <int NB, class Pick, class Fetch, class Use, class Allow>
+// Screen batch of the passes that replay a screen record and screen only the
+// chunks past it (position, vorticity/tension, XSPH): 4 candidates, so the
+// rare screen does not set their register budget.  Their VGPRs fall by 14-16
+// and one to two more waves per SIMD fit (batch position 1.27 -> 1.16 ms,
+// vorticity/tension 0.71 -> 0.59 ms per launch).
+constexpr int kReplayScreen = 4;
+template <int NB, int SB = 8, class Pick, class Fetch, class Use, class Allow>
 __device__ __forceinline__ void for_each_neighbor_nb(const GridDesc &G, const int2 *__restrict__ ranges,
                                                      const int g[3], const float4 *__restrict__ xs, int jmin,
                                                      Pick &&pick, Fetch &&fetch, Use &&use, Allow &&allow,
                                                      ScreenRec rec = ScreenRec{}) {
-    walk_chunks(G, ranges, g, xs, jmin, pick, allow, rec, [&](int base, uint32_t mask) {
+    walk_chunks<SB>(G, ranges, g, xs, jmin, pick, allow, rec, [&](int base, uint32_t mask) {
         while (mask) {
             run_bodies<NB>(
                 [&](int, bool &ok) {
@@ -580,7 +599,7 @@ typedef unsigned int u2v __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) u2v lds_u2;
 extern __shared__ u2v bp_flat_lds[];
 
-template <int NB, class Pick, class Fetch, class Use, class Allow>
+template <int NB, int SB = 8, class Pick, class Fetch, class Use, class Allow>
 __device__ __forceinline__ void for_each_neighbor_flat(const GridDesc &G, const int2 *__restrict__ ranges,
                                                        const int g[3], const float4 *__restrict__ xs, int jmin,
                                                        Pick &&pick, Fetch &&fetch, Use &&use, Allow &&allow,
@@ -676,30 +695,8 @@ __device__ __forceinline__ void for_each_neighbor_flat(const GridDesc &G, const 
                     mask = ahead;
                     if (c + 1 < kRecChunks) ahead = rec.in[(size_t)(c + 1) * rec.n + rec.slot];
                 } else if (keep != 0u || rec.out) {
-                    for (int t0 = 0; t0 < cn; t0 += 8) {
-                        float4 bb[8];
-                        if (rec.planes) {
-                            const float *__restrict__ px = rec.planes + (base + t0);
-                            float4_a4 q[6];
-#pragma unroll
-                            for (int a = 0; a < 3; ++a) {
-                                q[2 * a] = *reinterpret_cast<const float4_a4 *>(px + (size_t)a * rec.pstride);
-                                q[2 * a + 1] = *reinterpret_cast<const float4_a4 *>(px + (size_t)a * rec.pstride + 4);
-                            }
-#pragma unroll
-                            for (int u = 0; u < 8; ++u)
-                                bb[u] = make_float4(q[u >> 2][u & 3], q[2 + (u >> 2)][u & 3], q[4 + (u >> 2)][u & 3],
-                                                    0.0f);
-                        } else {
-                            const float4 *__restrict__ p = xs + (base + t0);
-#pragma unroll
-                            for (int u = 0; u < 8; ++u) bb[u] = p[u];
-                        }
-                        uint32_t m8 = 0u;
-#pragma unroll
-                        for (int u = 0; u < 8; ++u) m8 |= (uint32_t)pick(base + t0 + u, bb[u]) << u;
-                        mask |= m8 << t0;
-                    }
+                    for (int t0 = 0; t0 < cn; t0 += SB)
+                        mask |= screen_batch<SB>(xs, rec, base, t0, pick) << t0;
                     mask &= valid;
                     if (rec.out && c < kRecChunks) rec.out[(size_t)c * rec.n + rec.slot] = mask;
                 }
@@ -915,32 +912,18 @@ __device__ __forceinline__ void for_each_neighbor_staged(const GridDesc &G, cons
                 } else {
                     if (keep == 0u && !rec.out) continue;
                     for (int t0 = 0; t0 < cnt; t0 += 8) {
-                        float4 bb[8];
                         if constexpr (kLds) {
+                            float4 bb[8];
                             const lds_f4 *q = lds + (base + t0 + ldsbase);
 #pragma unroll
                             for (int u = 0; u < 8; ++u) bb[u] = as_float4(q[u]);
-                        } else if (!rec.planes) {
-                            const float4 *__restrict__ p = srcs[0] + (base + t0);
+                            uint32_t m8 = 0u;
 #pragma unroll
-                            for (int u = 0; u < 8; ++u) bb[u] = p[u];
+                            for (int u = 0; u < 8; ++u) m8 |= (uint32_t)pick(base + t0 + u, bb[u]) << u;
+                            mask |= m8 << t0;
                         } else {
-                            const float *__restrict__ px = rec.planes + (base + t0);
-                            float4_a4 q[6];
-#pragma unroll
-                            for (int a = 0; a < 3; ++a) {
-                                q[2 * a] = *reinterpret_cast<const float4_a4 *>(px + (size_t)a * rec.pstride);
-                                q[2 * a + 1] = *reinterpret_cast<const float4_a4 *>(px + (size_t)a * rec.pstride + 4);
-                            }
-#pragma unroll
-                            for (int u = 0; u < 8; ++u)
-                                bb[u] = make_float4(q[u >> 2][u & 3], q[2 + (u >> 2)][u & 3], q[4 + (u >> 2)][u & 3],
-                                                    0.0f);
+                            mask |= screen_batch<8>(srcs[0], rec, base, t0, pick) << t0;
                         }
-                        uint32_t m8 = 0u;
-#pragma unroll
-                        for (int u = 0; u < 8; ++u) m8 |= (uint32_t)pick(base + t0 + u, bb[u]) << u;
-                        mask |= m8 << t0;
                     }
                     mask &= valid;
                     if (rec.out && c < kRecChunks) rec.out[(size_t)c * rec.n + rec.slot] = mask;
@@ -1002,11 +985,11 @@ __device__ __forceinline__ void for_each_neighbor_staged(const GridDesc &G, cons
 }
 
 // The walk in its per-chunk (Flat = false) or flat form; same pairs, same order.
-template <bool Flat, int NB, class... A> __device__ __forceinline__ void walk_form(A &&...a) {
+template <bool Flat, int NB, int SB = 8, class... A> __device__ __forceinline__ void walk_form(A &&...a) {
     if constexpr (Flat)
-        for_each_neighbor_flat<NB>(static_cast<A &&>(a)...);
+        for_each_neighbor_flat<NB, SB>(static_cast<A &&>(a)...);
     else
-        for_each_neighbor_nb<NB>(static_cast<A &&>(a)...);
+        for_each_neighbor_nb<NB, SB>(static_cast<A &&>(a)...);
 }
 
 // Walks the CSR entries [e0, e1) in order, eight at a time: the entry
