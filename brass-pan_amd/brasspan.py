@@ -97,6 +97,7 @@ SIGNATURES = {
     "bp_solver_set_fusion": (_S, [_P, C.c_int]),
     "bp_solver_set_walk": (_S, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
     "bp_solver_set_partner_fetch": (_S, [_P, C.c_int]),
+    "bp_solver_set_stage_screen": (_S, [_P, C.c_int, C.c_int]),
     "bp_solver_set_block_sizes": (_S, [_P, C.POINTER(C.c_int)]),
     "bp_stream_copy_bandwidth": (_S, [C.c_size_t, C.c_int, C.POINTER(C.c_float)]),
     "bp_forward_checkpointed": (_S, [_P, C.c_int, C.c_float, C.c_int]),
@@ -273,10 +274,15 @@ class Solver:
         (bp_solver_set_partner_fetch; results are bit-identical)."""
         _check(self.lib.bp_solver_set_partner_fetch(self.h, int(position_partners)))
 
+    def set_stage_screen(self, lambda_candidates: int = 4, omega_normal_candidates: int = 4):
+        """Candidates the staged lambda and omega/normal walks screen per batch,
+        4 or 8 (bp_solver_set_stage_screen; results are bit-identical)."""
+        _check(self.lib.bp_solver_set_stage_screen(self.h, int(lambda_candidates), int(omega_normal_candidates)))
+
     def autotune(self, dt: float, substeps_per_trial: int = 2) -> dict:
         import json
-        buf = C.create_string_buffer(512)
-        _check(self.lib.bp_solver_autotune(self.h, float(dt), int(substeps_per_trial), buf, 512))
+        buf = C.create_string_buffer(1024)
+        _check(self.lib.bp_solver_autotune(self.h, float(dt), int(substeps_per_trial), buf, 1024))
         return json.loads(buf.value.decode())
 
     TUNED_PASSES = ("fluid_lambda", "fluid_position", "omega_normal", "vorticity_tension", "xsph",
@@ -296,6 +302,7 @@ class Solver:
                       region("omega_normal_walk", "omega_normal_stage_records"),
                       report["vorticity_tension_walk"] == "flat")
         self.set_partner_fetch(int(report.get("position_partners", 2)))
+        self.set_stage_screen(int(report.get("lambda_stage_screen", 8)), int(report.get("omega_normal_stage_screen", 8)))
 
     # ---------------------------------------------------------- adjoint
     def forward_checkpointed(self, substeps: int, subdt: float, every: int = 8):

@@ -25,8 +25,9 @@ __global__ __launch_bounds__(kBlock) void k_lambda_in(const float4 *__restrict__
 // packed, 0), so that pass fetches two words-of-four per partner (x* and this)
 // instead of four separate arrays; it searches from the same x* and cell.
 // Cap: 0 = every lane reads its candidates from global memory; else the
-// staged walk with a region of Cap records per wave and column offset.
-template <class SP, int Cap>
+// staged walk with a region of Cap records per wave and column offset,
+// screening SB candidates per batch (4 or 8: fewer registers, more waves).
+template <class SP, int Cap, int SB>
 __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, Cap ? 5 : 6) void k_lambda(const float4 *__restrict__ xs, const float *__restrict__ m,
                                                    const int2 *__restrict__ ranges, float *__restrict__ lambda,
                                                    float *__restrict__ density, float4 *__restrict__ lam_rec,
@@ -80,7 +81,7 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, Cap ? 5 : 6) void k_lambd
         // budget; the partner record is (x*, m), m negated for a non-fluid partner
         if constexpr (Cap != 0) {
             const float4 *const srcs[1] = {L.xm};
-            for_each_neighbor_staged<4, 1, Cap>(
+            for_each_neighbor_staged<4, 1, Cap, SB>(
                 G, ranges, g, srcs, srec ? 0 : i, pick, [&](int j, const StagedRec<1> &R) { body(j, R.r[0]); },
                 allow, rec);
         } else {
@@ -247,8 +248,8 @@ __global__ __launch_bounds__(kBlock) void k_position_apply(const float4 *__restr
 // registers of the terms it computes only.  Cap != 0: candidates and partner
 // records from the wave's LDS region of Cap records per column offset
 // (for_each_neighbor_staged, 4 waves per SIMD: it needs 111-121 VGPRs);
-// Cap = 0: per lane from global memory.
-template <class SP, int Mode, int Cap>
+// Cap = 0: per lane from global memory.  SB: the staged screen batch (4 or 8).
+template <class SP, int Mode, int Cap, int SB>
 __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, Cap ? 4 : 6) void k_post_a(const float4 *__restrict__ xs, const float4 *__restrict__ v,
                                                    const float *__restrict__ density,
                                                    const int2 *__restrict__ ranges, float4 *__restrict__ omega,
@@ -294,7 +295,7 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, Cap ? 4 : 6) void k_post_
         };
         if constexpr (Cap != 0) {
             const float4 *const srcs[2] = {xs, v};
-            for_each_neighbor_staged<kBodyBatch, 2, Cap>(G, ranges, g, srcs, 0, pick, body, AllowNotSelf{i}, rec);
+            for_each_neighbor_staged<kBodyBatch, 2, Cap, SB>(G, ranges, g, srcs, 0, pick, body, AllowNotSelf{i}, rec);
         } else {
             for_each_neighbor_nb<kBodyBatch>(
                 G, ranges, g, xs, 0, pick, [&](int j) { return StagedRec<2>{{ld32(xs, j), ld32(v, j)}}; }, body,
@@ -432,17 +433,21 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 6) void k_xsph(const floa
 template <class SP>
 void launch_lambda(const float4 *xs, const float *m, const int2 *ranges, float *lambda, float *density,
                    float4 *lam_rec, uint32_t *srec, const LambdaIn &L, int lin_ready, int n, const SP &sp,
-                   int block, int staged, hipStream_t st) {
+                   int block, int staged, int stage_sb, hipStream_t st) {
     if (!lin_ready) hipLaunchKernelGGL(k_lambda_in, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, m, L, n);
     const dim3 grid(grid_blocks(n, block)), blk(block);
     const size_t lds = staged ? staged_lds_bytes(block, 1, staged) : 0;
-#define BP_LAMBDA(C) \
-    hipLaunchKernelGGL((k_lambda<SP, C>), grid, blk, lds, st, xs, m, ranges, lambda, density, lam_rec, srec, L, n, sp)
-    switch (staged) {
-    case 128: BP_LAMBDA(128); break;
-    case 256: BP_LAMBDA(256); break;
-    case 512: BP_LAMBDA(512); break;
-    default: BP_LAMBDA(0); break;
+#define BP_LAMBDA(C, B)                                                                                          \
+    hipLaunchKernelGGL((k_lambda<SP, C, B>), grid, blk, lds, st, xs, m, ranges, lambda, density, lam_rec, srec, L, \
+                       n, sp)
+    switch (staged | (stage_sb == 4 ? 1 : 0)) {
+    case 128: BP_LAMBDA(128, 8); break;
+    case 256: BP_LAMBDA(256, 8); break;
+    case 512: BP_LAMBDA(512, 8); break;
+    case 129: BP_LAMBDA(128, 4); break;
+    case 257: BP_LAMBDA(256, 4); break;
+    case 513: BP_LAMBDA(512, 4); break;
+    default: BP_LAMBDA(0, 8); break;
     }
 #undef BP_LAMBDA
 }
@@ -470,22 +475,25 @@ template <class SP>
 void launch_post_a(const float4 *xs, const float4 *v, const float *density, const int2 *ranges, float4 *omega,
                    float4 *xom, float4 *normal, uint32_t *ccell, uint32_t *srec, const LambdaIn *one_phase,
                    int planes_ready, int n, const SP &sp, int do_omega, int do_normal, int block, int staged,
-                   hipStream_t st) {
+                   int stage_sb, hipStream_t st) {
     if (one_phase && !planes_ready)
         hipLaunchKernelGGL(k_lambda_in, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, nullptr, *one_phase, n);
     const float *planes = one_phase ? one_phase->planes : nullptr;
     const int pstride = one_phase ? one_phase->pstride : 0;
     const dim3 grid(grid_blocks(n, block)), blk(block);
     const size_t lds = staged ? staged_lds_bytes(block, 2, staged) : 0;
-#define BP_POST_A(M, C)                                                                                           \
-    hipLaunchKernelGGL((k_post_a<SP, M, C>), grid, blk, lds, st, xs, v, density, ranges, omega, xom, normal, ccell, \
-                       srec, planes, pstride, n, sp)
-#define BP_POST_A_CAP(M)                  \
-    switch (staged) {                     \
-    case 128: BP_POST_A(M, 128); break;   \
-    case 256: BP_POST_A(M, 256); break;   \
-    case 512: BP_POST_A(M, 512); break;   \
-    default: BP_POST_A(M, 0); break;      \
+#define BP_POST_A(M, C, B)                                                                                       \
+    hipLaunchKernelGGL((k_post_a<SP, M, C, B>), grid, blk, lds, st, xs, v, density, ranges, omega, xom, normal,    \
+                       ccell, srec, planes, pstride, n, sp)
+#define BP_POST_A_CAP(M)                                \
+    switch (staged | (stage_sb == 4 ? 1 : 0)) {         \
+    case 128: BP_POST_A(M, 128, 8); break;              \
+    case 256: BP_POST_A(M, 256, 8); break;              \
+    case 512: BP_POST_A(M, 512, 8); break;              \
+    case 129: BP_POST_A(M, 128, 4); break;              \
+    case 257: BP_POST_A(M, 256, 4); break;              \
+    case 513: BP_POST_A(M, 512, 4); break;              \
+    default: BP_POST_A(M, 0, 8); break;                 \
     }
     switch ((do_omega ? 1 : 0) | (do_normal ? 2 : 0)) {
     case 1: BP_POST_A_CAP(1); break;
@@ -525,14 +533,15 @@ void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const ui
 
 #define BP_INSTANTIATE_FLUID(SP)                                                                                \
     template void launch_lambda<SP>(const float4 *, const float *, const int2 *, float *, float *, float4 *,    \
-                                    uint32_t *, const LambdaIn &, int, int, const SP &, int, int, hipStream_t); \
+                                    uint32_t *, const LambdaIn &, int, int, const SP &, int, int, int,          \
+                                    hipStream_t);                                                               \
     template void launch_position<SP>(const float4 *, const float4 *, const int *, const int2 *,                \
                                       const uint32_t *, const uint32_t *, const uint32_t *, float4 *,           \
                                       unsigned long long *, float4 *, float4 *, float4 *, int, const SP &,      \
                                       const Epilogue &, int, int, int, hipStream_t);                            \
     template void launch_post_a<SP>(const float4 *, const float4 *, const float *, const int2 *, float4 *,      \
                                     float4 *, float4 *, uint32_t *, uint32_t *, const LambdaIn *, int, int,     \
-                                    const SP &, int, int, int, int, hipStream_t);                               \
+                                    const SP &, int, int, int, int, int, hipStream_t);                          \
     template void launch_post_b<SP>(const float4 *, float4 *, const float4 *, const float4 *, const float4 *,   \
                                     const float *, const int2 *, const uint32_t *, const uint32_t *, int,       \
                                     const SP &, int, int, float, int, int, hipStream_t);                        \

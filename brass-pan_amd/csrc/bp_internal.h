@@ -751,7 +751,7 @@ typedef __attribute__((address_space(3))) f4v lds_f4;
 __device__ __forceinline__ float4 as_float4(f4v v) { return make_float4(v.x, v.y, v.z, v.w); }
 __device__ __forceinline__ f4v as_f4v(float4 a) { return f4v{a.x, a.y, a.z, a.w}; }
 
-template <int NB, int NR, int Cap, class Pick, class Use, class Allow>
+template <int NB, int NR, int Cap, int SB = 8, class Pick, class Use, class Allow>
 __device__ __forceinline__ void for_each_neighbor_staged(const GridDesc &G, const int2 *__restrict__ ranges,
                                                          const int g[3], const float4 *const (&srcs)[NR], int jmin,
                                                          Pick &&pick, Use &&use, Allow &&allow, ScreenRec rec) {
@@ -911,18 +911,18 @@ __device__ __forceinline__ void for_each_neighbor_staged(const GridDesc &G, cons
                     if (keep == 0u) continue;
                 } else {
                     if (keep == 0u && !rec.out) continue;
-                    for (int t0 = 0; t0 < cnt; t0 += 8) {
+                    for (int t0 = 0; t0 < cnt; t0 += SB) {
                         if constexpr (kLds) {
-                            float4 bb[8];
+                            float4 bb[SB];
                             const lds_f4 *q = lds + (base + t0 + ldsbase);
 #pragma unroll
-                            for (int u = 0; u < 8; ++u) bb[u] = as_float4(q[u]);
+                            for (int u = 0; u < SB; ++u) bb[u] = as_float4(q[u]);
                             uint32_t m8 = 0u;
 #pragma unroll
-                            for (int u = 0; u < 8; ++u) m8 |= (uint32_t)pick(base + t0 + u, bb[u]) << u;
+                            for (int u = 0; u < SB; ++u) m8 |= (uint32_t)pick(base + t0 + u, bb[u]) << u;
                             mask |= m8 << t0;
                         } else {
-                            mask |= screen_batch<8>(srcs[0], rec, base, t0, pick) << t0;
+                            mask |= screen_batch<SB>(srcs[0], rec, base, t0, pick) << t0;
                         }
                     }
                     mask &= valid;

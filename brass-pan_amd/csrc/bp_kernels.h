@@ -30,11 +30,12 @@ void launch_reorder(const int *perm, const float4 *x_in, const float4 *v_in, con
 // omega/normal passes and replayed by the position pass of the same iteration
 // and by the vorticity/tension and XSPH passes.  `staged` (lambda and
 // omega/normal): 0 = per-lane walk, else the staged walk's region size in
-// records per wave and column offset (one of kStageCaps).
+// records per wave and column offset (one of kStageCaps); `stage_sb`: the
+// staged walk's screen batch, 4 or 8 candidates.
 template <class SP>
 void launch_lambda(const float4 *xs, const float *m, const int2 *ranges, float *lambda, float *density,
                    float4 *lam_rec, uint32_t *srec, const LambdaIn &L, int lin_ready, int n, const SP &sp,
-                   int block, int staged, hipStream_t st);
+                   int block, int staged, int stage_sb, hipStream_t st);
 template <class SP>
 void launch_position(const float4 *xs, const float4 *lam_rec, const int *build_keys, const int2 *ranges,
                      const uint32_t *srec, const uint32_t *fluid_stamp, const uint32_t *stamp, float4 *sum_tmp,
@@ -43,7 +44,8 @@ void launch_position(const float4 *xs, const float4 *lam_rec, const int *build_k
 template <class SP>
 void launch_post_a(const float4 *xs, const float4 *v, const float *density, const int2 *ranges, float4 *omega,
                    float4 *xom, float4 *normal, uint32_t *ccell, uint32_t *srec, const LambdaIn *one_phase,
-                   int planes_ready, int n, const SP &sp, int do_omega, int do_normal, int block, int staged, hipStream_t st);
+                   int planes_ready, int n, const SP &sp, int do_omega, int do_normal, int block, int staged,
+                   int stage_sb, hipStream_t st);
 template <class SP>
 void launch_post_b(const float4 *xs, float4 *v, const float4 *omega, const float4 *xom, const float4 *normal,
                    const float *density, const int2 *ranges, const uint32_t *ccell, const uint32_t *srec, int n,

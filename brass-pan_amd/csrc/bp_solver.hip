@@ -226,7 +226,7 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
         if (s->has_fluid) {
             TIMED(F_LAMBDA, launch_lambda(s->xs.cur(), s->m.cur(), s->ranges, s->lambda, s->density, s->lam_rec,
                                           H.rec ? H.rec[it] : s->srec, s->lin, planes_ready ? 1 : 0, N, sp(),
-                                          s->blk_lambda, s->staged_lambda, st));
+                                          s->blk_lambda, s->staged_lambda, s->stage_sb_lambda, st));
             if (H.lam) HIP_TRY(snap(H.lam[it], s->lambda, N, st));
             Epilogue E{};
             E.planes_next = (!no_fuse && s->any_planes && it + 1 < iters && !tail_groups && !pick) ? 1 : 0;
@@ -277,7 +277,7 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
             TIMED(F_POST_A, launch_post_a(s->xs.cur(), s->v.cur(), s->density, s->ranges, s->omega, s->xom, s->normal,
                                           s->ccell, s->srec, s->one_phase ? &s->lin : nullptr,
                                           finished && planes_ready ? 1 : 0, N, sp(), do_vort, do_ak, s->blk_post_a,
-                                          s->staged_post_a, st));
+                                          s->staged_post_a, s->stage_sb_post_a, st));
             TIMED(F_POST_B, launch_post_b(s->xs.cur(), s->v.cur(), s->omega, s->xom, s->normal, s->density, s->ranges,
                                           s->ccell, s->srec, N, sp(), do_vort, do_ak, subdt, s->blk_post_b,
                                           s->flat_post_b, st));
@@ -817,6 +817,17 @@ bp_status bp_solver_set_walk(bp_solver *s, int lambda_staged, int position_flat,
     return BP_OK;
 }
 
+bp_status bp_solver_set_stage_screen(bp_solver *s, int lambda_candidates, int omega_normal_candidates) {
+    if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
+    auto ok = [](int c) { return c == 4 || c == 8; };
+    if (!ok(lambda_candidates) || !ok(omega_normal_candidates))
+        return fail(BP_ERR_INVALID_ARG, "the staged walks screen 4 or 8 candidates per batch");
+    s->stage_sb_lambda = lambda_candidates;
+    s->stage_sb_post_a = omega_normal_candidates;
+    s->drop_graph();
+    return BP_OK;
+}
+
 bp_status bp_solver_set_partner_fetch(bp_solver *s, int position_partners) {
     if (!s) return fail(BP_ERR_INVALID_ARG, "solver is NULL");
     if (position_partners != 2 && position_partners != 4)
@@ -876,22 +887,28 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
     if (st != BP_OK) return st;
     // every workgroup size with each walk form of the lambda, position,
     // omega/normal and vorticity/tension passes (the others have one form).
-    // Form 0-3: the lambda and omega/normal passes per lane (0) or staged
-    // with each region size (kStageCaps); the position pass per chunk (0, 1)
-    // or flat (2, 3), fetching 2 (0, 2) or 4 (1, 3) partners per round trip;
-    // the vorticity/tension pass per chunk (0, 1) or flat (2, 3).  Two rounds
-    // over the candidates; a pass keeps the candidate with the lowest mean
-    // launch time summed over both rounds.
-    constexpr int kTrials = 16, kRounds = 2;
+    // Form 0-6: the lambda and omega/normal passes per lane (0) or staged
+    // with each region size (kStageCaps) screening 4 (1-3) or 8 (4-6)
+    // candidates per batch; the position pass per chunk (0, 1, 4) or flat
+    // (2, 3, 5, 6), fetching 2 (0, 2, 4, 6) or 4 (1, 3, 5) partners per round
+    // trip; the vorticity/tension pass per chunk or flat as the position pass.
+    // Two rounds over the candidates; a pass keeps the candidate with the
+    // lowest mean launch time summed over both rounds.
+    constexpr int kForms = 7, kTrials = 4 * kForms, kRounds = 2;
+    auto form_cap = [&](int form) { return form ? kStageCaps[(form - 1) % 3] : 0; };
+    auto form_sb = [](int form) { return form >= 4 ? 8 : 4; };
+    auto form_flat = [](int form) { return (form == 2 || form == 3 || form == 5 || form == 6) ? 1 : 0; };
+    auto form_partners = [](int form) { return (form == 1 || form == 3 || form == 5) ? 4 : 2; };
     float sum_ms[kTrials][6] = {};
     bool seen[kTrials][6] = {};
     for (int round = 0; round < kRounds && st == BP_OK; ++round)
         for (int t = 0; t < kTrials; ++t) {
-            const int c = t >> 2, form = t & 3;
+            const int c = t / kForms, form = t % kForms;
             s->blk_lambda = s->blk_position = s->blk_post_a = s->blk_post_b = s->blk_xsph = s->blk_pp = cand[c];
-            s->staged_lambda = s->staged_post_a = form ? kStageCaps[form - 1] : 0;
-            s->flat_position = s->flat_post_b = form >= 2 ? 1 : 0;
-            s->pos_partners = (form & 1) ? 4 : 2;
+            s->staged_lambda = s->staged_post_a = form_cap(form);
+            s->stage_sb_lambda = s->stage_sb_post_a = form_sb(form);
+            s->flat_position = s->flat_post_b = form_flat(form);
+            s->pos_partners = form_partners(form);
             st = restore();
             if (st != BP_OK) break;
             for (int f = 0; f < F_COUNT; ++f) { s->fam_ms[f] = 0.f; s->fam_n[f] = 0; }
@@ -905,16 +922,17 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
                 seen[t][f] = true;
             }
         }
-    int best_partners = 2;
+    int best_partners = 2, best_sb[2] = {4, 4};
     for (int t = 0; t < kTrials && st == BP_OK; ++t) {
-        const int c = t >> 2, form = t & 3;
-        const int cap = form ? kStageCaps[form - 1] : 0, flat = form >= 2 ? 1 : 0;
+        const int c = t / kForms, form = t % kForms;
         for (int f = 0; f < 6; ++f)
             if (seen[t][f] && sum_ms[t][f] < best_ms[f]) {
                 best_ms[f] = sum_ms[t][f];
                 best_blk[f] = cand[c];
-                best_flat[f] = (f == 0 || f == 2) ? cap : flat;  // region size or flat flag
-                if (f == 1) best_partners = (form & 1) ? 4 : 2;
+                best_flat[f] = (f == 0 || f == 2) ? form_cap(form) : form_flat(form);  // region size or flat flag
+                if (f == 0) best_sb[0] = form_sb(form);
+                if (f == 2) best_sb[1] = form_sb(form);
+                if (f == 1) best_partners = form_partners(form);
             }
     }
     s->blk_lambda = best_blk[0];
@@ -928,6 +946,8 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
     s->staged_post_a = best_flat[2];
     s->flat_post_b = best_flat[3];
     s->pos_partners = best_partners;
+    s->stage_sb_lambda = best_sb[0];
+    s->stage_sb_post_a = best_sb[1];
     (void)bp_solver_enable_kernel_timing(s, 0);
     // iteration fusion: the epilogue passes (next iteration's plane
     // collisions, the velocity/position update, the lambda pass's inputs) in
@@ -954,11 +974,11 @@ bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, cha
                       "\"iteration_fusion\": %d, \"lambda_walk\": \"%s\", \"position_walk\": \"%s\", "
                       "\"omega_normal_walk\": \"%s\", \"vorticity_tension_walk\": \"%s\", "
                       "\"lambda_stage_records\": %d, \"omega_normal_stage_records\": %d, "
-                      "\"position_partners\": %d}",
+                      "\"position_partners\": %d, \"lambda_stage_screen\": %d, \"omega_normal_stage_screen\": %d}",
                       best_blk[0], best_blk[1], best_blk[2], best_blk[3], best_blk[4], best_blk[5], s->fuse ? 1 : 0,
                       s->staged_lambda ? "staged" : "global", s->flat_position ? "flat" : "chunk",
                       s->staged_post_a ? "staged" : "global", s->flat_post_b ? "flat" : "chunk", s->staged_lambda,
-                      s->staged_post_a, s->pos_partners);
+                      s->staged_post_a, s->pos_partners, s->stage_sb_lambda, s->stage_sb_post_a);
     return BP_OK;
 }
 

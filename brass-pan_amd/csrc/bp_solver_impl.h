@@ -128,6 +128,9 @@ struct bp_solver {
         flat_post_b = 1;
     // partners the position pass fetches per round trip (2 or 4)
     int pos_partners = 2;
+    // candidates per screen batch of the staged lambda and omega/normal walks
+    // (4 or 8; bp_solver_set_stage_screen)
+    int stage_sb_lambda = 4, stage_sb_post_a = 4;
 
     // hipGraph of two substeps (the ping-pong buffers return to their start)
     bool use_graph = false;

@@ -292,6 +292,14 @@ bp_status bp_solver_set_walk(bp_solver *s, int lambda_staged, int position_flat,
  * chooses).  No counterpart in the reference. */
 bp_status bp_solver_set_partner_fetch(bp_solver *s, int position_partners);
 
+/* Candidates the staged lambda and omega/normal walks screen per batch from
+ * LDS, 4 (the default) or 8; anything else is BP_ERR_INVALID_ARG.  Four take
+ * fewer registers, so more waves fit, which pays on large workloads; eight
+ * issue more work per batch, which pays when every wave is resident anyway.
+ * Same pairs in the same order, so results are bit-identical
+ * (bp_solver_autotune chooses).  No counterpart in the reference. */
+bp_status bp_solver_set_stage_screen(bp_solver *s, int lambda_candidates, int omega_normal_candidates);
+
 /* Chooses the workgroup size of each neighbour pass among 64..512 and the
  * walk forms of the lambda, position, omega/normal and vorticity/tension
  * passes (bp_solver_set_walk, including the staged walks' region sizes) by
@@ -304,8 +312,10 @@ bp_status bp_solver_set_partner_fetch(bp_solver *s, int position_partners);
  * "position_walk"/"vorticity_tension_walk": "flat"|"chunk",
  * "lambda_stage_records"/"omega_normal_stage_records": 0|128|256|512 (the
  * staged region size, 0 for the per-lane walk), "position_partners": 2|4
- * (bp_solver_set_partner_fetch)} into `report` (may be NULL; 512 bytes hold
- * it).  No counterpart in the reference. */
+ * (bp_solver_set_partner_fetch), "lambda_stage_screen"/
+ * "omega_normal_stage_screen": 4|8 (bp_solver_set_stage_screen)} into
+ * `report` (may be NULL; 1024 bytes hold it).  No counterpart in the
+ * reference. */
 bp_status bp_solver_autotune(bp_solver *s, float dt, int substeps_per_trial, char *report, int report_len);
 
 /* Sets the neighbour passes' workgroup sizes, each 64, 128, 256 or 512, in

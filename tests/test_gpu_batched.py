@@ -111,8 +111,9 @@ def test_autotune_keeps_results():
     passes = {"fluid_lambda", "fluid_position", "omega_normal", "vorticity_tension", "xsph", "particle_particle"}
     assert set(report) == passes | {"iteration_fusion", "lambda_walk", "position_walk", "omega_normal_walk",
                                     "vorticity_tension_walk", "lambda_stage_records", "omega_normal_stage_records",
-                                    "position_partners"}
+                                    "position_partners", "lambda_stage_screen", "omega_normal_stage_screen"}
     assert report["position_partners"] in (2, 4)
+    assert report["lambda_stage_screen"] in (4, 8) and report["omega_normal_stage_screen"] in (4, 8)
     assert report["lambda_walk"] in ("staged", "global") and report["omega_normal_walk"] in ("staged", "global")
     for walk, records in (("lambda_walk", "lambda_stage_records"),
                           ("omega_normal_walk", "omega_normal_stage_records")):

@@ -387,6 +387,9 @@ def test_walk_forms_are_bit_identical(label, records):
     a.set_partner_fetch(4)
     if records == 256:
         b.set_partner_fetch(4)
+    # the staged walks screening 8 candidates per batch (4 by default)
+    if records != 128:
+        a.set_stage_screen(8, 8)
     a.step(4, 4 * DT)
     b.step(4, 4 * DT)
     pa, va, ma = a.get_state()
@@ -397,9 +400,10 @@ def test_walk_forms_are_bit_identical(label, records):
 
 
 def test_set_walk_rejects_region_sizes_not_built():
-    """bp_solver_set_walk takes 0, 1, 128, 256 or 512 for a staged pass and
-    bp_solver_set_partner_fetch 2 or 4; both return BP_ERR_INVALID_ARG for
-    anything else, leaving the solver usable."""
+    """bp_solver_set_walk takes 0, 1, 128, 256 or 512 for a staged pass,
+    bp_solver_set_partner_fetch 2 or 4 and bp_solver_set_stage_screen 4 or 8;
+    each returns BP_ERR_INVALID_ARG for anything else, leaving the solver
+    usable."""
     sc, prm = scenes.dam_break(dims=(8, 8, 8), grid=(64, 64, 64)), scenes.dam_break_params()
     case("set_walk argument check")
     s = bp.Solver(sc, prm)
@@ -411,6 +415,11 @@ def test_set_walk_rejects_region_sizes_not_built():
     for bad in (0, 1, 3, 8, -2):
         with pytest.raises(bp.BrassPanError):
             s.set_partner_fetch(bad)
+    for bad in (0, 2, 6, 16):
+        with pytest.raises(bp.BrassPanError):
+            s.set_stage_screen(bad, 4)
+        with pytest.raises(bp.BrassPanError):
+            s.set_stage_screen(8, bad)
     s.set_walk(512, False, 128, False)
     s.set_partner_fetch(4)
     s.step(1, DT)
