@@ -126,18 +126,27 @@ __device__ __forceinline__ unsigned long long to_fixed(float v) {
 //   partners j < k (not solid) found by its own search whose search contains
 //   k's cell.
 // A pair k owns whose partner j's search does not reach k's cell is delivered
-// to j through corr[j] (fixed point).  k_position_apply adds both parts.
+// to j through corr[j] (fixed point).  The new x* is
+//   out = (x* + own sum / rho0) + delivered / rho0,
+// the second term only where something was delivered.  With Direct the pass
+// writes x* + own sum / rho0 to `out` itself and sets each receiver's byte in
+// dflag (a plain store: every writer stores 1), so k_position_fixup reads one
+// byte per slot and the corrections only where they exist; otherwise it
+// writes the own sum to `out` and k_position_apply adds both parts and runs
+// the fused epilogue.
 // NB: partners fetched together per round trip (2 or 4; the pass waits on
 // these loads, and more in flight per wave costs registers).
-template <class SP, bool Flat, int NB>
+template <class SP, bool Flat, int NB, bool Direct>
 __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, Flat ? 5 : 6) void k_position(const float4 *__restrict__ xs,
                                                      const float4 *__restrict__ lam_rec,
                                                      const int *__restrict__ build_keys,
                                                      const int2 *__restrict__ ranges,
                                                      const uint32_t *__restrict__ srec,
                                                      const uint32_t *__restrict__ fluid_stamp,
-                                                     const uint32_t *__restrict__ stamp, float4 *__restrict__ sum_out,
-                                                     unsigned long long *__restrict__ corr, int n, SP sp) {
+                                                     const uint32_t *__restrict__ stamp, float4 *__restrict__ out,
+                                                     unsigned long long *__restrict__ corr,
+                                                     uint8_t *__restrict__ dflag, int n,
+                                                     SP sp) {
     const int k = pass_slot();
     if (k >= n) return;
     sp.with_scene(k, [&](const SceneConst &S) {
@@ -155,10 +164,16 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, Flat ? 5 : 6) void k_posi
         int g[3];
         unpack_cell(ck, g);
         f3 sum = mk3(0.f, 0.f, 0.f);
+        auto put = [&]() {
+            if constexpr (Direct)
+                out[k] = f4(pk + sum * F.inv_rest_density, a.w);
+            else
+                out[k] = f4(sum, 0.0f);
+        };
         // A solid's pairs all have a fluid partner: with no fluid particle in its
         // 27 cells the sum stays zero.
         if (!kf && fluid_stamp && !stamp_near(G, g, fluid_stamp, *stamp)) {
-            sum_out[k] = f4(sum, 0.0f);
+            put();
             return;
         }
         struct Partner {
@@ -192,16 +207,40 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, Flat ? 5 : 6) void k_posi
                     atomicAdd(&corr[3 * j + 0], to_fixed(ck2 * diff.x));
                     atomicAdd(&corr[3 * j + 1], to_fixed(ck2 * diff.y));
                     atomicAdd(&corr[3 * j + 2], to_fixed(ck2 * diff.z));
+                    if constexpr (Direct) dflag[j] = 1;
                 }
             },
             // partners j != k if k is fluid, only j < k if k is a solid
             [&](int base) { return kf ? ~chunk_bit(k, base) : chunk_below(k, base); }, rec);
-        sum_out[k] = f4(sum, 0.0f);
+        put();
     });
 }
 
-// Δx = (own sum + delivered pairs) / rho0, x* += Δx (particlesolver.cuh:919-932)
-// and the fused epilogue; clears corr for the next pass.
+// The delivered pairs of a direct position pass: x* += delivered / rho0 for
+// each slot whose dflag byte is set, clearing corr and dflag for the next pass.
+template <class SP>
+__global__ __launch_bounds__(kBlock) void k_position_fixup(float4 *__restrict__ xs_out,
+                                                           unsigned long long *__restrict__ corr,
+                                                           uint8_t *__restrict__ dflag, int n, SP sp) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n || dflag[k] == 0) return;
+    dflag[k] = 0;
+    const unsigned long long cx = corr[3 * k], cy = corr[3 * k + 1], cz = corr[3 * k + 2];
+    if ((cx | cy | cz) == 0ull) return;
+    corr[3 * k] = 0ull;
+    corr[3 * k + 1] = 0ull;
+    corr[3 * k + 2] = 0ull;
+    sp.with_scene_ref(k, [&](const SceneConst &S) {
+        const float inv = (float)(1.0 / kCorrScale);
+        const f3 dv = mk3((float)(long long)cx * inv, (float)(long long)cy * inv, (float)(long long)cz * inv);
+        const float4 o = xs_out[k];
+        xs_out[k] = f4(xyz(o) + dv * S.F.inv_rest_density, o.w);
+    });
+}
+
+// x* += own sum / rho0, then += delivered pairs / rho0 where any were
+// delivered (particlesolver.cuh:919-932), and the fused epilogue; clears corr
+// for the next pass.
 template <class SP>
 __global__ __launch_bounds__(kBlock) void k_position_apply(const float4 *__restrict__ xs,
                                                            const float4 *__restrict__ sum_in,
@@ -213,18 +252,16 @@ __global__ __launch_bounds__(kBlock) void k_position_apply(const float4 *__restr
     sp.with_scene_ref(k, [&](const SceneConst &S) {
         const FluidParams &F = S.F;
         const float4 a = xs[k];
-        f3 sum = xyz(sum_in[k]);
+        f3 out = xyz(a) + xyz(sum_in[k]) * F.inv_rest_density;
         const unsigned long long cx = corr[3 * k], cy = corr[3 * k + 1], cz = corr[3 * k + 2];
         if ((cx | cy | cz) != 0ull) {
             const float inv = (float)(1.0 / kCorrScale);
-            sum.x += (float)(long long)cx * inv;
-            sum.y += (float)(long long)cy * inv;
-            sum.z += (float)(long long)cz * inv;
+            const f3 dv = mk3((float)(long long)cx * inv, (float)(long long)cy * inv, (float)(long long)cz * inv);
+            out = out + dv * F.inv_rest_density;
             corr[3 * k] = 0ull;
             corr[3 * k + 1] = 0ull;
             corr[3 * k + 2] = 0ull;
         }
-        f3 out = xyz(a) + sum * F.inv_rest_density;
         if (E.planes_next || E.finish) {
             float4 x4 = x_io[k];
             const f3 xp = xyz(x4);
@@ -454,22 +491,35 @@ void launch_lambda(const float4 *xs, const float *m, const int2 *ranges, float *
 template <class SP>
 void launch_position(const float4 *xs, const float4 *lam_rec, const int *build_keys, const int2 *ranges,
                      const uint32_t *srec, const uint32_t *fluid_stamp, const uint32_t *stamp, float4 *sum_tmp,
-                     unsigned long long *corr, float4 *xs_out, float4 *x_io, float4 *v_out, int n, const SP &sp,
-                     const Epilogue &E, int block, int flat, int partners, hipStream_t st) {
+                     unsigned long long *corr, uint8_t *dflag, float4 *xs_out, float4 *x_io,
+                     float4 *v_out, int n, const SP &sp, const Epilogue &E, int block, int flat, int partners,
+                     hipStream_t st) {
     const dim3 grid(grid_blocks(n, block)), blk(block);
     const size_t lds = flat ? flat_lds_bytes(block) : 0;
-#define BP_POSITION(F, NB)                                                                                      \
-    hipLaunchKernelGGL((k_position<SP, F, NB>), grid, blk, lds, st, xs, lam_rec, build_keys, ranges, srec,    \
-                       fluid_stamp, stamp, sum_tmp, corr, n, sp)
-    switch ((flat ? 1 : 0) | (partners == 4 ? 2 : 0)) {
-    case 0: BP_POSITION(false, 2); break;
-    case 1: BP_POSITION(true, 2); break;
-    case 2: BP_POSITION(false, 4); break;
-    default: BP_POSITION(true, 4); break;
+    // no epilogue: the pass writes x* itself and a fix-up adds the deliveries
+    const bool direct = !E.planes_next && !E.finish && !E.lin.planes;
+    float4 *const out = direct ? xs_out : sum_tmp;
+#define BP_POSITION(F, NB, D)                                                                                   \
+    hipLaunchKernelGGL((k_position<SP, F, NB, D>), grid, blk, lds, st, xs, lam_rec, build_keys, ranges, srec, \
+                       fluid_stamp, stamp, out, corr, dflag, n, sp)
+    switch ((flat ? 1 : 0) | (partners == 4 ? 2 : 0) | (direct ? 4 : 0)) {
+    case 0: BP_POSITION(false, 2, false); break;
+    case 1: BP_POSITION(true, 2, false); break;
+    case 2: BP_POSITION(false, 4, false); break;
+    case 3: BP_POSITION(true, 4, false); break;
+    case 4: BP_POSITION(false, 2, true); break;
+    case 5: BP_POSITION(true, 2, true); break;
+    case 6: BP_POSITION(false, 4, true); break;
+    default: BP_POSITION(true, 4, true); break;
     }
 #undef BP_POSITION
-    hipLaunchKernelGGL(k_position_apply<SP>, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, sum_tmp, corr, xs_out,
-                       x_io, v_out, n, sp, E);
+    if (direct) {
+        hipLaunchKernelGGL(k_position_fixup<SP>, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs_out, corr, dflag, n,
+                           sp);
+    } else {
+        hipLaunchKernelGGL(k_position_apply<SP>, dim3(grid_blocks(n)), dim3(kBlock), 0, st, xs, sum_tmp, corr,
+                           xs_out, x_io, v_out, n, sp, E);
+    }
 }
 template <class SP>
 void launch_post_a(const float4 *xs, const float4 *v, const float *density, const int2 *ranges, float4 *omega,
@@ -537,8 +587,8 @@ void launch_xsph(const float4 *xs, const float4 *v, const int2 *ranges, const ui
                                     hipStream_t);                                                               \
     template void launch_position<SP>(const float4 *, const float4 *, const int *, const int2 *,                \
                                       const uint32_t *, const uint32_t *, const uint32_t *, float4 *,           \
-                                      unsigned long long *, float4 *, float4 *, float4 *, int, const SP &,      \
-                                      const Epilogue &, int, int, int, hipStream_t);                            \
+                                      unsigned long long *, uint8_t *, float4 *, float4 *, float4 *,            \
+                                      int, const SP &, const Epilogue &, int, int, int, hipStream_t);           \
     template void launch_post_a<SP>(const float4 *, const float4 *, const float *, const int2 *, float4 *,      \
                                     float4 *, float4 *, uint32_t *, uint32_t *, const LambdaIn *, int, int,     \
                                     const SP &, int, int, int, int, int, hipStream_t);                          \

@@ -39,8 +39,9 @@ void launch_lambda(const float4 *xs, const float *m, const int2 *ranges, float *
 template <class SP>
 void launch_position(const float4 *xs, const float4 *lam_rec, const int *build_keys, const int2 *ranges,
                      const uint32_t *srec, const uint32_t *fluid_stamp, const uint32_t *stamp, float4 *sum_tmp,
-                     unsigned long long *corr, float4 *xs_out, float4 *x_io, float4 *v_out, int n, const SP &sp,
-                     const Epilogue &E, int block, int flat, int partners, hipStream_t st);
+                     unsigned long long *corr, uint8_t *dflag, float4 *xs_out, float4 *x_io,
+                     float4 *v_out, int n, const SP &sp, const Epilogue &E, int block, int flat, int partners,
+                     hipStream_t st);
 template <class SP>
 void launch_post_a(const float4 *xs, const float4 *v, const float *density, const int2 *ranges, float4 *omega,
                    float4 *xom, float4 *normal, uint32_t *ccell, uint32_t *srec, const LambdaIn *one_phase,

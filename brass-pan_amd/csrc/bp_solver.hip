@@ -241,8 +241,8 @@ bp_status substep_t(bp_solver *s, float subdt, float inv_dt, f3 gstep, bool pick
                 E.m = s->m.cur();
             }
             TIMED(F_POSITION, launch_position(s->xs.cur(), s->lam_rec, sk, s->ranges, H.rec ? H.rec[it] : s->srec,
-                                              s->fluid_stamp, s->stamp, s->sum_tmp, s->corr, s->xs.alt(),
-                                              s->x.cur(), s->v.cur(), N, sp(), E, s->blk_position,
+                                              s->fluid_stamp, s->stamp, s->sum_tmp, s->corr, s->dflag,
+                                              s->xs.alt(), s->x.cur(), s->v.cur(), N, sp(), E, s->blk_position,
                                               s->flat_position, s->pos_partners, st));
             s->xs.flip();
             planes_done = E.planes_next;
@@ -572,6 +572,7 @@ bp_status create_impl(const bp_scene_desc *scs, const bp_params *params, int ns,
     }
     TRY(s->alloc(&s->sum_tmp, N));
     TRY(s->alloc(&s->corr, 3 * (size_t)N));
+    TRY(s->alloc(&s->dflag, (size_t)N));
     TRY(s->alloc(&s->omega, N));
     TRY(s->alloc(&s->xom, N));
     TRY(s->alloc(&s->normal, N));
@@ -599,6 +600,7 @@ bp_status create_impl(const bp_scene_desc *scs, const bp_params *params, int ns,
         HTRY(hipMemsetAsync(s->xom, 0, N * sizeof(float4), s->st));
         HTRY(hipMemsetAsync(s->normal, 0, N * sizeof(float4), s->st));
         HTRY(hipMemsetAsync(s->corr, 0, 3 * (size_t)N * sizeof(unsigned long long), s->st));
+        HTRY(hipMemsetAsync(s->dflag, 0, (size_t)N, s->st));
     }
     HTRY(hipMemsetAsync(s->ranges, 0xFF, (size_t)s->num_cells * sizeof(int2), s->st));
 

@@ -88,6 +88,7 @@ struct bp_solver {
     float4 *frame_buf = nullptr;  // bp_export_frame staging, allocated on first use
     float4 *sum_tmp = nullptr;
     unsigned long long *corr = nullptr;  // fixed-point pair corrections, kept zero between passes
+    uint8_t *dflag = nullptr;  // slots that received a correction, one byte each (kept zero)
     float4 *omega = nullptr, *normal = nullptr;  // normal.w: the density it was computed with
     float4 *xom = nullptr;                        // (x, |omega|) from the omega/normal pass
     uint32_t *cell_stamp = nullptr, *stamp = nullptr;  // cells near a solid (fluid + solid scenes)
