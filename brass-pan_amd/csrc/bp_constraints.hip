@@ -68,9 +68,11 @@ __global__ __launch_bounds__(kBlock) void k_mark_cells(const int *__restrict__ s
 // so it skips the walk when none of its 27 cells holds one (one cell_stamp
 // entry, k_mark_cells).  A partner's x* and x are fetched together, ahead of
 // the pair term.  (The staged walk, with both records in LDS, measured
-// slower here: 0.323 -> 0.337 ms per launch on the 512-scene batch.)
+// slower here: 0.323 -> 0.337 ms per launch on the 512-scene batch.)  Five
+// waves per SIMD: the batched kernel fits 96 VGPRs without spilling (it took
+// 101, four waves; batch 0.318 -> 0.299 ms per launch).
 template <class SP>
-__global__ __launch_bounds__(kMaxBlock) void k_particle_particle(const float4 *__restrict__ xs,
+__global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 5) void k_particle_particle(const float4 *__restrict__ xs,
                                                                  const float4 *__restrict__ x,
                                                                  const int2 *__restrict__ ranges,
                                                                  float4 *__restrict__ xs_out, int n, SP sp,
