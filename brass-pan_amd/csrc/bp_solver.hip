@@ -371,6 +371,10 @@ bp_status validate_scene(const bp_scene_desc *sc) {
     if (N < 0) return fail(BP_ERR_INVALID_ARG, "num_particles < 0");
     if (N > 0 && (!sc->positions || !sc->masses || !sc->phases))
         return fail(BP_ERR_INVALID_ARG, "positions, masses and phases are required");
+    // the lambda pass's partner record carries the mass with the partner's
+    // fluid flag in its sign, so a negative mass would not round-trip
+    for (int i = 0; i < N; ++i)
+        if (!(sc->masses[i] >= 0.0f)) return fail(BP_ERR_INVALID_ARG, "masses must be >= 0");
     for (int a = 0; a < 3; ++a) {
         if (!is_pow2(sc->grid_dims[a]) || sc->grid_dims[a] < 4 || sc->grid_dims[a] > 1024)
             return fail(BP_ERR_INVALID_ARG, "grid_dims must be powers of two in [4, 1024]");

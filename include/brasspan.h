@@ -84,7 +84,8 @@ typedef struct bp_scene_desc {
     int          num_particles;
     const float *positions;        /* [3N] scene.h:507                          */
     const float *velocities;       /* [3N] or NULL = zero (particlesolver.cuh:167) */
-    const float *masses;           /* [N]  scene.h:508                          */
+    const float *masses;           /* [N]  scene.h:508; each >= 0 (else       */
+                                   /*      BP_ERR_INVALID_ARG)                  */
     const int   *phases;           /* [N]  scene.h:509: fluid < 0, solid > 0    */
     const int   *group_ids;        /* [N] or NULL (render-only)                 */
     float        particle_radius;  /* solidRadius, scene.h:500                  */

@@ -132,3 +132,18 @@ def test_create_rejects_null_and_overlapping_rigid_ranges():
     arr.rranges[:] = arr.rranges[::-1].copy()  # disjoint but listed out of order: accepted
     st, msg = _create_desc_status(arr)
     assert st != 1, msg
+
+
+def test_create_rejects_negative_masses():
+    """A partner's mass travels in the lambda pass's record with the partner's
+    fluid flag in its sign, so every mass must be >= 0 (and not NaN); the
+    check runs before any device call (BP_ERR_INVALID_ARG)."""
+    for bad in (-1.0, float("nan")):
+        arr = bp.SceneArrays(small_scene(grid_dims=(64, 64, 64)))
+        arr.masses[3] = bad
+        st, msg = _create_desc_status(arr)
+        assert st == 1 and "masses" in msg, (bad, st, msg)
+    arr = bp.SceneArrays(small_scene(grid_dims=(64, 64, 64)))
+    arr.masses[3] = 0.0  # a massless particle is accepted
+    st, msg = _create_desc_status(arr)
+    assert st != 1, msg
