@@ -86,9 +86,7 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 5) void k_particle_partic
         const GridDesc &G = S.G;
         const float r = S.radius, fs = S.friction_static, fd = S.friction_dynamic;
         const float4 a = xs[i];
-        const float4 xa = x[i];
-        const f3 xi = xyz(xa), xip = xyz(a);
-        const float wi = xa.w;
+        const f3 xip = xyz(a);
         const int pi = phase_of(a);
         f3 sumD = mk3(0.f, 0.f, 0.f), sumF = mk3(0.f, 0.f, 0.f);
         int cnt = 0;
@@ -99,6 +97,10 @@ __global__ __launch_bounds__(kMaxBlock) kPassWaves(SP, 5) void k_particle_partic
             if (L.planes) store_lambda_in(L, i, xip, m[i], pi);
             return;
         }
+        // (x, w) only for the particles that walk
+        const float4 xa = x[i];
+        const f3 xi = xyz(xa);
+        const float wi = xa.w;
         const float r2x4 = r * r * 4.0f;
         const float thr = r * r * 0.001f * 0.001f;
         auto pick = [&](int, float4 b) {  // pi != pj also excludes i itself
